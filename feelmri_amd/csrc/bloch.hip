@@ -1,0 +1,322 @@
+// Per-node Bloch integrator (Cayley-Klein rotation, then T1/T2 relaxation).
+//
+// reference: cpp/feelmri/BlochSimulator.cpp:231-357 (static nodes) and
+// :92-228 (moving nodes), solve_mri<T>.
+//
+// One lane per node; node attributes come from an SoA table (coalesced).
+// The event table is walked in chunks of 256 events: each lane of the
+// workgroup stages one event (rf, |rf|^2, -gamma dt / 2, G, dt and, for
+// moving nodes, the POD weights of that step) into LDS, then every lane
+// integrates its node through the chunk reading the events by broadcast.
+// Step i -> i+1 uses event i+1; e1/e2 are refreshed only when dt changes;
+// |B| is floored at 1e-12; rotation first, then relaxation toward M0.
+#include "fm_common.h"
+
+namespace fm {
+
+constexpr int kBlochThreads = 256;
+constexpr int kBlochChunk = 256;
+constexpr int kBlochMaxModes = 16;
+
+template <typename T>
+struct alignas(16) Event {
+    T rf_re, rf_im, rf2, kappa;
+    T gx, gy, gz, dt;
+};
+
+template <typename T>
+__device__ inline void dev_sincos(T x, T *s, T *c);
+template <>
+__device__ inline void dev_sincos<double>(double x, double *s, double *c) { sincos(x, s, c); }
+template <>
+__device__ inline void dev_sincos<float>(float x, float *s, float *c) { sincosf(x, s, c); }
+
+template <typename T>
+__device__ inline T dev_exp(T x);
+template <>
+__device__ inline double dev_exp<double>(double x) { return exp(x); }
+template <>
+__device__ inline float dev_exp<float>(float x) { return expf(x); }
+
+template <typename T>
+__device__ inline T dev_sqrt(T x);
+template <>
+__device__ inline double dev_sqrt<double>(double x) { return sqrt(x); }
+template <>
+__device__ inline float dev_sqrt<float>(float x) { return sqrtf(x); }
+
+template <typename T>
+struct Cplx2;
+template <>
+struct Cplx2<double> {
+    using type = double2;
+};
+template <>
+struct Cplx2<float> {
+    using type = float2;
+};
+
+template <typename T, bool MOTION, bool HISTORY>
+__global__ __launch_bounds__(kBlochThreads) void k_bloch(
+    const double *__restrict__ packed, int64_t N, int64_t n_time, double M0d, double gammad,
+    const double *__restrict__ rf, const double *__restrict__ G, const double *__restrict__ dt,
+    const double *mxy_in, const double *mz_in,  // may alias mxy_fin / mz_fin
+    const float *__restrict__ modes, int R, const double *__restrict__ mcoef,
+    typename Cplx2<T>::type *__restrict__ mxy_out, T *__restrict__ mz_out,
+    double2 *mxy_fin, double *mz_fin)
+{
+    using C2 = typename Cplx2<T>::type;
+    __shared__ Event<T> sev[kBlochChunk];
+    __shared__ T scoef[MOTION ? kBlochChunk * (kBlochMaxModes + 3) : 1];
+
+    const int tid = threadIdx.x;
+    const int64_t p = (int64_t)blockIdx.x * kBlochThreads + tid;
+    const bool active = p < N;
+    const T M0 = (T)M0d, gamma = (T)gammad;
+
+    T x = 0, y = 0, z = 0, invT1 = 0, invT2 = 0, dB = 0;
+    T mxr = 0, mxi = 0, mz = 0;
+    float md[MOTION ? 3 * kBlochMaxModes : 1];
+    if (active) {
+        x = (T)packed[p];
+        y = (T)packed[N + p];
+        z = (T)packed[2 * N + p];
+        invT1 = (T)packed[3 * N + p];
+        invT2 = (T)packed[4 * N + p];
+        dB = (T)packed[5 * N + p];
+        mxr = (T)mxy_in[2 * p];
+        mxi = (T)mxy_in[2 * p + 1];
+        mz = (T)mz_in[p];
+        if (MOTION) {
+#pragma unroll
+            for (int r = 0; r < kBlochMaxModes; ++r) {
+                if (r < R) {
+                    md[r] = modes[(int64_t)r * N + p];
+                    md[kBlochMaxModes + r] = modes[(int64_t)(R + r) * N + p];
+                    md[2 * kBlochMaxModes + r] = modes[(int64_t)(2 * R + r) * N + p];
+                }
+            }
+        }
+        if (HISTORY) {
+            C2 o;
+            o.x = mxr;
+            o.y = mxi;
+            mxy_out[p] = o;
+            mz_out[p] = mz;
+        }
+    }
+    T e1 = 0, e2 = 0, dt_prev = (T)-1;
+    const int64_t n_steps = n_time > 0 ? n_time - 1 : 0;
+    const int RC = R + 3;
+
+    for (int64_t base = 0; base < n_steps; base += kBlochChunk) {
+        __syncthreads();
+        {
+            const int64_t i = base + tid;  // step i uses event i + 1
+            Event<T> ev;
+            if (i < n_steps) {
+                const double rr = rf[2 * (i + 1)], ri = rf[2 * (i + 1) + 1];
+                ev.rf_re = (T)rr;
+                ev.rf_im = (T)ri;
+                ev.rf2 = ev.rf_re * ev.rf_re + ev.rf_im * ev.rf_im;
+                ev.dt = (T)dt[i + 1];
+                ev.kappa = -(T)0.5 * gamma * ev.dt;
+                ev.gx = (T)G[3 * (i + 1)];
+                ev.gy = (T)G[3 * (i + 1) + 1];
+                ev.gz = (T)G[3 * (i + 1) + 2];
+            } else {
+                ev.rf_re = ev.rf_im = ev.rf2 = ev.kappa = ev.gx = ev.gy = ev.gz = ev.dt = 0;
+            }
+            sev[tid] = ev;
+            if (MOTION) {
+                for (int q = 0; q < RC; ++q)
+                    scoef[tid * RC + q] = (i < n_steps) ? (T)mcoef[i * RC + q] : (T)0;
+            }
+        }
+        __syncthreads();
+        const int cnt = (n_steps - base < kBlochChunk) ? (int)(n_steps - base) : kBlochChunk;
+        if (active) {
+            for (int e = 0; e < cnt; ++e) {
+                const Event<T> ev = sev[e];
+                if (ev.dt != dt_prev) {  // BlochSimulator.cpp:287-294
+                    e1 = dev_exp<T>(-ev.dt * invT1);
+                    e2 = dev_exp<T>(-ev.dt * invT2);
+                    dt_prev = ev.dt;
+                }
+                T px = x, py = y, pz = z;
+                if (MOTION) {
+                    const T *cf = &scoef[e * RC];
+                    T dx = cf[R], dy = cf[R + 1], dz = cf[R + 2];
+#pragma unroll
+                    for (int r = 0; r < kBlochMaxModes; ++r) {
+                        if (r < R) {
+                            dx += (T)md[r] * cf[r];
+                            dy += (T)md[kBlochMaxModes + r] * cf[r];
+                            dz += (T)md[2 * kBlochMaxModes + r] * cf[r];
+                        }
+                    }
+                    px += dx;
+                    py += dy;
+                    pz += dz;
+                }
+                const T Bz = px * ev.gx + py * ev.gy + pz * ev.gz + dB;
+                T Bn = dev_sqrt<T>(Bz * Bz + ev.rf2);
+                if (Bn < (T)1e-12) Bn = (T)1e-12;
+                const T inv = (T)1 / Bn;
+                const T nz = Bz * inv;
+                T s, c;
+                dev_sincos<T>(Bn * ev.kappa, &s, &c);
+                // alpha = c - i nz s ; beta = -i (rf/|B|) s
+                const T ar = c, ai = -nz * s;
+                const T br = ev.rf_im * inv * s, bi = -ev.rf_re * inv * s;
+                const T a2 = ar * ar + ai * ai, b2 = br * br + bi * bi;
+                const T car = ar, cai = -ai;
+                const T cab_r = car * br - cai * bi, cab_i = car * bi + cai * br;
+                const T ca2_r = car * car - cai * cai, ca2_i = (T)2 * car * cai;
+                const T bb_r = br * br - bi * bi, bb_i = (T)2 * br * bi;
+                const T cmr = mxr, cmi = -mxi;
+                const T nmr = (T)2 * cab_r * mz + (ca2_r * mxr - ca2_i * mxi) - (bb_r * cmr - bb_i * cmi);
+                const T nmi = (T)2 * cab_i * mz + (ca2_r * mxi + ca2_i * mxr) - (bb_r * cmi + bb_i * cmr);
+                const T ab_r = ar * br - ai * bi, ab_i = ar * bi + ai * br;
+                const T nmz = (a2 - b2) * mz - (T)2 * (ab_r * cmr - ab_i * cmi);
+                mxr = nmr * e2;
+                mxi = nmi * e2;
+                mz = nmz * e1 + ((T)1 - e1) * M0;
+                if (HISTORY) {
+                    const int64_t o = p + (base + e + 1) * N;
+                    C2 v;
+                    v.x = mxr;
+                    v.y = mxi;
+                    mxy_out[o] = v;
+                    mz_out[o] = mz;
+                }
+            }
+        }
+    }
+    if (!HISTORY && active) {
+        mxy_fin[p] = make_double2((double)mxr, (double)mxi);
+        mz_fin[p] = (double)mz;
+    }
+}
+
+template <typename T>
+__global__ void k_bloch_pack(int64_t N, const T *__restrict__ r0, const T *__restrict__ T1,
+                             const T *__restrict__ T2, const T *__restrict__ dB, double *__restrict__ out)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    out[n] = (double)r0[3 * n];
+    out[N + n] = (double)r0[3 * n + 1];
+    out[2 * N + n] = (double)r0[3 * n + 2];
+    out[3 * N + n] = 1.0 / (double)T1[n];
+    out[4 * N + n] = 1.0 / (double)T2[n];
+    out[5 * N + n] = (double)dB[n];
+}
+
+__global__ void k_store_spoil(int64_t N, double2 *__restrict__ mxy, const double *__restrict__ mz,
+                              float2 *__restrict__ mxy_store, float *__restrict__ mz_store, int64_t col,
+                              int spoil)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const double2 v = mxy[n];
+    if (col >= 0) {
+        mxy_store[n + col * N] = make_float2((float)v.x, (float)v.y);
+        mz_store[n + col * N] = (float)mz[n];
+    }
+    if (spoil) mxy[n] = make_double2(0.0, 0.0);
+}
+
+template <typename T>
+static int launch_bloch(bool motion, bool history, int64_t N, int64_t n_time, const double *packed,
+                        double M0, double gamma, const double *rf, const double *G, const double *dt,
+                        const double *mxy_in, const double *mz_in, const float *modes, int R,
+                        const double *mcoef, void *mxy_out, void *mz_out, hipStream_t st)
+{
+    using C2 = typename Cplx2<T>::type;
+    const unsigned blocks = (unsigned)cdiv(N, kBlochThreads);
+    C2 *mo = reinterpret_cast<C2 *>(mxy_out);
+    T *zo = reinterpret_cast<T *>(mz_out);
+    double2 *mf = reinterpret_cast<double2 *>(mxy_out);
+    double *zf = reinterpret_cast<double *>(mz_out);
+#define FM_BLOCH_ARGS packed, N, n_time, M0, gamma, rf, G, dt, mxy_in, mz_in, modes, R, mcoef, mo, zo, mf, zf
+    if (motion) {
+        if (history)
+            k_bloch<T, true, true><<<blocks, kBlochThreads, 0, st>>>(FM_BLOCH_ARGS);
+        else
+            k_bloch<T, true, false><<<blocks, kBlochThreads, 0, st>>>(FM_BLOCH_ARGS);
+    } else {
+        if (history)
+            k_bloch<T, false, true><<<blocks, kBlochThreads, 0, st>>>(FM_BLOCH_ARGS);
+        else
+            k_bloch<T, false, false><<<blocks, kBlochThreads, 0, st>>>(FM_BLOCH_ARGS);
+    }
+#undef FM_BLOCH_ARGS
+    FM_LAUNCH_CHECK("k_bloch");
+    return FM_OK;
+}
+
+}  // namespace fm
+
+using namespace fm;
+
+extern "C" int fm_bloch_nodes_pack(int64_t n_pos, int dtype, const void *r0, const void *T1,
+                                   const void *T2, const void *delta_B, double *packed, void *stream)
+{
+    FM_CHECK_ARG(n_pos >= 0, "negative n_pos");
+    FM_CHECK_ARG(dtype == FM_DTYPE_F32 || dtype == FM_DTYPE_F64, "unknown dtype %d", dtype);
+    if (n_pos == 0) return FM_OK;
+    FM_CHECK_ARG(r0 && T1 && T2 && delta_B && packed, "NULL array");
+    const unsigned blocks = (unsigned)cdiv(n_pos, 256);
+    hipStream_t st = as_stream(stream);
+    if (dtype == FM_DTYPE_F64)
+        k_bloch_pack<double><<<blocks, 256, 0, st>>>(n_pos, (const double *)r0, (const double *)T1,
+                                                    (const double *)T2, (const double *)delta_B, packed);
+    else
+        k_bloch_pack<float><<<blocks, 256, 0, st>>>(n_pos, (const float *)r0, (const float *)T1,
+                                                   (const float *)T2, (const float *)delta_B, packed);
+    FM_LAUNCH_CHECK("k_bloch_pack");
+    return FM_OK;
+}
+
+extern "C" int fm_bloch_execute(int precision, int64_t n_pos, int64_t n_time, const double *packed,
+                                double M0, double gamma, const double *rf, const double *G,
+                                const double *dt, const double *mxy_in, const double *mz_in,
+                                const float *modes, int n_modes, const double *motion_coef,
+                                void *mxy_out, void *mz_out, int history, void *stream)
+{
+    FM_CHECK_ARG(precision == FM_PREC_F64 || precision == FM_PREC_F32,
+                 "Bloch precision must be FM_PREC_F64 or FM_PREC_F32 (got %d)", precision);
+    FM_CHECK_ARG(n_pos >= 0 && n_time >= 0, "negative size");
+    FM_CHECK_ARG(n_modes >= 0 && n_modes <= kBlochMaxModes, "n_modes must be in [0, %d]", kBlochMaxModes);
+    if (n_pos == 0) return FM_OK;
+    FM_CHECK_ARG(packed && mxy_in && mz_in && mxy_out && mz_out, "NULL array");
+    FM_CHECK_ARG(n_time == 0 || (rf && G && dt), "NULL event table");
+    const bool motion = modes != nullptr && n_modes > 0;
+    FM_CHECK_ARG(!motion || motion_coef != nullptr, "motion_coef is NULL");
+    if (n_time == 0) {
+        set_error("empty event table (n_time = 0)");
+        return FM_ERR_ARG;
+    }
+    hipStream_t st = as_stream(stream);
+    if (precision == FM_PREC_F64)
+        return launch_bloch<double>(motion, history != 0, n_pos, n_time, packed, M0, gamma, rf, G, dt, mxy_in,
+                                    mz_in, modes, n_modes, motion_coef, mxy_out, mz_out, st);
+    return launch_bloch<float>(motion, history != 0, n_pos, n_time, packed, M0, gamma, rf, G, dt, mxy_in,
+                               mz_in, modes, n_modes, motion_coef, mxy_out, mz_out, st);
+}
+
+extern "C" int fm_bloch_store_spoil(int64_t n_pos, double *mxy_state, const double *mz_state,
+                                    void *mxy_store, void *mz_store, int64_t store_col, int spoil,
+                                    void *stream)
+{
+    FM_CHECK_ARG(n_pos >= 0, "negative n_pos");
+    if (n_pos == 0) return FM_OK;
+    FM_CHECK_ARG(mxy_state && mz_state, "NULL state");
+    FM_CHECK_ARG(store_col < 0 || (mxy_store && mz_store), "NULL store");
+    k_store_spoil<<<(unsigned)cdiv(n_pos, 256), 256, 0, as_stream(stream)>>>(
+        n_pos, reinterpret_cast<double2 *>(mxy_state), mz_state, reinterpret_cast<float2 *>(mxy_store),
+        reinterpret_cast<float *>(mz_store), store_col, spoil);
+    FM_LAUNCH_CHECK("k_store_spoil");
+    return FM_OK;
+}

@@ -1,0 +1,58 @@
+// Shared helpers for the libfeelmri_hip kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/feelmri_hip.h"
+
+namespace fm {
+
+// Per-thread message behind fm_last_error().
+void set_error(const char *fmt, ...);
+
+#define FM_CHECK_ARG(cond, ...)          \
+    do {                                 \
+        if (!(cond)) {                   \
+            ::fm::set_error(__VA_ARGS__); \
+            return FM_ERR_ARG;           \
+        }                                \
+    } while (0)
+
+#define FM_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t fm_e_ = (call);                                                       \
+        if (fm_e_ != hipSuccess) {                                                       \
+            ::fm::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(fm_e_),   \
+                            __FILE__, __LINE__);                                         \
+            return FM_ERR_HIP;                                                           \
+        }                                                                                \
+    } while (0)
+
+// Launch-status check after a <<<>>> launch.
+#define FM_LAUNCH_CHECK(name)                                                           \
+    do {                                                                                \
+        hipError_t fm_e_ = hipGetLastError();                                           \
+        if (fm_e_ != hipSuccess) {                                                      \
+            ::fm::set_error("launch of %s failed: %s", name, hipGetErrorString(fm_e_)); \
+            return FM_ERR_HIP;                                                          \
+        }                                                                               \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// (double)(float)pi / pi: the reference forms 2*pi from a float constant
+// (cpp/feelmri/MRI.h:13), so every k.x phase carries this factor.
+constexpr double kPiFOverPi = static_cast<double>(static_cast<float>(3.14159265358979323846)) /
+                              3.14159265358979323846;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// Number of CUs on the current device (cached per device).
+int device_cu_count();
+
+}  // namespace fm

@@ -1,0 +1,595 @@
+// Direct non-uniform k-space sum (any trajectory), FP64 and mixed forms.
+//
+//   S[s, v] = sum_n exp(i(-2 pi_f k_s . r_n + phi_n t_s)) exp(-t_s / T2_n) b[n, v]
+//
+// reference: cpp/feelmri/MRI.cpp:279-381 (static nodes), :121-242 (moving).
+//
+// Work decomposition.  Samples of equal t are grouped (the reference's t
+// cache) and cut into tiles of up to 512 samples; a workgroup owns one tile
+// (2 samples per lane) and one contiguous split of the nodes.  Nodes stream
+// through LDS in chunks of 256: each lane of the workgroup stages one node,
+// folding the t-dependent factors (T2* decay, off-resonance, and the POD
+// position update for moving nodes) into the staged coefficient, so the
+// inner loop per (node, sample) is just the k.x phase, one sincos and a
+// complex MAC.  All lanes read the same LDS node at the same time
+// (broadcast).  Node splits write partial sums that a second kernel adds in
+// a fixed order, so results do not depend on scheduling.
+//
+// Phase handling.  u = k'.x is accumulated in FP64 in cycles (k' carries the
+// reference's float-pi factor) and range-reduced to r = u - rint(u) in
+// [-1/2, 1/2].  FP64 form: sincospi(2r).  Mixed form: r rounded to FP32 and
+// fed to the hardware v_sin_f32 / v_cos_f32, which take revolutions;
+// partial sums over a 256-node chunk are kept in FP32 and added into FP64.
+#include <cstring>
+#include <type_traits>
+#include <unordered_map>
+
+#include "signal_plan.h"
+
+namespace fm {
+
+template <bool MIXED>
+struct CoefT;
+template <>
+struct CoefT<true> {
+    using T = float;
+};
+template <>
+struct CoefT<false> {
+    using T = double;
+};
+
+template <int VB, bool MIXED, bool SHARED_T>
+struct alignas(16) StagedNode {
+    using CT = typename CoefT<MIXED>::T;
+    double x, y, z;
+    double dwc;                  // phi/(2 pi), cycles/ms   (general kernel only)
+    double r2;                   // 1/T2, 1/ms              (general kernel only)
+    CT c[2 * VB];
+};
+// The shared-t kernel does not need dwc / r2; keep one layout for simplicity
+// of the loaders (LDS is not the constraint here).
+
+template <int VB, bool MIXED, bool SHARED_T, bool MOTION>
+__global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
+    const double *__restrict__ packed, int64_t N, int V, int v0,
+    const double4 *__restrict__ ks, const int32_t *__restrict__ perm,
+    const SignalTile *__restrict__ tiles, const double *__restrict__ tg_t,
+    const float *__restrict__ modes, int R, const double *__restrict__ mcoef,
+    int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S, int write_final)
+{
+    using CT = typename CoefT<MIXED>::T;
+    using Node = StagedNode<VB, MIXED, SHARED_T>;
+    __shared__ Node snode[kDirectThreads];
+    __shared__ double scoef[kMaxModes + 3];
+
+    const SignalTile tile = tiles[blockIdx.x];
+    const int64_t split = blockIdx.y;
+    const int64_t n_begin = split * nodes_per_split;
+    const int64_t n_end = (n_begin + nodes_per_split < N) ? n_begin + nodes_per_split : N;
+    const int tid = threadIdx.x;
+
+    const double *px = packed;
+    const double *py = packed + N;
+    const double *pz = packed + 2 * N;
+    const double *pphi = packed + 3 * N;
+    const double *pr2 = packed + 4 * N;
+    const double *pb = packed + 5 * N;  // rows 2v, 2v+1
+
+    double tg = 0.0;
+    if (SHARED_T) tg = tg_t[tile.group];
+    if (MOTION) {
+        for (int q = tid; q < R + 3; q += kDirectThreads) scoef[q] = mcoef[(int64_t)tile.group * (R + 3) + q];
+    }
+
+    int sid[kDirectSPL];
+    bool valid[kDirectSPL];
+    double kxs[kDirectSPL], kys[kDirectSPL], kzs[kDirectSPL], ts[kDirectSPL];
+#pragma unroll
+    for (int j = 0; j < kDirectSPL; ++j) {
+        const int idx = tid + j * kDirectThreads;
+        valid[j] = idx < tile.count;
+        sid[j] = valid[j] ? perm[tile.offset + idx] : 0;
+        const double4 k = ks[sid[j]];
+        kxs[j] = k.x;
+        kys[j] = k.y;
+        kzs[j] = k.z;
+        ts[j] = k.w;
+    }
+
+    double tot_re[kDirectSPL][VB], tot_im[kDirectSPL][VB];
+#pragma unroll
+    for (int j = 0; j < kDirectSPL; ++j)
+#pragma unroll
+        for (int v = 0; v < VB; ++v) tot_re[j][v] = tot_im[j][v] = 0.0;
+
+    for (int64_t base = n_begin; base < n_end; base += kDirectThreads) {
+        __syncthreads();  // previous chunk fully consumed (and scoef visible)
+        {
+            const int64_t n = base + tid;
+            Node nd;
+            if (n < n_end) {
+                double x = px[n], y = py[n], z = pz[n];
+                if (MOTION) {
+                    double dx = scoef[R], dy = scoef[R + 1], dz = scoef[R + 2];
+                    for (int r = 0; r < R; ++r) {
+                        const double w = scoef[r];
+                        dx += (double)modes[(int64_t)(0 * R + r) * N + n] * w;
+                        dy += (double)modes[(int64_t)(1 * R + r) * N + n] * w;
+                        dz += (double)modes[(int64_t)(2 * R + r) * N + n] * w;
+                    }
+                    x += dx;
+                    y += dy;
+                    z += dz;
+                }
+                nd.x = x;
+                nd.y = y;
+                nd.z = z;
+                const double phi = pphi[n], r2 = pr2[n];
+                nd.dwc = phi * (1.0 / kTwoPi);
+                nd.r2 = r2;
+                if (SHARED_T) {
+                    // c = b exp(-t/T2) exp(i phi t)   (MRI.cpp:347-354)
+                    const double d = exp(-tg * r2);
+                    double so, co;
+                    sincos(phi * tg, &so, &co);
+                    const double fr = d * co, fi = d * so;
+#pragma unroll
+                    for (int v = 0; v < VB; ++v) {
+                        const double br = pb[(int64_t)(2 * (v0 + v)) * N + n];
+                        const double bi = pb[(int64_t)(2 * (v0 + v) + 1) * N + n];
+                        nd.c[2 * v] = (CT)(br * fr - bi * fi);
+                        nd.c[2 * v + 1] = (CT)(br * fi + bi * fr);
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VB; ++v) {
+                        nd.c[2 * v] = (CT)pb[(int64_t)(2 * (v0 + v)) * N + n];
+                        nd.c[2 * v + 1] = (CT)pb[(int64_t)(2 * (v0 + v) + 1) * N + n];
+                    }
+                }
+            } else {
+                nd.x = nd.y = nd.z = 0.0;
+                nd.dwc = nd.r2 = 0.0;
+#pragma unroll
+                for (int v = 0; v < 2 * VB; ++v) nd.c[v] = (CT)0;
+            }
+            snode[tid] = nd;
+        }
+        __syncthreads();
+        const int cnt = (n_end - base < kDirectThreads) ? (int)(n_end - base) : kDirectThreads;
+
+        CT acc_re[kDirectSPL][VB], acc_im[kDirectSPL][VB];
+#pragma unroll
+        for (int j = 0; j < kDirectSPL; ++j)
+#pragma unroll
+            for (int v = 0; v < VB; ++v) acc_re[j][v] = acc_im[j][v] = (CT)0;
+
+#pragma unroll 2
+        for (int m = 0; m < cnt; ++m) {
+            const Node &nd = snode[m];
+#pragma unroll
+            for (int j = 0; j < kDirectSPL; ++j) {
+                double u = fma(kxs[j], nd.x, fma(kys[j], nd.y, kzs[j] * nd.z));
+                if (!SHARED_T) u = fma(-nd.dwc, ts[j], u);  // phase - phi t, in cycles
+                const double rr = u - __builtin_rint(u);
+                CT cs, sn;
+                if (MIXED) {
+                    const float rf = (float)rr;
+                    sn = __builtin_amdgcn_sinf(rf);
+                    cs = __builtin_amdgcn_cosf(rf);
+                } else {
+                    double s_, c_;
+                    sincospi(2.0 * rr, &s_, &c_);
+                    sn = (CT)s_;
+                    cs = (CT)c_;
+                }
+                if (!SHARED_T) {
+                    // exp(-t/T2) per (node, sample)
+                    CT d;
+                    if (MIXED) {
+                        d = __builtin_amdgcn_exp2f((float)(-ts[j] * nd.r2 * 1.4426950408889634));
+                    } else {
+                        d = (CT)exp(-ts[j] * nd.r2);
+                    }
+                    cs *= d;
+                    sn *= d;
+                }
+                // e = cs - i sn ; acc += e * c
+#pragma unroll
+                for (int v = 0; v < VB; ++v) {
+                    const CT cr = nd.c[2 * v], ci = nd.c[2 * v + 1];
+                    acc_re[j][v] = fma(cs, cr, fma(sn, ci, acc_re[j][v]));
+                    acc_im[j][v] = fma(cs, ci, fma(-sn, cr, acc_im[j][v]));
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kDirectSPL; ++j)
+#pragma unroll
+            for (int v = 0; v < VB; ++v) {
+                tot_re[j][v] += (double)acc_re[j][v];
+                tot_im[j][v] += (double)acc_im[j][v];
+            }
+    }
+
+#pragma unroll
+    for (int j = 0; j < kDirectSPL; ++j) {
+        if (!valid[j]) continue;
+        const int64_t row = write_final ? (int64_t)sid[j] : split * S + sid[j];
+#pragma unroll
+        for (int v = 0; v < VB; ++v) dst[row * V + v0 + v] = make_double2(tot_re[j][v], tot_im[j][v]);
+    }
+}
+
+// out[s, v] = sum over splits, in split order.
+__global__ void k_reduce_splits(const double2 *__restrict__ part, int64_t n_splits, int64_t SV,
+                                double2 *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= SV) return;
+    double re = 0.0, im = 0.0;
+    for (int64_t s = 0; s < n_splits; ++s) {
+        const double2 p = part[s * SV + i];
+        re += p.x;
+        im += p.y;
+    }
+    out[i] = make_double2(re, im);
+}
+
+template <typename T>
+__global__ void k_nodes_pack(int64_t N, int V, const T *__restrict__ r0, const T *__restrict__ phi,
+                             const T *__restrict__ T2, const T *__restrict__ mxy,
+                             const double *__restrict__ w, double *__restrict__ out)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    out[n] = (double)r0[3 * n];
+    out[N + n] = (double)r0[3 * n + 1];
+    out[2 * N + n] = (double)r0[3 * n + 2];
+    out[3 * N + n] = (double)phi[n];
+    out[4 * N + n] = 1.0 / (double)T2[n];
+    const double ww = w ? w[n] : 1.0;
+    for (int v = 0; v < V; ++v) {
+        out[(int64_t)(5 + 2 * v) * N + n] = ww * (double)mxy[2 * (n * V + v)];
+        out[(int64_t)(6 + 2 * v) * N + n] = ww * (double)mxy[2 * (n * V + v) + 1];
+    }
+}
+
+// CSR SpMV, one thread per (row, encoding); the row's terms are added in
+// column-index order.
+__global__ void k_csr_spmv_c128(int64_t n_rows, int V, const int64_t *__restrict__ indptr,
+                                const int64_t *__restrict__ indices, const double *__restrict__ data,
+                                const double2 *__restrict__ x, double2 *__restrict__ y)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_rows * V) return;
+    const int64_t r = i / V, v = i % V;
+    double re = 0.0, im = 0.0;
+    for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+        const double a = data[k];
+        const double2 xv = x[indices[k] * V + v];
+        re += a * xv.x;
+        im += a * xv.y;
+    }
+    y[i] = make_double2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+struct DirectLaunch {
+    int64_t n_splits;
+    int64_t nodes_per_split;
+};
+
+static DirectLaunch direct_launch_shape(const fm_signal_plan *p, int64_t N)
+{
+    // Aim for ~8 workgroups per CU over the whole grid; keep >= 2048 nodes
+    // per split so the staging cost stays amortised.
+    const int64_t target = 8 * (int64_t)device_cu_count();
+    int64_t splits = cdiv(target, p->n_tiles > 0 ? p->n_tiles : 1);
+    const int64_t max_splits = cdiv(N, 2048);
+    if (splits > max_splits) splits = max_splits;
+    if (splits < 1) splits = 1;
+    int64_t nps = cdiv(N, splits);
+    nps = cdiv(nps, kDirectThreads) * kDirectThreads;
+    splits = cdiv(N, nps);
+    return {splits, nps};
+}
+
+template <int VB, bool MIXED>
+static void launch_direct_vb(const fm_signal_plan *p, bool motion, dim3 grid, const double *packed,
+                             int64_t N, int V, int v0, const float *modes, int R,
+                             const double *mcoef, int64_t nps, double2 *dst, int write_final,
+                             hipStream_t st)
+{
+    if (p->shared_t) {
+        if (motion)
+            k_signal_direct<VB, MIXED, true, true><<<grid, kDirectThreads, 0, st>>>(
+                packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles, p->d_tg_t, modes, R, mcoef, nps,
+                dst, p->n_samples, write_final);
+        else
+            k_signal_direct<VB, MIXED, true, false><<<grid, kDirectThreads, 0, st>>>(
+                packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles, p->d_tg_t, modes, R, mcoef, nps,
+                dst, p->n_samples, write_final);
+    } else {
+        k_signal_direct<VB, MIXED, false, false><<<grid, kDirectThreads, 0, st>>>(
+            packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles, p->d_tg_t, modes, R, mcoef, nps, dst,
+            p->n_samples, write_final);
+    }
+}
+
+template <bool MIXED>
+static void launch_direct(const fm_signal_plan *p, bool motion, dim3 grid, const double *packed,
+                          int64_t N, int V, int v0, int vb, const float *modes, int R,
+                          const double *mcoef, int64_t nps, double2 *dst, int write_final,
+                          hipStream_t st)
+{
+    switch (vb) {
+    case 1: launch_direct_vb<1, MIXED>(p, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st); break;
+    case 2: launch_direct_vb<2, MIXED>(p, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st); break;
+    case 3: launch_direct_vb<3, MIXED>(p, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st); break;
+    default: launch_direct_vb<4, MIXED>(p, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st); break;
+    }
+}
+
+static size_t direct_workspace(const fm_signal_plan *p, int64_t N, int V)
+{
+    if (N <= 0 || p->n_samples == 0) return 0;
+    const DirectLaunch L = direct_launch_shape(p, N);
+    if (L.n_splits <= 1) return 0;
+    return (size_t)L.n_splits * (size_t)p->n_samples * (size_t)V * sizeof(double2);
+}
+
+static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int V,
+                          const double *packed, const float *modes, int R, const double *mcoef,
+                          void *ws, size_t ws_bytes, double *out, hipStream_t st)
+{
+    const int64_t S = p->n_samples;
+    if (S == 0) return FM_OK;
+    if (N == 0) {
+        FM_HIP(hipMemsetAsync(out, 0, (size_t)S * V * sizeof(double2), st));
+        return FM_OK;
+    }
+    const bool motion = modes != nullptr && R > 0;
+    if (motion && !p->shared_t) {
+        set_error("moving nodes need samples grouped by time (plan has no shared-t tiling)");
+        return FM_ERR_UNSUPPORTED;
+    }
+    const DirectLaunch L = direct_launch_shape(p, N);
+    const size_t need = direct_workspace(p, N, V);
+    if (need > ws_bytes) {
+        set_error("signal workspace too small: need %zu bytes, got %zu", need, ws_bytes);
+        return FM_ERR_WORKSPACE;
+    }
+    double2 *dst = L.n_splits > 1 ? reinterpret_cast<double2 *>(ws) : reinterpret_cast<double2 *>(out);
+    const int write_final = L.n_splits > 1 ? 0 : 1;
+    dim3 grid((unsigned)p->n_tiles, (unsigned)L.n_splits);
+    for (int v0 = 0; v0 < V; v0 += kMaxVB) {
+        const int vb = (V - v0 < kMaxVB) ? V - v0 : kMaxVB;
+        if (precision == FM_PREC_F64)
+            launch_direct<false>(p, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
+        else
+            launch_direct<true>(p, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
+        FM_LAUNCH_CHECK("k_signal_direct");
+    }
+    if (L.n_splits > 1) {
+        const int64_t SV = S * V;
+        k_reduce_splits<<<(unsigned)cdiv(SV, 256), 256, 0, st>>>(dst, L.n_splits, SV,
+                                                               reinterpret_cast<double2 *>(out));
+        FM_LAUNCH_CHECK("k_reduce_splits");
+    }
+    return FM_OK;
+}
+
+}  // namespace fm
+
+using namespace fm;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int64_t n_lines,
+                                     int64_t n_kz, const double *kx, const double *ky,
+                                     const double *kz, const double *t, int method)
+{
+    FM_CHECK_ARG(plan != nullptr, "plan pointer is NULL");
+    *plan = nullptr;
+    FM_CHECK_ARG(n_meas >= 0 && n_lines >= 0 && n_kz >= 0, "negative k-space dimension");
+    FM_CHECK_ARG(method == FM_METHOD_AUTO || method == FM_METHOD_DIRECT || method == FM_METHOD_CARTESIAN,
+                 "unknown method %d", method);
+    const int64_t S = n_meas * n_lines * n_kz;
+    FM_CHECK_ARG(S < (int64_t(1) << 31), "too many k-space samples (%lld)", (long long)S);
+    FM_CHECK_ARG(S == 0 || (kx && ky && kz && t), "NULL k-space array");
+
+    auto *p = new fm_signal_plan();
+    FM_HIP(hipGetDevice(&p->device));
+    p->n_meas = n_meas;
+    p->n_lines = n_lines;
+    p->n_kz = n_kz;
+    p->n_samples = S;
+
+    // distinct t in first-appearance order (equal values share decay/phase)
+    std::unordered_map<uint64_t, int32_t> gid;
+    p->sample_group.resize((size_t)S);
+    for (int64_t s = 0; s < S; ++s) {
+        double tv = t[s];
+        if (tv == 0.0) tv = 0.0;  // -0.0 and 0.0 compare equal in the reference's cache
+        uint64_t key;
+        std::memcpy(&key, &tv, sizeof key);
+        auto it = gid.find(key);
+        int32_t g;
+        if (it == gid.end()) {
+            g = (int32_t)p->tg_t.size();
+            gid.emplace(key, g);
+            p->tg_t.push_back(t[s]);
+        } else {
+            g = it->second;
+        }
+        p->sample_group[(size_t)s] = g;
+    }
+    p->n_tgroups = (int64_t)p->tg_t.size();
+
+    if (method != FM_METHOD_DIRECT && S > 0) {
+        const bool ok = cartesian_analyse(p, kx, ky, kz, t);
+        if (method == FM_METHOD_CARTESIAN && !ok) {
+            delete p;
+            set_error("k-space sampling does not factorise as a Cartesian grid "
+                      "(k not additively separable per axis, or t not a function of the readout index)");
+            return FM_ERR_UNSUPPORTED;
+        }
+        p->method = ok ? FM_METHOD_CARTESIAN : FM_METHOD_DIRECT;
+    }
+
+    // direct-path tiling: shared-t tiles when groups are large enough to
+    // fill most of a 512-sample tile, else plain consecutive tiles
+    std::vector<int32_t> perm((size_t)S);
+    std::vector<SignalTile> tiles;
+    const double avg_group = p->n_tgroups ? (double)S / (double)p->n_tgroups : 0.0;
+    p->shared_t = avg_group >= 64.0 || S == 0;
+    if (p->shared_t) {
+        std::vector<int64_t> count((size_t)p->n_tgroups + 1, 0);
+        for (int64_t s = 0; s < S; ++s) count[(size_t)p->sample_group[(size_t)s] + 1]++;
+        for (int64_t g = 0; g < p->n_tgroups; ++g) count[(size_t)g + 1] += count[(size_t)g];
+        std::vector<int64_t> fill(count.begin(), count.end() - 1);
+        for (int64_t s = 0; s < S; ++s) perm[(size_t)fill[(size_t)p->sample_group[(size_t)s]]++] = (int32_t)s;
+        for (int64_t g = 0; g < p->n_tgroups; ++g) {
+            for (int64_t o = count[(size_t)g]; o < count[(size_t)g + 1]; o += kDirectTile) {
+                const int64_t c = std::min<int64_t>(kDirectTile, count[(size_t)g + 1] - o);
+                tiles.push_back({(int32_t)g, (int32_t)o, (int32_t)c, 0});
+            }
+        }
+    } else {
+        for (int64_t s = 0; s < S; ++s) perm[(size_t)s] = (int32_t)s;
+        for (int64_t o = 0; o < S; o += kDirectTile)
+            tiles.push_back({-1, (int32_t)o, (int32_t)std::min<int64_t>(kDirectTile, S - o), 0});
+    }
+    p->n_tiles = (int64_t)tiles.size();
+
+    std::vector<double4> ks((size_t)S);
+    for (int64_t s = 0; s < S; ++s)
+        ks[(size_t)s] = make_double4(kx[s] * kPiFOverPi, ky[s] * kPiFOverPi, kz[s] * kPiFOverPi, t[s]);
+
+    auto fail = [&](hipError_t e, const char *what) {
+        set_error("%s failed: %s", what, hipGetErrorString(e));
+        fm_signal_plan_destroy(p);
+        return FM_ERR_HIP;
+    };
+    hipError_t e;
+    if (S > 0) {
+        if ((e = hipMalloc(&p->d_ks, sizeof(double4) * (size_t)S)) != hipSuccess) return fail(e, "hipMalloc");
+        if ((e = hipMalloc(&p->d_perm, sizeof(int32_t) * (size_t)S)) != hipSuccess) return fail(e, "hipMalloc");
+        if ((e = hipMalloc(&p->d_tiles, sizeof(SignalTile) * tiles.size())) != hipSuccess) return fail(e, "hipMalloc");
+        if ((e = hipMalloc(&p->d_tg_t, sizeof(double) * (size_t)p->n_tgroups)) != hipSuccess) return fail(e, "hipMalloc");
+        if ((e = hipMemcpy(p->d_ks, ks.data(), sizeof(double4) * (size_t)S, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        if ((e = hipMemcpy(p->d_perm, perm.data(), sizeof(int32_t) * (size_t)S, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        if ((e = hipMemcpy(p->d_tiles, tiles.data(), sizeof(SignalTile) * tiles.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        if ((e = hipMemcpy(p->d_tg_t, p->tg_t.data(), sizeof(double) * (size_t)p->n_tgroups, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        if (p->method == FM_METHOD_CARTESIAN) {
+            const int rc = cartesian_upload(p);
+            if (rc != FM_OK) {
+                fm_signal_plan_destroy(p);
+                return rc;
+            }
+        }
+    }
+    *plan = p;
+    return FM_OK;
+}
+
+extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
+{
+    if (!p) return FM_OK;
+    (void)hipFree(p->d_ks);
+    (void)hipFree(p->d_perm);
+    (void)hipFree(p->d_tiles);
+    (void)hipFree(p->d_tg_t);
+    cartesian_free(p);
+    delete p;
+    return FM_OK;
+}
+
+extern "C" int fm_signal_plan_info(const fm_signal_plan *p, int64_t *n_samples, int64_t *n_tgroups,
+                                   int *method, double *tgroup_t)
+{
+    FM_CHECK_ARG(p != nullptr, "plan is NULL");
+    if (n_samples) *n_samples = p->n_samples;
+    if (n_tgroups) *n_tgroups = p->n_tgroups;
+    if (method) *method = p->method;
+    if (tgroup_t && p->n_tgroups) std::memcpy(tgroup_t, p->tg_t.data(), sizeof(double) * (size_t)p->n_tgroups);
+    return FM_OK;
+}
+
+extern "C" int fm_signal_workspace_size(const fm_signal_plan *p, int precision, int64_t n_nodes,
+                                        int n_enc, size_t *bytes)
+{
+    FM_CHECK_ARG(p != nullptr && bytes != nullptr, "NULL argument");
+    FM_CHECK_ARG(n_nodes >= 0 && n_enc >= 1, "bad sizes");
+    if (p->method == FM_METHOD_CARTESIAN)
+        *bytes = cartesian_workspace(p, precision, n_nodes, n_enc);
+    else
+        *bytes = direct_workspace(p, n_nodes, n_enc);
+    return FM_OK;
+}
+
+extern "C" int fm_signal_nodes_pack(int64_t n_nodes, int n_enc, int dtype, const void *r0,
+                                    const void *phi_dB0, const void *T2, const void *mxy,
+                                    const double *weights, double *packed, void *stream)
+{
+    FM_CHECK_ARG(n_nodes >= 0 && n_enc >= 1, "bad sizes (n_nodes=%lld, n_enc=%d)", (long long)n_nodes, n_enc);
+    FM_CHECK_ARG(dtype == FM_DTYPE_F32 || dtype == FM_DTYPE_F64, "unknown dtype %d", dtype);
+    if (n_nodes == 0) return FM_OK;
+    FM_CHECK_ARG(r0 && phi_dB0 && T2 && mxy && packed, "NULL array");
+    const unsigned blocks = (unsigned)cdiv(n_nodes, 256);
+    hipStream_t st = as_stream(stream);
+    if (dtype == FM_DTYPE_F64)
+        k_nodes_pack<double><<<blocks, 256, 0, st>>>(n_nodes, n_enc, (const double *)r0, (const double *)phi_dB0,
+                                                    (const double *)T2, (const double *)mxy, weights, packed);
+    else
+        k_nodes_pack<float><<<blocks, 256, 0, st>>>(n_nodes, n_enc, (const float *)r0, (const float *)phi_dB0,
+                                                   (const float *)T2, (const float *)mxy, weights, packed);
+    FM_LAUNCH_CHECK("k_nodes_pack");
+    return FM_OK;
+}
+
+extern "C" int fm_signal_execute(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc,
+                                 const double *packed, const float *modes, int n_modes,
+                                 const double *motion_coef, void *workspace, size_t workspace_bytes,
+                                 double *out, void *stream)
+{
+    FM_CHECK_ARG(p != nullptr, "plan is NULL");
+    FM_CHECK_ARG(precision == FM_PREC_F64 || precision == FM_PREC_MIXED,
+                 "signal precision must be FM_PREC_F64 or FM_PREC_MIXED (got %d)", precision);
+    FM_CHECK_ARG(n_nodes >= 0 && n_enc >= 1, "bad sizes (n_nodes=%lld, n_enc=%d)", (long long)n_nodes, n_enc);
+    FM_CHECK_ARG(p->n_samples == 0 || out != nullptr, "out is NULL");
+    FM_CHECK_ARG(n_nodes == 0 || packed != nullptr, "packed is NULL");
+    FM_CHECK_ARG(n_modes >= 0 && n_modes <= kMaxModes, "n_modes must be in [0, %d]", kMaxModes);
+    FM_CHECK_ARG(!(modes != nullptr && n_modes > 0) || motion_coef != nullptr, "motion_coef is NULL");
+    int dev = 0;
+    FM_HIP(hipGetDevice(&dev));
+    FM_CHECK_ARG(dev == p->device, "plan was created on device %d, current device is %d", p->device, dev);
+    hipStream_t st = as_stream(stream);
+    const bool motion = modes != nullptr && n_modes > 0;
+    if (p->method == FM_METHOD_CARTESIAN && !motion)
+        return cartesian_execute(p, precision, n_nodes, n_enc, packed, workspace, workspace_bytes, out, st);
+    if (p->method == FM_METHOD_CARTESIAN && motion) {
+        set_error("Cartesian GEMM path takes static nodes; build the plan with FM_METHOD_DIRECT for moving nodes");
+        return FM_ERR_UNSUPPORTED;
+    }
+    return direct_execute(p, precision, n_nodes, n_enc, packed, modes, n_modes, motion_coef, workspace,
+                          workspace_bytes, out, st);
+}
+
+extern "C" int fm_csr_spmv_c128(int64_t n_rows, int n_enc, const int64_t *indptr, const int64_t *indices,
+                                const double *data, const double *x, double *y, void *stream)
+{
+    FM_CHECK_ARG(n_rows >= 0 && n_enc >= 1, "bad sizes");
+    if (n_rows == 0) return FM_OK;
+    FM_CHECK_ARG(indptr && indices && data && x && y, "NULL array");
+    const int64_t total = n_rows * n_enc;
+    k_csr_spmv_c128<<<(unsigned)cdiv(total, 256), 256, 0, as_stream(stream)>>>(
+        n_rows, n_enc, indptr, indices, data, reinterpret_cast<const double2 *>(x), reinterpret_cast<double2 *>(y));
+    FM_LAUNCH_CHECK("k_csr_spmv_c128");
+    return FM_OK;
+}

@@ -1,0 +1,64 @@
+// Internal layout of fm_signal_plan (shared by the direct and Cartesian paths).
+#pragma once
+#include <vector>
+
+#include "fm_common.h"
+
+namespace fm {
+
+// Samples handled by one workgroup of the direct kernels (256 lanes x 2).
+constexpr int kDirectThreads = 256;
+constexpr int kDirectSPL = 2;
+constexpr int kDirectTile = kDirectThreads * kDirectSPL;
+// Encodings handled per direct launch (register-resident accumulators).
+constexpr int kMaxVB = 4;
+// Upper bound on POD modes fused into the loaders.
+constexpr int kMaxModes = 32;
+
+struct SignalTile {
+    int32_t group;   // t-group index (shared-t kernels) or -1
+    int32_t offset;  // first entry in perm
+    int32_t count;   // samples in the tile
+    int32_t pad;
+};
+
+struct CartesianTables {
+    bool ok = false;
+    int64_t n_ro = 0, n_ln = 0, n_kz = 0;
+    // Additive split k(i,j,k) = F(i) + G(j) + H(k) per axis, cycles/m,
+    // already multiplied by (double)(float)pi / pi.
+    std::vector<double> F, G, H;  // (n_ro,3), (n_ln,3), (n_kz,3)
+    std::vector<double> t;        // (n_ro)
+    double *d_F = nullptr, *d_G = nullptr, *d_H = nullptr, *d_t = nullptr;
+};
+
+}  // namespace fm
+
+struct fm_signal_plan {
+    int device = 0;
+    int64_t n_meas = 0, n_lines = 0, n_kz = 0, n_samples = 0;
+    int method = FM_METHOD_DIRECT;
+    bool shared_t = true;         // tiles group samples of equal t
+    int64_t n_tgroups = 0;
+    std::vector<double> tg_t;     // distinct t, first-appearance order
+    std::vector<int32_t> sample_group;
+    int64_t n_tiles = 0;
+    // device tables
+    double4 *d_ks = nullptr;      // per sample: kx', ky', kz' (cycles/m, pi_f-scaled), t
+    int32_t *d_perm = nullptr;    // samples grouped by tile
+    fm::SignalTile *d_tiles = nullptr;
+    double *d_tg_t = nullptr;
+    fm::CartesianTables cart;
+};
+
+namespace fm {
+// Cartesian path (signal_gemm.hip).
+bool cartesian_analyse(fm_signal_plan *p, const double *kx, const double *ky, const double *kz,
+                       const double *t);
+int cartesian_upload(fm_signal_plan *p);
+void cartesian_free(fm_signal_plan *p);
+size_t cartesian_workspace(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc);
+int cartesian_execute(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc,
+                      const double *packed, void *ws, size_t ws_bytes, double *out,
+                      hipStream_t stream);
+}  // namespace fm

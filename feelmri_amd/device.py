@@ -1,0 +1,369 @@
+"""Device-resident API over the C ABI: torch tensors in HBM in, tensors out.
+
+The reference-shaped, numpy-in/numpy-out calls (``feelmri_amd.MRI.Signal``
+and friends) are thin shims over these.  torch provides device memory and
+streams only; every computation runs in libfeelmri_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+PRECISIONS = {"f64": _lib.PREC_F64, "mixed": _lib.PREC_MIXED, "f32": _lib.PREC_F32}
+METHODS = {"auto": _lib.METHOD_AUTO, "direct": _lib.METHOD_DIRECT, "cartesian": _lib.METHOD_CARTESIAN}
+METHOD_NAMES = {v: k for k, v in METHODS.items()}
+
+
+def default_device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("feelmri_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_handle(stream: torch.cuda.Stream | None = None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def as_device(a, dtype: torch.dtype, device) -> torch.Tensor:
+    """Contiguous device tensor of ``dtype`` from numpy / torch / scalars."""
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=dtype).contiguous()
+    arr = np.ascontiguousarray(np.asarray(a))
+    return torch.from_numpy(arr).to(device=device, dtype=dtype).contiguous()
+
+
+def _prec(precision) -> int:
+    if isinstance(precision, int):
+        return precision
+    try:
+        return PRECISIONS[precision]
+    except KeyError:
+        raise ValueError(f"unknown precision {precision!r}; expected one of {list(PRECISIONS)}") from None
+
+
+# ---------------------------------------------------------------------------
+# Signal
+# ---------------------------------------------------------------------------
+class SignalPlan:
+    """Resident sampling tables for one trajectory (``fm_signal_plan``).
+
+    ``kloc`` is the reference's ``(kx, ky, kz)`` of shape (n_meas, n_lines,
+    n_kz) in cycles/m and ``t`` the sample times in ms (MRI.h:19-20).
+    """
+
+    def __init__(self, kloc, t, method: str = "auto", device=None):
+        self.device = torch.device(device) if device is not None else default_device()
+        kx, ky, kz = (np.ascontiguousarray(np.asarray(k, dtype=np.float64)) for k in kloc)
+        tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64))
+        if tt.ndim != 3:
+            raise ValueError(f"t must be 3-D (n_meas, n_lines, n_kz), got shape {tt.shape}")
+        for name, k in (("kx", kx), ("ky", ky), ("kz", kz)):
+            if k.shape != tt.shape:
+                raise ValueError(f"{name} shape {k.shape} differs from t shape {tt.shape}")
+        self.shape = tuple(int(s) for s in tt.shape)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.call("fm_signal_plan_create", ctypes.byref(handle), *self.shape,
+                      kx.ctypes.data_as(ctypes.c_void_p), ky.ctypes.data_as(ctypes.c_void_p),
+                      kz.ctypes.data_as(ctypes.c_void_p), tt.ctypes.data_as(ctypes.c_void_p),
+                      METHODS[method])
+        self._h = handle
+        ns, ng, meth = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        _lib.call("fm_signal_plan_info", self._h, ctypes.byref(ns), ctypes.byref(ng),
+                  ctypes.byref(meth), None)
+        self.n_samples, self.n_tgroups, self.method = ns.value, ng.value, METHOD_NAMES[meth.value]
+        tg = np.empty(self.n_tgroups, dtype=np.float64)
+        _lib.call("fm_signal_plan_info", self._h, None, None, None,
+                  tg.ctypes.data_as(ctypes.c_void_p))
+        self.tgroup_t = tg
+        self._t = tt
+
+    def sample_groups(self) -> np.ndarray:
+        """Index into ``tgroup_t`` of every sample, in sample order."""
+        order = np.argsort(self.tgroup_t, kind="stable")
+        idx = np.searchsorted(self.tgroup_t[order], self._t.ravel())
+        return order[np.clip(idx, 0, max(self.n_tgroups - 1, 0))]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().fm_signal_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def workspace_bytes(self, precision, n_nodes: int, n_enc: int) -> int:
+        out = ctypes.c_size_t()
+        _lib.call("fm_signal_workspace_size", self._h, _prec(precision), int(n_nodes), int(n_enc),
+                  ctypes.byref(out))
+        return int(out.value)
+
+    def execute(self, nodes: "SignalNodes", precision="f64", motion=None, out=None,
+                workspace=None, stream=None) -> torch.Tensor:
+        """Evaluate; returns (n_samples, n_enc) complex128 on the device.
+
+        ``motion``: (modes_soa (3R, N) float32 tensor, coef (n_tgroups, R+3)
+        float64 tensor) or None.
+        """
+        prec = _prec(precision)
+        S, V, N = self.n_samples, nodes.n_enc, nodes.n_nodes
+        if out is None:
+            out = torch.empty((S, V), dtype=torch.complex128, device=self.device)
+        elif out.dtype != torch.complex128 or tuple(out.shape) != (S, V) or not out.is_contiguous():
+            raise ValueError("out must be a contiguous complex128 tensor of shape (n_samples, n_enc)")
+        need = self.workspace_bytes(prec, N, V)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=self.device)
+        modes = coef = None
+        R = 0
+        if motion is not None:
+            modes, coef = motion
+            R = int(modes.shape[0]) // 3
+            if modes.dtype != torch.float32 or modes.shape[1] != N:
+                raise ValueError("motion modes must be float32 of shape (3R, n_nodes)")
+            if coef.dtype != torch.float64 or tuple(coef.shape) != (self.n_tgroups, R + 3):
+                raise ValueError("motion coefficients must be float64 of shape (n_tgroups, R+3)")
+        with torch.cuda.device(self.device):
+            _lib.call("fm_signal_execute", self._h, prec, N, V, ptr(nodes.packed), ptr(modes), R,
+                      ptr(coef), ptr(workspace), workspace.numel(), ptr(out), stream_handle(stream))
+        return out
+
+
+class SignalNodes:
+    """Packed SoA node table of ``fm_signal_nodes_pack``."""
+
+    def __init__(self, packed: torch.Tensor, n_nodes: int, n_enc: int):
+        self.packed, self.n_nodes, self.n_enc = packed, n_nodes, n_enc
+
+    @classmethod
+    def from_arrays(cls, r0, phi_dB0, T2, mxy, weights=None, dtype=torch.float64, device=None,
+                    stream=None) -> "SignalNodes":
+        """r0 (N,3), phi_dB0 (N,) rad/ms, T2 (N,) ms, mxy (N,V) complex (= M Mxy,
+        or Mxy with ``weights`` the lumped diagonal of M)."""
+        device = torch.device(device) if device is not None else default_device()
+        cdtype = torch.complex128 if dtype == torch.float64 else torch.complex64
+        r0 = as_device(r0, dtype, device).reshape(-1, 3)
+        N = r0.shape[0]
+        phi = as_device(phi_dB0, dtype, device).reshape(-1)
+        T2 = as_device(T2, dtype, device).reshape(-1)
+        mx = as_device(mxy, cdtype, device)
+        if mx.ndim == 1:
+            mx = mx.reshape(-1, 1)
+        if phi.numel() != N or T2.numel() != N or mx.shape[0] != N:
+            raise ValueError(f"node arrays disagree on the node count ({N}, {phi.numel()}, "
+                             f"{T2.numel()}, {mx.shape[0]})")
+        V = int(mx.shape[1])
+        w = None if weights is None else as_device(weights, torch.float64, device).reshape(-1)
+        if w is not None and w.numel() != N:
+            raise ValueError("weights length differs from the node count")
+        packed = torch.empty((5 + 2 * V, N), dtype=torch.float64, device=device)
+        dt_code = _lib.DTYPE_F64 if dtype == torch.float64 else _lib.DTYPE_F32
+        with torch.cuda.device(device):
+            _lib.call("fm_signal_nodes_pack", N, V, dt_code, ptr(r0), ptr(phi), ptr(T2),
+                      ptr(torch.view_as_real(mx)), ptr(w), ptr(packed), stream_handle(stream))
+        return cls(packed, N, V)
+
+
+def csr_spmv(indptr, indices, data, x: torch.Tensor, stream=None) -> torch.Tensor:
+    """y = M x on the device (M in CSR, x (n, V) complex128)."""
+    device = x.device
+    indptr = as_device(indptr, torch.int64, device)
+    indices = as_device(indices, torch.int64, device)
+    data = as_device(data, torch.float64, device)
+    x = x.to(torch.complex128).contiguous()
+    if x.ndim == 1:
+        x = x.reshape(-1, 1)
+    n_rows = indptr.numel() - 1
+    y = torch.empty((n_rows, x.shape[1]), dtype=torch.complex128, device=device)
+    with torch.cuda.device(device):
+        _lib.call("fm_csr_spmv_c128", n_rows, int(x.shape[1]), ptr(indptr), ptr(indices), ptr(data),
+                  ptr(x), ptr(y), stream_handle(stream))
+    return y
+
+
+# ---------------------------------------------------------------------------
+# Bloch
+# ---------------------------------------------------------------------------
+class BlochNodes:
+    """Packed SoA node table of ``fm_bloch_nodes_pack`` (x, y, z, 1/T1, 1/T2, dB)."""
+
+    def __init__(self, packed: torch.Tensor, n_pos: int):
+        self.packed, self.n_pos = packed, n_pos
+
+    @classmethod
+    def from_arrays(cls, r0, T1, T2, delta_B, dtype=torch.float64, device=None, stream=None):
+        device = torch.device(device) if device is not None else default_device()
+        r0 = as_device(r0, dtype, device).reshape(-1, 3)
+        N = r0.shape[0]
+        T1 = as_device(T1, dtype, device).reshape(-1)
+        T2 = as_device(T2, dtype, device).reshape(-1)
+        dB = as_device(delta_B, dtype, device).reshape(-1)
+        for name, a in (("T1", T1), ("T2", T2), ("delta_B", dB)):
+            if a.numel() != N:
+                raise ValueError(f"{name} has {a.numel()} entries, expected {N}")
+        packed = torch.empty((6, N), dtype=torch.float64, device=device)
+        dt_code = _lib.DTYPE_F64 if dtype == torch.float64 else _lib.DTYPE_F32
+        with torch.cuda.device(device):
+            _lib.call("fm_bloch_nodes_pack", N, dt_code, ptr(r0), ptr(T1), ptr(T2), ptr(dB),
+                      ptr(packed), stream_handle(stream))
+        return cls(packed, N)
+
+
+class EventTable:
+    """Device copy of one block's (rf, G, dt) event arrays."""
+
+    def __init__(self, rf, G, dt, device=None):
+        device = torch.device(device) if device is not None else default_device()
+        self.rf = as_device(np.asarray(rf).reshape(-1), torch.complex128, device)
+        self.G = as_device(np.asarray(G).reshape(-1, 3), torch.float64, device)
+        self.dt = as_device(np.asarray(dt).reshape(-1), torch.float64, device)
+        self.n_time = int(self.rf.numel())
+        if self.G.shape[0] != self.n_time or self.dt.numel() != self.n_time:
+            raise ValueError("rf, G and dt must have the same number of events")
+
+
+def bloch(nodes: BlochNodes, events: EventTable, M0: float, gamma: float, mxy0: torch.Tensor,
+          mz0: torch.Tensor, precision="f64", motion=None, history: bool = False, mxy_out=None,
+          mz_out=None, stream=None):
+    """Integrate one event table.  ``mxy0`` complex128 (N,), ``mz0`` float64 (N,).
+
+    Returns (mxy, mz): (N,) final state, or (N, T) column-major history
+    (returned as a (T, N) tensor transposed to (N, T)).
+    """
+    prec = _prec(precision)
+    N, T = nodes.n_pos, events.n_time
+    device = nodes.packed.device
+    real = torch.float64 if prec == _lib.PREC_F64 else torch.float32
+    cplx = torch.complex128 if prec == _lib.PREC_F64 else torch.complex64
+    if history:
+        shape_c, shape_r = (T, N), (T, N)
+    else:
+        # final state is always complex128 / float64 (chainable)
+        shape_c, shape_r = (N,), (N,)
+        real, cplx = torch.float64, torch.complex128
+    if mxy_out is None:
+        mxy_out = torch.empty(shape_c, dtype=cplx, device=device)
+    if mz_out is None:
+        mz_out = torch.empty(shape_r, dtype=real, device=device)
+    modes = coef = None
+    R = 0
+    if motion is not None:
+        modes, coef = motion
+        R = int(modes.shape[0]) // 3
+        if coef.dtype != torch.float64 or tuple(coef.shape) != (max(T - 1, 0), R + 3):
+            raise ValueError("Bloch motion coefficients must be float64 of shape (n_time-1, R+3)")
+    with torch.cuda.device(device):
+        _lib.call("fm_bloch_execute", prec, N, T, ptr(nodes.packed), float(M0), float(gamma),
+                  ptr(torch.view_as_real(events.rf)), ptr(events.G), ptr(events.dt),
+                  ptr(torch.view_as_real(mxy0)), ptr(mz0), ptr(modes), R, ptr(coef),
+                  ptr(mxy_out), ptr(mz_out), int(bool(history)), stream_handle(stream))
+    if history:
+        return mxy_out.T, mz_out.T
+    return mxy_out, mz_out
+
+
+def bloch_store_spoil(mxy_state, mz_state, mxy_store=None, mz_store=None, col: int = -1,
+                      spoil: bool = False, stream=None):
+    N = int(mz_state.numel())
+    with torch.cuda.device(mz_state.device):
+        _lib.call("fm_bloch_store_spoil", N, ptr(torch.view_as_real(mxy_state)), ptr(mz_state),
+                  ptr(mxy_store), ptr(mz_store), int(col), int(bool(spoil)), stream_handle(stream))
+
+
+# ---------------------------------------------------------------------------
+# P1 mass
+# ---------------------------------------------------------------------------
+class P1Colouring:
+    """Host greedy colouring of a tet mesh, resident on the device."""
+
+    def __init__(self, elems, n_nodes: int, device=None):
+        device = torch.device(device) if device is not None else default_device()
+        el = np.ascontiguousarray(np.asarray(elems), dtype=np.int32)
+        if el.ndim != 2 or el.shape[1] != 4:
+            raise ValueError(f"P1 tetrahedra expected: elems of shape (E, 4), got {el.shape}")
+        E = el.shape[0]
+        perm = np.empty(E, dtype=np.int32)
+        offs = np.zeros(65, dtype=np.int64)
+        nc = ctypes.c_int()
+        _lib.call("fm_mass_p1_color", E, int(n_nodes), el.ctypes.data_as(ctypes.c_void_p),
+                  perm.ctypes.data_as(ctypes.c_void_p), offs.ctypes.data_as(ctypes.c_void_p), 64,
+                  ctypes.byref(nc))
+        self.n_colours = nc.value
+        self.offsets = np.ascontiguousarray(offs[: self.n_colours + 1])
+        self.n_elems, self.n_nodes = E, int(n_nodes)
+        self.elems = torch.from_numpy(el).to(device)
+        self.perm = torch.from_numpy(perm).to(device)
+        self.device = device
+
+
+def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None) -> torch.Tensor:
+    """w_i = sum_{e ni i} |T_e| / 4 (float64, (N,))."""
+    nodes = nodes.contiguous()
+    if nodes.dtype not in (torch.float32, torch.float64) or nodes.shape != (colouring.n_nodes, 3):
+        raise ValueError("nodes must be float32/float64 of shape (n_nodes, 3)")
+    if out is None:
+        out = torch.empty(colouring.n_nodes, dtype=torch.float64, device=nodes.device)
+    dt_code = _lib.DTYPE_F64 if nodes.dtype == torch.float64 else _lib.DTYPE_F32
+    with torch.cuda.device(nodes.device):
+        _lib.call("fm_mass_p1_lumped", colouring.n_elems, colouring.n_nodes, ptr(colouring.elems),
+                  dt_code, ptr(nodes), ptr(colouring.perm),
+                  colouring.offsets.ctypes.data_as(ctypes.c_void_p), colouring.n_colours, ptr(out),
+                  stream_handle(stream))
+    return out
+
+
+def element_matrices(elems: torch.Tensor, nodes: torch.Tensor, stream=None) -> torch.Tensor:
+    E = int(elems.shape[0])
+    vals = torch.empty((E, 4, 4), dtype=torch.float64, device=nodes.device)
+    dt_code = _lib.DTYPE_F64 if nodes.dtype == torch.float64 else _lib.DTYPE_F32
+    with torch.cuda.device(nodes.device):
+        _lib.call("fm_mass_p1_element_matrices", E, ptr(elems.contiguous()), dt_code,
+                  ptr(nodes.contiguous()), ptr(vals), stream_handle(stream))
+    return vals
+
+
+# ---------------------------------------------------------------------------
+# POD
+# ---------------------------------------------------------------------------
+def pod_tensordot(modes: torch.Tensor, weights: torch.Tensor, stream=None) -> torch.Tensor:
+    P, C, M = (int(s) for s in modes.shape)
+    out = torch.empty((P, C), dtype=modes.dtype, device=modes.device)
+    dt_code = _lib.DTYPE_F64 if modes.dtype == torch.float64 else _lib.DTYPE_F32
+    with torch.cuda.device(modes.device):
+        _lib.call("fm_pod_tensordot", dt_code, P, C, M, ptr(modes.contiguous()),
+                  ptr(weights.contiguous()), ptr(out), stream_handle(stream))
+    return out
+
+
+def modes_soa(modes_pcr: torch.Tensor, stream=None) -> torch.Tensor:
+    """(P, 3, R) float32 modes -> (3R, P) rows for the fused loaders."""
+    P, C, R = (int(s) for s in modes_pcr.shape)
+    if C != 3:
+        raise ValueError("modes must have 3 components")
+    out = torch.empty((3 * R, P), dtype=torch.float32, device=modes_pcr.device)
+    with torch.cuda.device(modes_pcr.device):
+        _lib.call("fm_modes_pack", P, R, ptr(modes_pcr.to(torch.float32).contiguous()), ptr(out),
+                  stream_handle(stream))
+    return out
+
+
+def pod_positions(xyz_soa: torch.Tensor, modes: torch.Tensor | None, coef: torch.Tensor,
+                  stream=None) -> torch.Tensor:
+    """r0 + sum_r modes_r w_r + trans -> (N, 3) float64."""
+    N = int(xyz_soa.shape[1])
+    R = 0 if modes is None else int(modes.shape[0]) // 3
+    out = torch.empty((N, 3), dtype=torch.float64, device=xyz_soa.device)
+    with torch.cuda.device(xyz_soa.device):
+        _lib.call("fm_pod_positions", N, ptr(xyz_soa.contiguous()), ptr(modes), R,
+                  ptr(coef.contiguous()), ptr(out), stream_handle(stream))
+    return out

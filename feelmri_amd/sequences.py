@@ -1,0 +1,150 @@
+"""Sequence event tables for the Bloch integrator.
+
+A block is an event table ``(rf (T,) complex mT, G (T,3) mT/m, dt (T,) ms)``
+plus the flags ``BlochSolver.solve`` reads (``empty``, ``store``) and its
+start time ``t0`` (the motion time shift, python/feelmri/Bloch.py:335-336).
+
+The event grid follows ``SequenceBlock._discretization``
+(python/feelmri/Bloch.py:105-129): the union of the RF start/end times and a
+``dt_rf`` grid over each pulse, the gradient knots and a ``dt_gr`` grid over
+each gradient, and a ``dt`` grid over the block extent (``arange``, so the
+extent's end is not on the grid), sorted and de-duplicated; then
+``dt = diff(times, prepend=0)`` (Bloch.py:340).  Step i -> i+1 of the
+integrator uses the RF/gradient values at ``times[i+1]``.
+
+Pulses here are hard (constant B1 over (start, end]); gradients are
+trapezoids.  The reference's sinc pulses and slice-profile machinery
+(python/feelmri/MRObjects.py, MRImaging.py) are host-side sequence design
+and are not restated.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+GAMMA = 42.58e6 * 2.0 * np.pi / 1e6   # rad/(ms*mT), python/feelmri/MRObjects.py:39-40
+
+
+@dataclass
+class HardPulse:
+    start: float        # ms
+    dur: float          # ms
+    flip: float         # rad
+    phase: float = 0.0  # rad
+
+    @property
+    def b1(self) -> float:
+        return self.flip / (GAMMA * self.dur)
+
+    def __call__(self, t):
+        t = np.asarray(t, dtype=np.float64)
+        on = (t > self.start) & (t <= self.start + self.dur)
+        return np.where(on, self.b1 * np.exp(1j * self.phase), 0.0 + 0.0j)
+
+
+@dataclass
+class Trapezoid:
+    axis: int           # 0, 1, 2
+    start: float        # ms
+    amp: float          # mT/m
+    ramp: float         # ms
+    flat: float         # ms
+
+    @property
+    def timings(self) -> np.ndarray:
+        s = self.start
+        return np.array([s, s + self.ramp, s + self.ramp + self.flat, s + 2 * self.ramp + self.flat])
+
+    @property
+    def amplitudes(self) -> np.ndarray:
+        return np.array([0.0, self.amp, self.amp, 0.0])
+
+    def __call__(self, t):
+        return np.interp(np.asarray(t, dtype=np.float64), self.timings, self.amplitudes,
+                         left=0.0, right=0.0)
+
+
+@dataclass
+class Block:
+    times: np.ndarray
+    rf: np.ndarray
+    G: np.ndarray
+    dt: np.ndarray
+    empty: bool = False
+    store: bool = False
+    t0: float = 0.0
+    info: dict = field(default_factory=dict)
+
+    @property
+    def n_events(self) -> int:
+        return int(self.times.size)
+
+    def as_dict(self) -> dict:
+        return dict(rf=self.rf, G=self.G, dt=self.dt, empty=self.empty, store=self.store,
+                    t0=self.t0)
+
+
+def make_block(t0: float, dur: float, pulses=(), gradients=(), dt_rf: float = 0.01,
+               dt_gr: float = -1.0, dt: float = 10.0, empty: bool = False,
+               store: bool = False) -> Block:
+    """Discretise one block starting at ``t0`` (ms) with extent ``dur``."""
+    parts = []
+    if gradients:
+        parts.append(np.concatenate([g.timings for g in gradients]))
+        if dt_gr > 0:
+            parts += [np.arange(g.timings[0], g.timings[-1], dt_gr) for g in gradients]
+    if pulses:
+        parts.append(np.concatenate([[p.start, p.start + p.dur] for p in pulses]))
+        if dt_rf > 0:
+            parts += [np.arange(p.start, p.start + p.dur, dt_rf) for p in pulses]
+    ext0 = min([t0] + [p.start for p in pulses] + [g.timings[0] for g in gradients])
+    ext1 = max([t0 + dur] + [p.start + p.dur for p in pulses] + [g.timings[-1] for g in gradients])
+    parts.append(np.arange(ext0, ext1, dt))
+    times = np.unique(np.sort(np.concatenate(parts)))
+    dts = np.diff(times, prepend=0.0)
+    rf = np.zeros(times.size, dtype=np.complex128)
+    for p in pulses:
+        rf += p(times)
+    G = np.zeros((times.size, 3), dtype=np.float64)
+    for g in gradients:
+        G[:, g.axis] += g(times)
+    return Block(times=times, rf=rf, G=G, dt=dts, empty=empty, store=store, t0=ext0,
+                 info=dict(extent=(ext0, ext1)))
+
+
+def delay_block(t0: float, dur: float, dt: float = 1.0) -> Block:
+    """Empty (relaxation-only) block, as ``Sequence.add_block(Quantity)``
+    builds it (Bloch.py:195-204)."""
+    return make_block(t0, dur, dt=dt, empty=True)
+
+
+def spoiled_gre(flip_deg: float = 15.0, TR: float = 5.0, n_dummy: int = 16, n_lines: int = 64,
+                rf_dur: float = 0.2, dt_rf: float = 0.01, dt_delay: float = 1.0,
+                phase_cycle: bool = False) -> list[Block]:
+    """Spoiled gradient echo: per TR one imaging block (hard pulse) and one
+    delay block.  Dummy TRs are not stored; each of the ``n_lines`` imaging
+    blocks stores its post-pulse Mxy (``store_magnetization``).  Spoiling is
+    the reference's artificial one: Mxy is zeroed after every non-empty block
+    (Bloch.py:363-368)."""
+    blocks: list[Block] = []
+    t = 0.0
+    flip = np.deg2rad(flip_deg)
+    for k in range(n_dummy + n_lines):
+        ph = (np.pi * k * k / 2.0) if phase_cycle else 0.0
+        pulse = HardPulse(start=t, dur=rf_dur, flip=flip, phase=ph)
+        blk = make_block(t, rf_dur, pulses=[pulse], dt_rf=dt_rf, dt=1.0,
+                         store=(k >= n_dummy))
+        blocks.append(blk)
+        t_end = blk.info["extent"][1]
+        blocks.append(delay_block(t_end, TR - rf_dur, dt=dt_delay))
+        t = t + TR
+    return blocks
+
+
+def ernst_steady_state(flip_deg: float, TR_eff: float, T1):
+    """Ideal-spoiling FLASH steady state just before a pulse:
+    Mz- = M0 (1 - E1) / (1 - E1 cos a), E1 = exp(-TR_eff / T1)."""
+    a = np.deg2rad(flip_deg)
+    E1 = np.exp(-TR_eff / np.asarray(T1, dtype=np.float64))
+    return (1.0 - E1) / (1.0 - E1 * np.cos(a))

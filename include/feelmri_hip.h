@@ -1,0 +1,197 @@
+/*
+ * feelmri_hip.h -- C ABI of libfeelmri_hip.so, the MI355X (gfx950) signal-
+ * generation path of FEelMRI.
+ *
+ * Every entry point returns an int status (FM_OK = 0).  On failure
+ * fm_last_error() returns a message for the calling thread.  Device
+ * pointers are plain HIP device addresses (e.g. a torch tensor's
+ * data_ptr()); `stream` is a hipStream_t passed as void*.  Kernels run
+ * asynchronously on that stream; buffers are caller-owned.  Host pointers
+ * are marked "host".  Complex arrays are interleaved (re, im).
+ *
+ * Units follow the reference: positions m, k cycles/m, t/dt/T1/T2 ms,
+ * phi_dB0 rad/ms, delta_B mT, G mT/m, rf mT, gamma rad/(ms*mT).
+ *
+ * The reference interface each entry point replaces is cited as
+ * path:line in the reference repository (hmella/feelmri).
+ */
+#ifndef FEELMRI_HIP_H
+#define FEELMRI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    FM_OK = 0,
+    FM_ERR_ARG = 1,          /* invalid argument or shape */
+    FM_ERR_HIP = 2,          /* HIP runtime error */
+    FM_ERR_UNSUPPORTED = 3,  /* combination not supported by the chosen method */
+    FM_ERR_WORKSPACE = 4     /* workspace too small */
+};
+
+/* Arithmetic form. */
+enum {
+    FM_PREC_F64 = 0,    /* plain FP64 (the drop-in form) */
+    FM_PREC_MIXED = 1,  /* FP64 range reduction + FP32 sincos (direct);
+                           BF16 hi+lo MFMA with FP32 accumulation (Cartesian) */
+    FM_PREC_F32 = 2     /* FP32 throughout (Bloch) */
+};
+
+/* Element type of reference-layout inputs handed to the pack routines. */
+enum { FM_DTYPE_F32 = 0, FM_DTYPE_F64 = 1 };
+
+/* Signal evaluation method. */
+enum {
+    FM_METHOD_AUTO = 0,       /* Cartesian GEMM when the sampling factorises, else direct */
+    FM_METHOD_DIRECT = 1,     /* direct non-uniform sum (any trajectory) */
+    FM_METHOD_CARTESIAN = 2   /* factorised complex GEMM on the matrix cores */
+};
+
+const char *fm_last_error(void);
+int fm_version(void);
+
+/* ------------------------------------------------------------------------
+ * Signal -- replaces MRI.Signal (cpp/feelmri/MRI.h:43-82 bindings;
+ * cpp/feelmri/MRI.cpp:279-381 static nodes, :121-242 moving nodes):
+ *
+ *   S[s, v] = sum_n exp(i(-2 pi_f k_s . r_n(t_s) + phi_dB0_n t_s))
+ *                   exp(-t_s / T2_n) (M Mxy)[n, v]
+ *
+ * with pi_f = (double)(float)pi (MRI.h:13).  Split into a plan (host
+ * analysis of the sampling, made once per trajectory) and an execute call
+ * (device work only, no allocation).
+ * ---------------------------------------------------------------------- */
+typedef struct fm_signal_plan fm_signal_plan;
+
+/* kx, ky, kz, t: host arrays of shape (n_meas, n_lines, n_kz), C order --
+ * the kloc and t arguments of MRI.Signal (MRI.h:19-20).  Samples with equal
+ * t are grouped (the reference's t cache, MRI.cpp:343-359); the plan keeps
+ * the sampling tables resident in device memory (allocated here, once). */
+int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int64_t n_lines, int64_t n_kz,
+                          const double *kx, const double *ky, const double *kz,
+                          const double *t, int method);
+int fm_signal_plan_destroy(fm_signal_plan *plan);
+/* n_tgroups: number of distinct t; method: the method chosen
+ * (FM_METHOD_DIRECT or FM_METHOD_CARTESIAN).  tgroup_t (host, may be
+ * NULL) receives the n_tgroups distinct times in the order the motion
+ * coefficient table of fm_signal_execute must follow. */
+int fm_signal_plan_info(const fm_signal_plan *plan, int64_t *n_samples, int64_t *n_tgroups,
+                        int *method, double *tgroup_t);
+/* Workspace (device bytes) fm_signal_execute needs for this plan. */
+int fm_signal_workspace_size(const fm_signal_plan *plan, int precision, int64_t n_nodes,
+                             int n_enc, size_t *bytes);
+
+/* Pack reference-layout node inputs into the SoA table fm_signal_execute
+ * reads: rows x, y, z, phi_dB0 (rad/ms), 1/T2, then Re/Im of (w .* Mxy)[:, v]
+ * for each encoding v; (5 + 2 n_enc) rows of n_nodes doubles.
+ *   r0 (n_nodes, 3) row-major, phi_dB0 (n_nodes), T2 (n_nodes): dtype `dtype`
+ *   mxy (n_nodes, n_enc) complex of dtype `dtype` (M * Mxy already formed,
+ *       or Mxy when `weights` holds the lumped diagonal of M)
+ *   weights (n_nodes) float64 lumped diagonal of M, or NULL for identity. */
+int fm_signal_nodes_pack(int64_t n_nodes, int n_enc, int dtype, const void *r0,
+                         const void *phi_dB0, const void *T2, const void *mxy,
+                         const double *weights, double *packed, void *stream);
+
+/* Evaluate the signal.  out: (n_samples, n_enc) complex128, sample order
+ * (i, j, k) of the plan's arrays.  Motion (MRI.cpp:192-204): modes (NULL
+ * for static nodes) is (3 n_modes) rows of n_nodes float32, row c*n_modes+r
+ * holding component c of mode r; motion_coef (device) is (n_tgroups,
+ * n_modes + 3) doubles: per distinct t the mode weights followed by a
+ * uniform translation, so r_n(t) = r0_n + sum_r modes[:, r][n] w_r + trans. */
+int fm_signal_execute(const fm_signal_plan *plan, int precision, int64_t n_nodes, int n_enc,
+                      const double *packed, const float *modes, int n_modes,
+                      const double *motion_coef, void *workspace, size_t workspace_bytes,
+                      double *out, void *stream);
+
+/* y = M x for a CSR matrix M (int64 indptr/indices, float64 data) and
+ * x (n_cols, n_enc) complex128: the M * Mxy product of MRI.cpp:326 when M
+ * is the consistent mass matrix (examples/free_running.py:177). */
+int fm_csr_spmv_c128(int64_t n_rows, int n_enc, const int64_t *indptr, const int64_t *indices,
+                     const double *data, const double *x, double *y, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Bloch -- replaces BlochSimulator.solve_mri (cpp/feelmri/BlochSimulator.h:
+ * 50-80 bindings; BlochSimulator.cpp:231-357 static, :92-228 moving).
+ * One lane per node; the event table is staged through LDS.
+ * ---------------------------------------------------------------------- */
+/* Pack r0 (n,3) row-major, T1, T2, delta_B (n) of dtype `dtype` into the
+ * SoA table fm_bloch_execute reads: rows x, y, z, 1/T1, 1/T2, delta_B. */
+int fm_bloch_nodes_pack(int64_t n_pos, int dtype, const void *r0, const void *T1,
+                        const void *T2, const void *delta_B, double *packed, void *stream);
+
+/* precision FM_PREC_F64 or FM_PREC_F32.
+ * rf (n_time) complex128, G (n_time, 3) row-major, dt (n_time): device.
+ * mxy_in/mz_in: initial state, complex128 / float64 (n_pos).
+ * modes/motion_coef: as fm_signal_execute, with motion_coef holding one
+ * row per step i (n_time - 1 rows), evaluated at the end of the step.
+ * history != 0: mxy_out (n_pos, n_time) complex and mz_out (n_pos, n_time)
+ * column-major (element (p, i) at p + i n_pos), including column 0;
+ * history output element type is complex128/float64 for F64 and
+ * complex64/float32 for F32.  history == 0: final state only, (n_pos)
+ * complex128/float64 for both precisions, so it can seed the next call.
+ * mxy_out/mz_out may alias mxy_in/mz_in only when history == 0. */
+int fm_bloch_execute(int precision, int64_t n_pos, int64_t n_time, const double *packed,
+                     double M0, double gamma, const double *rf, const double *G,
+                     const double *dt, const double *mxy_in, const double *mz_in,
+                     const float *modes, int n_modes, const double *motion_coef,
+                     void *mxy_out, void *mz_out, int history, void *stream);
+
+/* Artificial spoiling and storage between blocks (python/feelmri/
+ * Bloch.py:357-368): copies the state into column `store_col` of the
+ * (n_pos, n_store) column-major stores when store_col >= 0, then zeroes
+ * mxy_state when `spoil` is set.  State: complex128/float64; stores:
+ * complex64/float32 (the reference's BlochSolver arrays, Bloch.py:321-322). */
+int fm_bloch_store_spoil(int64_t n_pos, double *mxy_state, const double *mz_state,
+                         void *mxy_store, void *mz_store, int64_t store_col, int spoil,
+                         void *stream);
+
+/* ------------------------------------------------------------------------
+ * P1 mass -- replaces Assemble.MassAssemble (cpp/feelmri/Assemble.h:21-39;
+ * Assemble.cpp:116-155) followed by lumping (python/feelmri/
+ * Phantom.py:379-399).  P1 tetrahedra only.
+ * ---------------------------------------------------------------------- */
+/* Host: greedy element colouring so that no two elements of a colour share
+ * a node.  elems (n_elems, 4) int32, host.  perm (host, n_elems) receives
+ * the elements ordered by colour; color_offsets (host, max_colors + 1)
+ * the start of each colour in perm.  At most 64 colours. */
+int fm_mass_p1_color(int64_t n_elems, int64_t n_nodes, const int32_t *elems, int32_t *perm,
+                     int64_t *color_offsets, int max_colors, int *n_colors);
+
+/* Lumped weights w_i = sum_{e ni i} |T_e| / 4 by an atomic-free scatter:
+ * one launch per colour.  elems (n_elems, 4) int32, nodes (n_nodes, 3)
+ * row-major of dtype `dtype`, perm: device; color_offsets: host. */
+int fm_mass_p1_lumped(int64_t n_elems, int64_t n_nodes, const int32_t *elems, int dtype,
+                      const void *nodes, const int32_t *perm, const int64_t *color_offsets,
+                      int n_colors, double *w, void *stream);
+
+/* Per-element 4x4 consistent mass matrices |T_e|/20 (1 + delta_ij),
+ * (n_elems, 16) float64 -- the triplet values of Assemble.cpp:148-150. */
+int fm_mass_p1_element_matrices(int64_t n_elems, const int32_t *elems, int dtype,
+                                const void *nodes, double *vals, void *stream);
+
+/* ------------------------------------------------------------------------
+ * POD -- replaces POD.tensordot_modes_weights (cpp/feelmri/POD.h:17-23;
+ * POD.cpp:5-38): out (P, C) = modes (P, C, M) . weights (M), dtype f32/f64.
+ * ---------------------------------------------------------------------- */
+int fm_pod_tensordot(int dtype, int64_t P, int64_t C, int64_t M, const void *modes,
+                     const void *weights, void *out, void *stream);
+
+/* Node positions under POD motion (python/feelmri/Motion.py:240-254,
+ * :272-290, :303-309, :58-68): out (n_nodes, 3) row-major float64 =
+ * r0 + sum_r modes[:, r] w_r + trans for one coefficient row
+ * (n_modes + 3 doubles, device). */
+int fm_pod_positions(int64_t n_nodes, const double *packed_xyz, const float *modes,
+                     int n_modes, const double *coef, double *out, void *stream);
+
+/* Convert reference (P, 3, R) float32 modes to the (3R, P) SoA rows the
+ * signal and Bloch kernels read. */
+int fm_modes_pack(int64_t P, int R, const float *modes_pcr, float *modes_soa, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEELMRI_HIP_H */

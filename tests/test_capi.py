@@ -1,0 +1,69 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares.
+
+No compute calls: this runs on machines without a GPU.
+"""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        names.update(re.findall(r"\b(fm_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "fm_signal_execute" in names and "fm_bloch_execute" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    from feelmri_amd import _lib
+
+    lib = _lib.load()
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, f"missing exports: {missing}"
+    # the ctypes table covers the same set
+    assert sorted(_lib.exported_symbols()) == _declared()
+
+
+def test_error_string_is_available():
+    from feelmri_amd import _lib
+
+    lib = _lib.load()
+    assert isinstance(lib.fm_last_error(), bytes)
+    assert lib.fm_version() >= 1
+
+
+def test_argument_errors_do_not_need_a_gpu():
+    from feelmri_amd import _lib
+
+    with pytest.raises(_lib.FeelmriHipError, match="plan is NULL"):
+        _lib.call("fm_signal_execute", None, 0, 1, 1, None, None, 99, None, None, 0, None, None)
+
+
+def test_colouring_is_a_valid_partition():
+    import numpy as np
+
+    from feelmri_amd import _lib, phantoms
+
+    m = phantoms.box_mesh((0, 0, 0), (1, 1, 1), (6, 5, 4))
+    E = m.n_elems
+    perm = np.empty(E, np.int32)
+    offs = np.zeros(65, np.int64)
+    nc = ctypes.c_int()
+    el = np.ascontiguousarray(m.elems)
+    _lib.call("fm_mass_p1_color", E, m.n_nodes, el.ctypes.data_as(ctypes.c_void_p),
+              perm.ctypes.data_as(ctypes.c_void_p), offs.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nc))
+    assert sorted(perm.tolist()) == list(range(E))
+    for c in range(nc.value):
+        ids = el[perm[offs[c]:offs[c + 1]]].ravel()
+        assert len(ids) == len(set(ids.tolist())), "two elements of one colour share a node"

@@ -1,0 +1,162 @@
+"""Bloch, P1 mass and POD parity on the GPU against the FP64 oracle.
+
+Bloch FP64 relL2 <= 1e-10, FP32 <= 1e-4; mass and POD FP64 <= 1e-12.
+"""
+import numpy as np
+import pytest
+
+from feelmri_amd import Motion, phantoms, sequences
+
+pytestmark = pytest.mark.gpu
+GAMMA = sequences.GAMMA
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _bloch_inputs(n, T, seed=0):
+    rng = np.random.default_rng(seed)
+    r0 = rng.uniform(-0.05, 0.05, (n, 3))
+    T1 = rng.uniform(500, 1500, n)
+    T2 = rng.uniform(40, 100, n)
+    dB = rng.normal(0, 20 / 42.58e6 * 1e3, n)
+    rf = (rng.normal(size=T) + 1j * rng.normal(size=T)) * 2e-3
+    rf[rng.uniform(size=T) < 0.4] = 0
+    G = rng.normal(size=(T, 3)) * 5
+    dt = rng.choice([0.01, 0.01, 0.01, 0.02, 1.0], T)
+    mxy0 = (rng.normal(size=n) + 1j * rng.normal(size=n)) * 0.1
+    mz0 = rng.uniform(0.5, 1.0, n)
+    return r0, T1, T2, dB, rf, G, dt, mxy0, mz0
+
+
+@pytest.mark.parametrize("n,T", [(1000, 300), (257, 513), (1, 2), (300, 1)])
+def test_bloch_history(gpu, n, T):
+    from feelmri_amd import BlochSimulator
+    import oracle
+
+    r0, T1, T2, dB, rf, G, dt, mxy0, mz0 = _bloch_inputs(n, T, seed=n + T)
+    ref = oracle.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0, history=True)
+    for prec, tol in (("f64", 1e-10), ("f32", 1e-4)):
+        mxy, mz = BlochSimulator.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0,
+                                           precision=prec)
+        assert mxy.shape == (n, T) and mxy.dtype == np.complex64 and mz.dtype == np.float32
+        # outputs keep the reference's float32 dtypes: compare at float32 resolution
+        e1, e2 = rel_l2(mxy, ref[0]), rel_l2(mz, ref[1])
+        print(prec, n, T, f"{e1:.2e} {e2:.2e}")
+        assert e1 <= max(tol, 1e-6) and e2 <= max(tol, 1e-6)
+
+
+def test_bloch_final_state_f64_exact(gpu):
+    import torch
+
+    from feelmri_amd import device
+    import oracle
+
+    n, T = 3000, 700
+    r0, T1, T2, dB, rf, G, dt, mxy0, mz0 = _bloch_inputs(n, T, seed=7)
+    ref = oracle.solve_mri(r0, T1, T2, dB, 0.9, GAMMA, rf, G, dt, None, mxy0, mz0, history=False)
+    nodes = device.BlochNodes.from_arrays(r0, T1, T2, dB, device=gpu)
+    ev = device.EventTable(rf, G, dt, device=gpu)
+    mxy, mz = device.bloch(nodes, ev, 0.9, GAMMA, torch.from_numpy(mxy0).to(gpu), torch.from_numpy(mz0).to(gpu))
+    assert rel_l2(mxy.cpu().numpy(), ref[0]) <= 1e-10
+    assert rel_l2(mz.cpu().numpy(), ref[1]) <= 1e-10
+
+
+def _pod(n, cls=Motion.POD, seed=3):
+    rng = np.random.default_rng(seed)
+    times = np.linspace(0, 40.0, 10)
+    base = rng.normal(size=(n, 3, 3)) * 1e-3
+    coeff = np.stack([np.sin(2 * np.pi * times / 40.0 * (k + 1)) for k in range(3)], 0)
+    return cls(times=times, data=np.einsum("pcr,rt->pct", base, coeff), n_modes=3, is_periodic=True)
+
+
+@pytest.mark.parametrize("kind", ["pod", "velocity", "sum", "callable"])
+def test_bloch_moving(gpu, kind):
+    from feelmri_amd import BlochSimulator
+    import oracle
+
+    n, T = 800, 90
+    r0, T1, T2, dB, rf, G, dt, mxy0, mz0 = _bloch_inputs(n, T, seed=21)
+    resp = Motion.RespiratoryMotion(times=np.linspace(0, 100, 25), data=np.sin(np.linspace(0, 6, 25)) * 0.004,
+                                    is_periodic=True, direction=np.array([1, 1, 0]))
+    pod = _pod(n, Motion.PODVelocity if kind == "velocity" else Motion.POD)
+    m = {"pod": pod, "velocity": pod, "sum": resp + pod, "callable": (lambda tt: pod(tt) * 2.0)}[kind]
+    ref = oracle.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0, m, history=True)
+    mxy, mz = BlochSimulator.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0, m)
+    assert rel_l2(mxy, ref[0]) <= 1e-6 and rel_l2(mz, ref[1]) <= 1e-6
+
+
+def test_bloch_solver_spoiled_gre_chain(gpu):
+    from feelmri_amd.Bloch import BlochSolver
+    import oracle
+
+    n = 2000
+    rng = np.random.default_rng(5)
+    r0 = rng.uniform(-0.05, 0.05, (n, 3))
+    T1 = rng.uniform(500, 1500, n)
+    T2 = rng.uniform(40, 100, n)
+    dB = rng.normal(0, 1e-4, n)
+    blocks = sequences.spoiled_gre(flip_deg=15, TR=5.0, n_dummy=6, n_lines=4, rf_dur=0.2)
+    want_xy, want_z, _, _ = oracle.bloch_solver_solve([b.as_dict() for b in blocks], r0, T1, T2, dB, 1.0, GAMMA,
+                                                      np.zeros(n), np.ones(n))
+    solver = BlochSolver(blocks, r0, T1, T2, delta_B=dB)
+    got_xy, got_z = solver.solve()
+    assert got_xy.shape == want_xy.shape == (n, 4)
+    assert rel_l2(got_xy, want_xy) <= 1e-6 and rel_l2(got_z, want_z) <= 1e-6
+
+
+def test_lumped_mass_c0(gpu):
+    from feelmri_amd import Assemble
+    import oracle
+
+    m = phantoms.slab_mesh()
+    w = Assemble.lumped_mass(m.elems, m.nodes, as_vector=True)
+    ref = oracle.mass_p1_lumped(m.elems, m.nodes)
+    assert np.max(np.abs(w - ref) / ref) <= 1e-12
+    np.testing.assert_allclose(w.sum(), 1e-4, rtol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_consistent_mass_assembly(gpu, dtype):
+    from feelmri_amd import Assemble
+    import oracle
+
+    m = phantoms.ellipsoid_mesh(n=14)
+    nodes = m.nodes.astype(dtype)
+    M = Assemble.MassAssemble(m.elems, nodes, None, None)
+    ref = oracle.MassAssemble(m.elems, nodes.astype(np.float64))
+    diff = abs(M.astype(np.float64) - ref).max()
+    assert diff <= (1e-12 if dtype == np.float64 else 1e-6) * abs(ref).max()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_pod_tensordot(gpu, dtype):
+    from feelmri_amd import POD
+
+    rng = np.random.default_rng(9)
+    modes = rng.normal(size=(5000, 3, 10)).astype(dtype)
+    w = rng.normal(size=10).astype(dtype)
+    got = POD.tensordot_modes_weights(modes, w)
+    want = np.tensordot(modes.astype(np.float64), w.astype(np.float64), axes=([2], [0]))
+    assert got.dtype == dtype
+    np.testing.assert_allclose(got, want, rtol=1e-5 if dtype == np.float32 else 1e-12, atol=1e-5)
+
+
+def test_pod_positions_match_call(gpu):
+    import torch
+
+    from feelmri_amd import device
+
+    n = 3000
+    pod = _pod(n)
+    pod.update_timeshift(1.5)
+    r0 = np.random.default_rng(1).uniform(-0.05, 0.05, (n, 3))
+    modes = device.modes_soa(torch.from_numpy(Motion.motion_modes(pod)).to(gpu))
+    coef = torch.from_numpy(Motion.motion_coefficients(pod, [7.0])[0]).to(gpu)
+    xyz = torch.from_numpy(np.ascontiguousarray(r0.T)).to(gpu)
+    got = device.pod_positions(xyz, modes, coef).cpu().numpy()
+    np.testing.assert_allclose(got, r0 + pod(7.0), rtol=1e-6, atol=1e-9)

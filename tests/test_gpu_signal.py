@@ -1,0 +1,186 @@
+"""Signal parity on the GPU: HIP direct sum vs the FP64 oracle.
+
+Tolerances (relative L2 over the whole output):
+  FP64 form   <= 1e-10  (the bar is 1e-6; same arithmetic up to rounding)
+  mixed form  <= 1e-4   (FP64 range reduction + FP32 sincos / accumulation)
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from feelmri_amd import Motion, trajectories
+
+pytestmark = pytest.mark.gpu
+
+TOL_F64 = 1e-10
+TOL_MIXED = 1e-4
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def _nodes(n, v=1, seed=0):
+    rng = np.random.default_rng(seed)
+    r0 = rng.uniform(-0.06, 0.06, (n, 3))
+    phi = rng.normal(0, 2 * np.pi * 20 / 1000, n)
+    T2 = rng.uniform(40, 100, n)
+    mxy = (rng.normal(size=(n, v)) + 1j * rng.normal(size=(n, v))) * 0.1
+    w = rng.uniform(0.5, 1.5, n) * 1e-6
+    return r0, phi, T2, mxy, w
+
+
+def _check(kloc, t, n=3000, v=1, seed=0, M="diag", motion=None, precisions=("f64", "mixed"), gpu=None):
+    from feelmri_amd import MRI
+    import oracle
+
+    r0, phi, T2, mxy, w = _nodes(n, v, seed)
+    if M == "diag":
+        Mm = sp.diags(w)
+    elif M == "csr":
+        Mm = (sp.random(n, n, density=3.0 / max(n, 1), random_state=seed) + sp.diags(w * 1e6)) * 1e-6
+        Mm = sp.csc_matrix(Mm)
+    else:
+        Mm = None
+    ref = oracle.Signal(0, Mm, kloc, t, r0, phi, T2, mxy, motion)
+    errs = {}
+    for prec in precisions:
+        got = MRI.Signal(0, Mm, kloc, t, r0, phi, T2, mxy, motion, precision=prec)
+        assert got.shape == ref.shape
+        errs[prec] = rel_l2(got, ref)
+    print("relL2", {k: f"{e:.2e}" for k, e in errs.items()})
+    if "f64" in errs:
+        assert errs["f64"] <= TOL_F64, errs
+    if "mixed" in errs:
+        assert errs["mixed"] <= TOL_MIXED, errs
+    return errs
+
+
+def test_cartesian_small_diag(gpu):
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (16, 12, 4), t_first=1.2, t_readout=2.0)
+    _check(kloc, t, n=3000, v=1)
+
+
+@pytest.mark.parametrize("v", [2, 3, 5])
+def test_multiple_encodings(gpu, v):
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (12, 8, 2), t_first=0.5)
+    _check(kloc, t, n=1500, v=v, seed=v)
+
+
+def test_consistent_mass_csr(gpu):
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (10, 10, 1))
+    _check(kloc, t, n=2000, M="csr")
+
+
+def test_identity_mass(gpu):
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (10, 10, 1))
+    _check(kloc, t, n=700, M=None)
+
+
+def test_radial(gpu):
+    kloc, t = trajectories.radial(0.24, 128, 16, t_first=0.3)
+    _check(kloc, t, n=5000, seed=3)
+
+
+def test_spiral_small_groups_general_kernel(gpu):
+    kloc, t = trajectories.spiral(0.24, 64, 200, 8, t_first=0.2)
+    _check(kloc, t, n=4000, seed=4)
+
+
+def test_random_times_per_sample(gpu):
+    rng = np.random.default_rng(5)
+    kloc = tuple(rng.uniform(-200, 200, (30, 7, 2)) for _ in range(3))
+    t = rng.uniform(0.0, 4.0, (30, 7, 2))
+    _check(kloc, t, n=2600, v=2, seed=5)
+
+
+@pytest.mark.parametrize("n", [1, 255, 257, 4099])
+def test_ragged_node_counts(gpu, n):
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (8, 6, 1))
+    _check(kloc, t, n=n, seed=n)
+
+
+def test_single_sample(gpu):
+    kloc = tuple(np.array([[[v]]]) for v in (13.0, -7.0, 2.0))
+    t = np.array([[[1.5]]])
+    _check(kloc, t, n=1000)
+
+
+def test_zero_nodes_gives_zeros(gpu):
+    from feelmri_amd import MRI
+
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (4, 4, 1))
+    z = np.zeros((0, 3))
+    out = MRI.Signal(0, None, kloc, t, z, np.zeros(0), np.zeros(0), np.zeros((0, 1), np.complex128))
+    assert out.shape == (4, 4, 1, 1) and not np.any(out)
+
+
+def test_float32_inputs_use_float_overload(gpu):
+    from feelmri_amd import MRI
+    import oracle
+
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (8, 8, 1), dtype=np.float32)
+    r0, phi, T2, mxy, w = _nodes(2000)
+    args = (r0.astype(np.float32), phi.astype(np.float32), T2.astype(np.float32), mxy.astype(np.complex64))
+    out = MRI.Signal(0, sp.diags(w.astype(np.float32)), kloc, t, *args)
+    assert out.dtype == np.complex64
+    ref = oracle.Signal(0, sp.diags(w.astype(np.float32)), kloc, t, *args)
+    assert rel_l2(out, ref) <= TOL_MIXED
+
+
+def _pod(n, cls=Motion.POD, seed=11):
+    rng = np.random.default_rng(seed)
+    nt = 10
+    times = np.linspace(0, 40.0, nt)
+    base = rng.normal(size=(n, 3, 3)) * 1e-3
+    coeff = np.stack([np.sin(2 * np.pi * times / 40.0 * (k + 1)) for k in range(3)], 0)
+    data = np.einsum("pcr,rt->pct", base, coeff)
+    return cls(times=times, data=data, n_modes=3, is_periodic=True)
+
+
+@pytest.mark.parametrize("kind", ["pod", "velocity", "resp", "sum", "callable"])
+def test_moving_nodes(gpu, kind):
+    n = 2500
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (12, 6, 2), t_first=0.4)
+    resp = Motion.RespiratoryMotion(times=np.linspace(0, 100, 25), data=np.sin(np.linspace(0, 6, 25)) * 0.004,
+                                    is_periodic=True, direction=np.array([0, 1, 0]))
+    if kind == "pod":
+        m = _pod(n)
+    elif kind == "velocity":
+        m = _pod(n, Motion.PODVelocity)
+    elif kind == "resp":
+        m = resp
+    elif kind == "sum":
+        m = resp + _pod(n)
+    else:
+        pod = _pod(n)
+        m = lambda tt: pod(tt) * 1.5  # noqa: E731
+    if hasattr(m, "update_timeshift"):
+        m.update_timeshift(3.0)
+    _check(kloc, t, n=n, motion=m, seed=12)
+
+
+def test_c0_slab_full_kspace(gpu):
+    """61 206-node slab, 64 x 64 Cartesian, FOV 0.24 m, 2 ms readout."""
+    from feelmri_amd import phantoms
+
+    m = phantoms.slab_mesh()
+    tis = phantoms.draw_tissue(m.n_nodes, seed=1000)
+    import oracle
+
+    w = oracle.mass_p1_lumped(m.elems, m.nodes)
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (64, 64, 1), t_first=1.25, t_readout=2.0)
+    rng = np.random.default_rng(1000)
+    mxy = (1j * np.sin(np.deg2rad(15)) * tis.M0 * rng.uniform(0.05, 0.2, m.n_nodes))[:, None]
+    from feelmri_amd import MRI
+
+    ref = oracle.Signal(0, sp.diags(w), kloc, t, m.nodes, tis.delta_omega, tis.T2star, mxy)
+    errs = {}
+    for prec in ("f64", "mixed"):
+        got = MRI.Signal(0, sp.diags(w), kloc, t, m.nodes, tis.delta_omega, tis.T2star, mxy, precision=prec)
+        errs[prec] = rel_l2(got, ref)
+    print("c0 relL2", errs)
+    assert errs["f64"] <= TOL_F64 and errs["mixed"] <= TOL_MIXED
