@@ -1,0 +1,258 @@
+#!/usr/bin/env python
+"""Signal-generation throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mixed|f64]
+                    [--method auto|direct|cartesian] [--config c1|c0|c2-radial|c2-spiral]
+
+A step is one evaluation of the k-space signal of the workload (all
+samples, all encodings) from node data already resident in HBM.  The
+default workload is c1: a 512 000-node P1 box phantom with spherical
+inclusions, 3-D Cartesian 128 x 128 x 32, 3 ms readout, after a spoiled-GRE
+Bloch solve (15 deg, TR 5 ms, 16 dummy TRs) and lumped-mass weighting,
+all run on the GPU during set-up.
+
+Prints one JSON line: node*k-sample/s of the chosen form, its relative L2
+error against the FP64 GPU form over the full k-space and against the FP64
+oracle on a sample of k-space points, the roofline of the dominant kernel
+(timed live with HIP events on the launch stream) and the FP64 CPU oracle
+timed on the host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAKS = {  # MI355X published dense peaks
+    "bf16_mfma_tflops": 2500.0,
+    "fp64_mfma_tflops": 78.6,
+    "fp64_valu_tflops": 78.6,
+    "fp32_valu_tflops": 157.3,
+    "hbm_gbs": 8000.0,
+}
+
+
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b) -> float:
+    import numpy as np
+
+    nb = float(np.linalg.norm(b))
+    return float(np.linalg.norm(a - b)) / (nb if nb > 0 else 1.0)
+
+
+def build_workload(name: str, scale: float):
+    from feelmri_amd import workloads
+
+    if name == "c0":
+        return workloads.c0(scale)
+    if name == "c1":
+        return workloads.c1(scale)
+    if name.startswith("c2"):
+        return workloads.c2(scale, kind=name.split("-")[1] if "-" in name else "radial")
+    raise SystemExit(f"unknown config {name}")
+
+
+def prepare_nodes(wl, dev):
+    """Lumped weights, spoiled-GRE steady state, packed signal nodes (GPU)."""
+    import numpy as np
+    import torch
+
+    from feelmri_amd import device
+    from feelmri_amd.Bloch import BlochSolver
+
+    m, tis = wl.mesh, wl.tissue
+    col = device.P1Colouring(m.elems, m.n_nodes, device=dev)
+    nodes_d = torch.from_numpy(m.nodes).to(dev)
+    w = device.lumped_mass(col, nodes_d)
+    solver = BlochSolver(wl.blocks, m.nodes, tis.T1, tis.T2, delta_B=tis.delta_B_mT, dev=dev)
+    mxy_store, _ = solver.solve_device()
+    mxy = mxy_store[:, -1].to(torch.complex128) * torch.from_numpy(tis.M0).to(dev)
+    mxy = mxy.reshape(-1, 1).repeat(1, wl.n_enc)
+    nodes = device.SignalNodes.from_arrays(nodes_d, torch.from_numpy(tis.delta_omega).to(dev),
+                                           torch.from_numpy(tis.T2star).to(dev), mxy, weights=w, device=dev)
+    torch.cuda.synchronize()
+    return nodes, w, mxy
+
+
+def time_form(plan, nodes, form, steps, warmup):
+    import torch
+
+    from feelmri_amd import device
+
+    out = torch.empty((plan.n_samples, nodes.n_enc), dtype=torch.complex128, device=plan.device)
+    ws = torch.empty(max(plan.workspace_bytes(form, nodes.n_nodes, nodes.n_enc), 1), dtype=torch.uint8,
+                     device=plan.device)
+    for _ in range(warmup):
+        plan.execute(nodes, precision=form, out=out, workspace=ws)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        plan.execute(nodes, precision=form, out=out, workspace=ws, stream=stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_event = ev0.elapsed_time(ev1) / steps
+    return out, ms_event, wall / steps * 1e3
+
+
+def cpu_baseline(wl, w, mxy_host, target_s: float):
+    """FP64 oracle (fast-math build, OpenMP) on a subset of readout indices."""
+    import numpy as np
+
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    kx, ky, kz = (np.asarray(k) for k in wl.kloc)
+    t = np.asarray(wl.t)
+    n_ro = t.shape[0]
+    mm = w[:, None] * mxy_host
+    args = (wl.mesh.nodes, wl.tissue.delta_omega, wl.tissue.T2star, mm)
+    # probe: one readout index, a few lines
+    sl = (slice(0, 1), slice(0, min(8, t.shape[1])), slice(None))
+    t0 = time.perf_counter()
+    oracle.signal_samples(kx[sl], ky[sl], kz[sl], t[sl], *args, n_threads=threads, fast=True)
+    probe = time.perf_counter() - t0
+    per_sample = probe / t[sl].size
+    per_ro = per_sample * t[0].size
+    n_pick = int(max(1, min(n_ro, round(target_s / max(per_ro, 1e-9)))))
+    idx = np.unique(np.linspace(0, n_ro - 1, n_pick).round().astype(int))
+    t0 = time.perf_counter()
+    oracle.signal_samples(kx[idx], ky[idx], kz[idx], t[idx], *args, n_threads=threads, fast=True)
+    dt = time.perf_counter() - t0
+    work = wl.n_nodes * t[idx].size * wl.n_enc
+    return {"value": work / dt, "unit": "node*k-sample/s", "cores": threads, "kind": "port",
+            "sample": f"{idx.size} of {n_ro} readout indices ({t[idx].size} k-samples x {wl.n_nodes} nodes), "
+                      f"{dt:.1f} s on {threads} threads, fast-math FP64 oracle"}
+
+
+def oracle_subset_error(wl, w, mxy_host, out, n_pick=96):
+    import numpy as np
+
+    import oracle
+
+    rng = np.random.default_rng(7)
+    S = wl.n_samples
+    idx = np.sort(rng.choice(S, size=min(n_pick, S), replace=False))
+    kx, ky, kz = (np.asarray(k).ravel()[idx] for k in wl.kloc)
+    t = np.asarray(wl.t).ravel()[idx]
+    ref = oracle.signal_samples(kx, ky, kz, t, wl.mesh.nodes, wl.tissue.delta_omega, wl.tissue.T2star,
+                                w[:, None] * mxy_host, n_threads=min(16, os.cpu_count() or 1))
+    return rel_l2(out[idx], ref)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--form", default="mixed", choices=["mixed", "f64"])
+    ap.add_argument("--method", default="auto", choices=["auto", "direct", "cartesian"])
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison form")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "node-sharded k-space partial sums would be combined with one reduce, see DESIGN.md",
+              file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from feelmri_amd import device
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    log(f"building workload {args.config} (scale {args.scale})")
+    wl = build_workload(args.config, args.scale)
+    log(f"{wl.n_nodes} nodes, {wl.mesh.n_elems} tets, {wl.n_samples} k-samples; preparing on GPU")
+    nodes, w, mxy = prepare_nodes(wl, dev)
+    plan = device.SignalPlan(wl.kloc, wl.t, method=args.method, device=dev)
+    log(f"plan: method {plan.method}, {plan.n_tgroups} distinct t")
+    work = wl.node_ksamples
+
+    out, ms_ev, ms_wall = time_form(plan, nodes, args.form, args.steps, args.warmup)
+    value = work / (ms_ev * 1e-3)
+    log(f"{args.form}: {ms_ev:.3f} ms/step (events), {ms_wall:.3f} ms/step (wall), {value:.3e} node*k/s")
+    out_h = out.cpu().numpy()
+
+    forms = {args.form: {"ms_per_step": ms_ev, "node_ksamples_per_s": value}}
+    err_f64 = None
+    if not args.no_f64 and args.form != "f64":
+        ref_steps = 1 if plan.method == "direct" else max(1, args.steps)
+        out64, ms64, _ = time_form(plan, nodes, "f64", ref_steps, 1)
+        forms["f64"] = {"ms_per_step": ms64, "node_ksamples_per_s": work / (ms64 * 1e-3)}
+        err_f64 = rel_l2(out_h, out64.cpu().numpy())
+        log(f"f64: {ms64:.3f} ms/step; mixed vs f64 relL2 {err_f64:.2e}; speed-up {ms64 / ms_ev:.2f}x")
+    w_h = w.cpu().numpy()
+    mxy_h = mxy.cpu().numpy()
+    err_oracle = oracle_subset_error(wl, w_h, mxy_h, out_h)
+    log(f"relL2 vs FP64 oracle on sampled k-points: {err_oracle:.2e}")
+
+    # roofline of the dominant kernel
+    if plan.method == "cartesian":
+        bound, unit = "mfma", "TFLOP/s"
+        flops = 8.0 * work
+        peak = PEAKS["bf16_mfma_tflops"] if args.form == "mixed" else PEAKS["fp64_mfma_tflops"]
+        dom = "k_cart_gemm"
+    else:
+        bound, unit = "valu", "TFLOP/s"
+        flops = (8.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
+        peak = PEAKS["fp64_valu_tflops"]
+        dom = "k_signal_direct"
+    achieved = flops / (ms_ev * 1e-3) / 1e12
+    roof = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": None,
+            "per_unit": "8 flops per node*k-sample*encoding (complex MAC)" if plan.method == "cartesian"
+            else "8 flops + 8 per encoding and 2 transcendentals per node*k-sample"}
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
+        cpu = cpu_baseline(wl, w_h, mxy_h, args.cpu_seconds)
+        log(f"CPU: {cpu['value']:.3e} node*k/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "node*k-sample/s at relL2<=1e-4 vs FP64 oracle",
+        "value": value,
+        "unit": "node*k-sample/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_ev,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"mixed": "bf16x3->f32 (cartesian) / f32 sincos + f64 phase (direct)", "f64": "f64"}[args.form]
+        if plan.method == "cartesian" else ({"mixed": "f32 (f64 phase)", "f64": "f64"}[args.form]),
+        "data": "synthetic",
+        "config": {"workload": wl.info["desc"], "config": wl.name, "nodes": wl.n_nodes,
+                   "k_samples": wl.n_samples, "encodings": wl.n_enc, "method": plan.method, "form": args.form},
+        "relL2_vs_f64_gpu": err_f64,
+        "relL2_vs_oracle_sampled": err_oracle,
+        "forms": forms,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

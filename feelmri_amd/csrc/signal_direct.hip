@@ -89,7 +89,9 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     for (int j = 0; j < kDirectSPL; ++j) {
         const int idx = tid + j * kDirectThreads;
         valid[j] = idx < tile.count;
-        sid[j] = valid[j] ? perm[tile.offset + idx] : 0;
+        // grouped tiles index the t-ordered permutation; consecutive tiles
+        // cover samples [offset, offset + count) directly
+        sid[j] = valid[j] ? (SHARED_T ? perm[tile.offset + idx] : tile.offset + idx) : 0;
         const double4 k = ks[sid[j]];
         kxs[j] = k.x;
         kys[j] = k.y;
@@ -283,12 +285,18 @@ struct DirectLaunch {
     int64_t nodes_per_split;
 };
 
-static DirectLaunch direct_launch_shape(const fm_signal_plan *p, int64_t N)
+static int64_t direct_tiles(const fm_signal_plan *p, bool grouped)
+{
+    return grouped ? p->n_tiles_grouped : p->n_tiles_consec;
+}
+
+static DirectLaunch direct_launch_shape(const fm_signal_plan *p, int64_t N, bool grouped)
 {
     // Aim for ~8 workgroups per CU over the whole grid; keep >= 2048 nodes
     // per split so the staging cost stays amortised.
     const int64_t target = 8 * (int64_t)device_cu_count();
-    int64_t splits = cdiv(target, p->n_tiles > 0 ? p->n_tiles : 1);
+    const int64_t nt = direct_tiles(p, grouped);
+    int64_t splits = cdiv(target, nt > 0 ? nt : 1);
     const int64_t max_splits = cdiv(N, 2048);
     if (splits > max_splits) splits = max_splits;
     if (splits < 1) splits = 1;
@@ -299,47 +307,54 @@ static DirectLaunch direct_launch_shape(const fm_signal_plan *p, int64_t N)
 }
 
 template <int VB, bool MIXED>
-static void launch_direct_vb(const fm_signal_plan *p, bool motion, dim3 grid, const double *packed,
-                             int64_t N, int V, int v0, const float *modes, int R,
-                             const double *mcoef, int64_t nps, double2 *dst, int write_final,
+static void launch_direct_vb(const fm_signal_plan *p, bool grouped, bool motion, dim3 grid,
+                             const double *packed, int64_t N, int V, int v0, const float *modes,
+                             int R, const double *mcoef, int64_t nps, double2 *dst, int write_final,
                              hipStream_t st)
 {
-    if (p->shared_t) {
+    if (grouped) {
         if (motion)
             k_signal_direct<VB, MIXED, true, true><<<grid, kDirectThreads, 0, st>>>(
-                packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles, p->d_tg_t, modes, R, mcoef, nps,
-                dst, p->n_samples, write_final);
+                packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles_grouped, p->d_tg_t, modes, R, mcoef,
+                nps, dst, p->n_samples, write_final);
         else
             k_signal_direct<VB, MIXED, true, false><<<grid, kDirectThreads, 0, st>>>(
-                packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles, p->d_tg_t, modes, R, mcoef, nps,
-                dst, p->n_samples, write_final);
+                packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles_grouped, p->d_tg_t, modes, R, mcoef,
+                nps, dst, p->n_samples, write_final);
     } else {
         k_signal_direct<VB, MIXED, false, false><<<grid, kDirectThreads, 0, st>>>(
-            packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles, p->d_tg_t, modes, R, mcoef, nps, dst,
-            p->n_samples, write_final);
+            packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles_consec, p->d_tg_t, modes, R, mcoef, nps,
+            dst, p->n_samples, write_final);
     }
 }
 
 template <bool MIXED>
-static void launch_direct(const fm_signal_plan *p, bool motion, dim3 grid, const double *packed,
-                          int64_t N, int V, int v0, int vb, const float *modes, int R,
-                          const double *mcoef, int64_t nps, double2 *dst, int write_final,
+static void launch_direct(const fm_signal_plan *p, bool grouped, bool motion, dim3 grid,
+                          const double *packed, int64_t N, int V, int v0, int vb, const float *modes,
+                          int R, const double *mcoef, int64_t nps, double2 *dst, int write_final,
                           hipStream_t st)
 {
+#define FM_LD(VBV) launch_direct_vb<VBV, MIXED>(p, grouped, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st)
     switch (vb) {
-    case 1: launch_direct_vb<1, MIXED>(p, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st); break;
-    case 2: launch_direct_vb<2, MIXED>(p, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st); break;
-    case 3: launch_direct_vb<3, MIXED>(p, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st); break;
-    default: launch_direct_vb<4, MIXED>(p, motion, grid, packed, N, V, v0, modes, R, mcoef, nps, dst, write_final, st); break;
+    case 1: FM_LD(1); break;
+    case 2: FM_LD(2); break;
+    case 3: FM_LD(3); break;
+    default: FM_LD(4); break;
     }
+#undef FM_LD
 }
 
 static size_t direct_workspace(const fm_signal_plan *p, int64_t N, int V)
 {
     if (N <= 0 || p->n_samples == 0) return 0;
-    const DirectLaunch L = direct_launch_shape(p, N);
-    if (L.n_splits <= 1) return 0;
-    return (size_t)L.n_splits * (size_t)p->n_samples * (size_t)V * sizeof(double2);
+    // the larger of the two tilings' needs (motion forces the grouped one)
+    size_t need = 0;
+    for (int g = 0; g < 2; ++g) {
+        const DirectLaunch L = direct_launch_shape(p, N, g == 1);
+        if (L.n_splits > 1)
+            need = std::max(need, (size_t)L.n_splits * (size_t)p->n_samples * (size_t)V * sizeof(double2));
+    }
+    return need;
 }
 
 static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int V,
@@ -353,25 +368,23 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
         return FM_OK;
     }
     const bool motion = modes != nullptr && R > 0;
-    if (motion && !p->shared_t) {
-        set_error("moving nodes need samples grouped by time (plan has no shared-t tiling)");
-        return FM_ERR_UNSUPPORTED;
-    }
-    const DirectLaunch L = direct_launch_shape(p, N);
-    const size_t need = direct_workspace(p, N, V);
+    // positions depend on t, so moving nodes always take the t-grouped tiles
+    const bool grouped = motion || p->prefer_grouped;
+    const DirectLaunch L = direct_launch_shape(p, N, grouped);
+    const size_t need = L.n_splits > 1 ? (size_t)L.n_splits * (size_t)S * (size_t)V * sizeof(double2) : 0;
     if (need > ws_bytes) {
         set_error("signal workspace too small: need %zu bytes, got %zu", need, ws_bytes);
         return FM_ERR_WORKSPACE;
     }
     double2 *dst = L.n_splits > 1 ? reinterpret_cast<double2 *>(ws) : reinterpret_cast<double2 *>(out);
     const int write_final = L.n_splits > 1 ? 0 : 1;
-    dim3 grid((unsigned)p->n_tiles, (unsigned)L.n_splits);
+    dim3 grid((unsigned)direct_tiles(p, grouped), (unsigned)L.n_splits);
     for (int v0 = 0; v0 < V; v0 += kMaxVB) {
         const int vb = (V - v0 < kMaxVB) ? V - v0 : kMaxVB;
         if (precision == FM_PREC_F64)
-            launch_direct<false>(p, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
+            launch_direct<false>(p, grouped, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
         else
-            launch_direct<true>(p, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
+            launch_direct<true>(p, grouped, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
         FM_LAUNCH_CHECK("k_signal_direct");
     }
     if (L.n_splits > 1) {
@@ -442,13 +455,13 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         p->method = ok ? FM_METHOD_CARTESIAN : FM_METHOD_DIRECT;
     }
 
-    // direct-path tiling: shared-t tiles when groups are large enough to
-    // fill most of a 512-sample tile, else plain consecutive tiles
+    // Tilings.  Grouped: samples ordered by t-group, each tile within one
+    // group, so the t-dependent node factors are staged once per tile.
+    // Consecutive: plain runs of samples, per-sample t.  Static nodes prefer
+    // the grouped tiles when groups fill most of a 512-sample tile.
     std::vector<int32_t> perm((size_t)S);
-    std::vector<SignalTile> tiles;
-    const double avg_group = p->n_tgroups ? (double)S / (double)p->n_tgroups : 0.0;
-    p->shared_t = avg_group >= 64.0 || S == 0;
-    if (p->shared_t) {
+    std::vector<SignalTile> tiles_g, tiles_c;
+    {
         std::vector<int64_t> count((size_t)p->n_tgroups + 1, 0);
         for (int64_t s = 0; s < S; ++s) count[(size_t)p->sample_group[(size_t)s] + 1]++;
         for (int64_t g = 0; g < p->n_tgroups; ++g) count[(size_t)g + 1] += count[(size_t)g];
@@ -457,15 +470,16 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         for (int64_t g = 0; g < p->n_tgroups; ++g) {
             for (int64_t o = count[(size_t)g]; o < count[(size_t)g + 1]; o += kDirectTile) {
                 const int64_t c = std::min<int64_t>(kDirectTile, count[(size_t)g + 1] - o);
-                tiles.push_back({(int32_t)g, (int32_t)o, (int32_t)c, 0});
+                tiles_g.push_back({(int32_t)g, (int32_t)o, (int32_t)c, 0});
             }
         }
-    } else {
-        for (int64_t s = 0; s < S; ++s) perm[(size_t)s] = (int32_t)s;
         for (int64_t o = 0; o < S; o += kDirectTile)
-            tiles.push_back({-1, (int32_t)o, (int32_t)std::min<int64_t>(kDirectTile, S - o), 0});
+            tiles_c.push_back({-1, (int32_t)o, (int32_t)std::min<int64_t>(kDirectTile, S - o), 0});
     }
-    p->n_tiles = (int64_t)tiles.size();
+    const double avg_group = p->n_tgroups ? (double)S / (double)p->n_tgroups : 0.0;
+    p->prefer_grouped = avg_group >= 64.0;
+    p->n_tiles_grouped = (int64_t)tiles_g.size();
+    p->n_tiles_consec = (int64_t)tiles_c.size();
 
     std::vector<double4> ks((size_t)S);
     for (int64_t s = 0; s < S; ++s)
@@ -480,11 +494,13 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
     if (S > 0) {
         if ((e = hipMalloc(&p->d_ks, sizeof(double4) * (size_t)S)) != hipSuccess) return fail(e, "hipMalloc");
         if ((e = hipMalloc(&p->d_perm, sizeof(int32_t) * (size_t)S)) != hipSuccess) return fail(e, "hipMalloc");
-        if ((e = hipMalloc(&p->d_tiles, sizeof(SignalTile) * tiles.size())) != hipSuccess) return fail(e, "hipMalloc");
+        if ((e = hipMalloc(&p->d_tiles_grouped, sizeof(SignalTile) * tiles_g.size())) != hipSuccess) return fail(e, "hipMalloc");
+        if ((e = hipMalloc(&p->d_tiles_consec, sizeof(SignalTile) * tiles_c.size())) != hipSuccess) return fail(e, "hipMalloc");
         if ((e = hipMalloc(&p->d_tg_t, sizeof(double) * (size_t)p->n_tgroups)) != hipSuccess) return fail(e, "hipMalloc");
         if ((e = hipMemcpy(p->d_ks, ks.data(), sizeof(double4) * (size_t)S, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if ((e = hipMemcpy(p->d_perm, perm.data(), sizeof(int32_t) * (size_t)S, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
-        if ((e = hipMemcpy(p->d_tiles, tiles.data(), sizeof(SignalTile) * tiles.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        if ((e = hipMemcpy(p->d_tiles_grouped, tiles_g.data(), sizeof(SignalTile) * tiles_g.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        if ((e = hipMemcpy(p->d_tiles_consec, tiles_c.data(), sizeof(SignalTile) * tiles_c.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if ((e = hipMemcpy(p->d_tg_t, p->tg_t.data(), sizeof(double) * (size_t)p->n_tgroups, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if (p->method == FM_METHOD_CARTESIAN) {
             const int rc = cartesian_upload(p);
@@ -503,7 +519,8 @@ extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
     if (!p) return FM_OK;
     (void)hipFree(p->d_ks);
     (void)hipFree(p->d_perm);
-    (void)hipFree(p->d_tiles);
+    (void)hipFree(p->d_tiles_grouped);
+    (void)hipFree(p->d_tiles_consec);
     (void)hipFree(p->d_tg_t);
     cartesian_free(p);
     delete p;

@@ -38,15 +38,18 @@ struct fm_signal_plan {
     int device = 0;
     int64_t n_meas = 0, n_lines = 0, n_kz = 0, n_samples = 0;
     int method = FM_METHOD_DIRECT;
-    bool shared_t = true;         // tiles group samples of equal t
+    bool prefer_grouped = true;   // static nodes: use the t-grouped tiles
     int64_t n_tgroups = 0;
     std::vector<double> tg_t;     // distinct t, first-appearance order
     std::vector<int32_t> sample_group;
-    int64_t n_tiles = 0;
+    // Two tilings of the samples: t-grouped tiles (each tile one distinct t;
+    // always used for moving nodes) and consecutive tiles (any t per sample).
+    int64_t n_tiles_grouped = 0, n_tiles_consec = 0;
     // device tables
     double4 *d_ks = nullptr;      // per sample: kx', ky', kz' (cycles/m, pi_f-scaled), t
-    int32_t *d_perm = nullptr;    // samples grouped by tile
-    fm::SignalTile *d_tiles = nullptr;
+    int32_t *d_perm = nullptr;    // samples ordered by t-group (grouped tiles index it)
+    fm::SignalTile *d_tiles_grouped = nullptr;
+    fm::SignalTile *d_tiles_consec = nullptr;  // offsets are sample indices
     double *d_tg_t = nullptr;
     fm::CartesianTables cart;
 };

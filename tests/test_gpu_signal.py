@@ -33,7 +33,22 @@ def _nodes(n, v=1, seed=0):
     return r0, phi, T2, mxy, w
 
 
-def _check(kloc, t, n=3000, v=1, seed=0, M="diag", motion=None, precisions=("f64", "mixed"), gpu=None):
+def descriptor_callable(obj):
+    """t -> displacement formed in FP64 from the device tables of a motion
+    object (float32 modes and coefficients, exactly what the kernels read)."""
+    modes = Motion.motion_modes(obj).astype(np.float64)
+    R = Motion.n_modes(obj)
+
+    def f(t):
+        c = Motion.motion_coefficients(obj, [t])[0]
+        d = np.tensordot(modes, c[:R], axes=([2], [0])) if R else 0.0
+        return d + c[R:]
+
+    return f
+
+
+def _check(kloc, t, n=3000, v=1, seed=0, M="diag", motion=None, precisions=("f64", "mixed"), gpu=None,
+           oracle_motion=None):
     from feelmri_amd import MRI
     import oracle
 
@@ -45,7 +60,7 @@ def _check(kloc, t, n=3000, v=1, seed=0, M="diag", motion=None, precisions=("f64
         Mm = sp.csc_matrix(Mm)
     else:
         Mm = None
-    ref = oracle.Signal(0, Mm, kloc, t, r0, phi, T2, mxy, motion)
+    ref = oracle.Signal(0, Mm, kloc, t, r0, phi, T2, mxy, oracle_motion if oracle_motion else motion)
     errs = {}
     for prec in precisions:
         got = MRI.Signal(0, Mm, kloc, t, r0, phi, T2, mxy, motion, precision=prec)
@@ -160,7 +175,19 @@ def test_moving_nodes(gpu, kind):
         m = lambda tt: pod(tt) * 1.5  # noqa: E731
     if hasattr(m, "update_timeshift"):
         m.update_timeshift(3.0)
-    _check(kloc, t, n=n, motion=m, seed=12)
+    # kernel parity: the oracle moves the nodes with the same float32 tables
+    exact = descriptor_callable(m) if kind != "callable" else None
+    _check(kloc, t, n=n, motion=m, seed=12, oracle_motion=exact)
+    if kind != "callable":
+        # semantics: against the object's own __call__ (float32 trajectories,
+        # so the FP64 bar is the float32 displacement resolution)
+        from feelmri_amd import MRI
+        import oracle
+
+        r0, phi, T2, mxy, w = _nodes(n, 1, 12)
+        ref = oracle.Signal(0, sp.diags(w), kloc, t, r0, phi, T2, mxy, m)
+        got = MRI.Signal(0, sp.diags(w), kloc, t, r0, phi, T2, mxy, m, precision="f64")
+        assert rel_l2(got, ref) <= 1e-5
 
 
 def test_c0_slab_full_kspace(gpu):
