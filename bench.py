@@ -193,13 +193,20 @@ def main() -> int:
     out_h = out.cpu().numpy()
 
     forms = {args.form: {"ms_per_step": ms_ev, "node_ksamples_per_s": value}}
-    err_f64 = None
+    err_f64 = err_img = None
     if not args.no_f64 and args.form != "f64":
         ref_steps = 1 if plan.method == "direct" else max(1, args.steps)
         out64, ms64, _ = time_form(plan, nodes, "f64", ref_steps, 1)
         forms["f64"] = {"ms_per_step": ms64, "node_ksamples_per_s": work / (ms64 * 1e-3)}
-        err_f64 = rel_l2(out_h, out64.cpu().numpy())
-        log(f"f64: {ms64:.3f} ms/step; mixed vs f64 relL2 {err_f64:.2e}; speed-up {ms64 / ms_ev:.2f}x")
+        out64_h = out64.cpu().numpy()
+        err_f64 = rel_l2(out_h, out64_h)
+        if wl.name in ("c0", "c1"):
+            from feelmri_amd.recon import image_rel_l2
+
+            shape = (*wl.t.shape, wl.n_enc)
+            err_img = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape))
+        log(f"f64: {ms64:.3f} ms/step; {args.form} vs f64 relL2 {err_f64:.2e} (image {err_img}); "
+            f"speed-up {ms64 / ms_ev:.2f}x")
     w_h = w.cpu().numpy()
     mxy_h = mxy.cpu().numpy()
     err_oracle = oracle_subset_error(wl, w_h, mxy_h, out_h)
@@ -245,6 +252,7 @@ def main() -> int:
         "config": {"workload": wl.info["desc"], "config": wl.name, "nodes": wl.n_nodes,
                    "k_samples": wl.n_samples, "encodings": wl.n_enc, "method": plan.method, "form": args.form},
         "relL2_vs_f64_gpu": err_f64,
+        "image_relL2_vs_f64_gpu": err_img,
         "relL2_vs_oracle_sampled": err_oracle,
         "forms": forms,
         "roofline": roof,

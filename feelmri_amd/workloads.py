@@ -69,6 +69,88 @@ def c1(scale: float = 1.0, n_kspace=(128, 128, 32)) -> Workload:
                                    f"{'x'.join(map(str, n_kspace))} k-space, 3 ms readout"))
 
 
+@dataclass
+class FlowWorkload(Workload):
+    """c3: per cardiac phase, one Bloch block per velocity encoding (bipolar
+    gradients, nodes moving with the Poiseuille velocity), then one Signal
+    call with all encodings and the POD displacement as motion."""
+    n_phases: int = 20
+    period: float = 1000.0                 # ms
+    venc: float = 1.5                      # m/s
+    motion: object = None                  # POD (displacement), R modes
+    velocity: object = None                # PODVelocity (Poiseuille field)
+    enc_blocks: list = field(default_factory=list)   # one Block per encoding
+
+    @property
+    def node_ksamples(self) -> int:
+        return self.n_nodes * self.n_samples * self.n_enc * self.n_phases
+
+
+def _smooth_modes(nodes: np.ndarray, n_modes: int, rng, amp: float = 5e-4, scale: float = 0.05):
+    """Random smooth displacement fields: one low-frequency sinusoid of
+    position per component and mode, amplitude ``amp`` metres."""
+    P = nodes.shape[0]
+    phi = np.empty((P, 3, n_modes))
+    for r in range(n_modes):
+        for c in range(3):
+            k = rng.normal(0, 1.0 / scale, 3)
+            phi[:, c, r] = amp * np.sin(2 * np.pi * (nodes @ k) * 0.1 + rng.uniform(0, 2 * np.pi))
+    return phi
+
+
+def c3(scale: float = 1.0, n_phases: int = 20, n_modes: int = 10, kspace=(96, 96, 24)) -> FlowWorkload:
+    from . import Motion
+
+    n_len = max(8, int(round(400 * scale)))
+    n_cross = max(6, int(round(80 * scale)))
+    m = phantoms.curved_tube_mesh(radius=0.02, length=0.20, n_len=n_len, n_cross=n_cross)
+    rng = np.random.default_rng(1003)
+    tis = phantoms.draw_tissue(m.n_nodes, seed=1003, r2s_range=(10.0, 50.0))
+    period = 1000.0
+    times = np.linspace(0.0, period, n_phases + 1)[:-1]
+    # Poiseuille velocity along the arc tangent, peak 1 m/s, cardiac waveform
+    s, u, v = m.info["s"], m.info["u"], m.info["v"]
+    rho2 = (u * u + v * v) / m.info["radius"] ** 2
+    ang = s / m.info["bend_radius"]
+    tangent = np.stack([np.cos(ang), np.sin(ang), np.zeros_like(ang)], axis=1)
+    wave = np.maximum(0.0, np.sin(2 * np.pi * times / period)) ** 2 + 0.1
+    speed = (1.0 - np.clip(rho2, 0, 1))[:, None] * wave[None, :]          # m/s
+    vel = tangent[:, :, None] * speed[:, None, :] * 1e-3                    # m/ms
+    # the field is one spatial profile times a waveform: one POD mode holds it
+    velocity = Motion.PODVelocity(times=times, data=vel, n_modes=1, is_periodic=True)
+    # node displacement from random smooth modes with periodic weights
+    base = _smooth_modes(m.nodes, n_modes, rng)
+    n_modes = min(n_modes, max(1, n_phases // 2))  # harmonics below Nyquist of the phase grid
+    base = base[:, :, :n_modes]
+    h = np.arange(1, (n_modes + 1) // 2 + 1)  # enough harmonics for rank n_modes in time
+    coef = rng.normal(size=(n_modes, 2, h.size))
+    a = (coef[:, 0, :, None] * np.cos(2 * np.pi * h[None, :, None] * times / period) +
+         coef[:, 1, :, None] * np.sin(2 * np.pi * h[None, :, None] * times / period)).sum(1)
+    disp = np.einsum("pcr,rt->pct", base, a)
+    motion = Motion.POD(times=times, data=disp, n_modes=n_modes, is_periodic=True)
+    # encodings: reference + bipolar along x, y, z; VENC 1.5 m/s
+    venc = 1.5
+    ramp, flat = 0.1, 0.4
+    area_lobe = flat + ramp                       # ms, per unit amplitude
+    sep = flat + 2 * ramp                         # lobe centre separation
+    gmax = np.pi / (sequences.GAMMA * area_lobe * sep * venc * 1e-3)   # mT/m
+    enc_blocks = []
+    for d in range(4):
+        pulse = sequences.HardPulse(start=0.0, dur=0.2, flip=np.deg2rad(15.0))
+        grads = []
+        if d > 0:
+            grads = [sequences.Trapezoid(d - 1, 0.25, gmax, ramp, flat),
+                     sequences.Trapezoid(d - 1, 0.25 + sep, -gmax, ramp, flat)]
+        enc_blocks.append(sequences.make_block(0.0, 0.2, pulses=[pulse], gradients=grads, dt_rf=0.01,
+                                               dt_gr=0.01, dt=1.0, store=True))
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.06), kspace, t_first=1.0, t_readout=2.0)
+    wl = FlowWorkload("c3", m, tis, kloc, t, n_enc=4, n_phases=n_phases, period=period, venc=venc,
+                      motion=motion, velocity=velocity, enc_blocks=enc_blocks,
+                      info=dict(desc=f"4-D flow, {m.n_nodes}-node curved tube, {n_modes} POD modes, 4 encodings, "
+                                     f"{'x'.join(map(str, kspace))} x {n_phases} phases, direct per-line sum"))
+    return wl
+
+
 def c2(scale: float = 1.0, kind: str = "radial") -> Workload:
     n = max(6, int(round(124 * scale)))
     m = phantoms.ellipsoid_mesh(n=n)
