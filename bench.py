@@ -164,6 +164,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison form")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device-to-device copy of this many MiB first (a known-size kernel for "
+                         "scaling hardware traffic counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -179,6 +182,14 @@ def main() -> int:
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration_mib} MiB read + written")
+        del src, dst
     log(f"building workload {args.config} (scale {args.scale})")
     wl = build_workload(args.config, args.scale)
     log(f"{wl.n_nodes} nodes, {wl.mesh.n_elems} tets, {wl.n_samples} k-samples; preparing on GPU")

@@ -1,0 +1,135 @@
+// FP64 Cartesian GEMM on v_mfma_f64_16x16x4_f64 (the plain FP64 form of
+// the factorised signal; see signal_gemm.hip for the factorisation).
+#include "cart_gemm.h"
+
+namespace fm {
+
+// ---------------------------------------------------------------------------
+// FP64 GEMM: v_mfma_f64_16x16x4_f64
+// ---------------------------------------------------------------------------
+// Workgroup tile: RB*16 rows x 64 columns; wave w owns columns [16w, 16w+16).
+// A 16-node step is 4 MFMA k-steps of 4 nodes; lane l supplies row/column
+// l%16 and node 4*st + l/16 of k-step st.
+template <int RB>
+__global__ __launch_bounds__(kGemmThreads) void k_cart_gemm_f64(
+    const double *__restrict__ E, const double2 *__restrict__ Ey, const double2 *__restrict__ Ez, int64_t Np,
+    int64_t Mp, int64_t M, int64_t NC, int64_t n_kzV, int64_t n_rt, int64_t n_ct, int64_t n_splits,
+    int64_t steps_per_split, int64_t ksteps, double2 *__restrict__ part)
+{
+    constexpr int kStage = RB * 2 * 4 * 64;            // doubles per stage
+    constexpr int kPer = kStage / kGemmThreads;        // = 2 RB
+    __shared__ double sE[2][kStage];
+    int64_t split, rt, ct;
+    gemm_tile_ids(n_rt, n_ct, n_splits, split, rt, ct);
+    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+    const int64_t nrb = Mp / 16;
+    const int64_t k0 = split * steps_per_split;
+    const int64_t k1 = (k0 + steps_per_split < ksteps) ? k0 + steps_per_split : ksteps;
+
+    const int64_t col = ct * 64 + wave * 16 + (lane % 16);
+    const int64_t colc = col < NC ? col : NC - 1;
+    const int64_t j = colc / n_kzV, kv = colc % n_kzV;
+    const double2 *eyrow = Ey + j * Np + lane / 16;
+    const double2 *ezrow = Ez + kv * Np + lane / 16;
+
+    f64x4 acc_re[RB], acc_im[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc_re[r][q] = acc_im[r][q] = 0.0;
+
+    double pre[kPer];
+    double2 cy[4], cz[4], ny[4], nz[4];
+    if (k0 < k1) {
+        const double *src = E + (k0 * nrb + rt * RB) * 2 * 4 * 64;
+#pragma unroll
+        for (int e = 0; e < kPer; ++e) sE[0][tid + e * kGemmThreads] = src[tid + e * kGemmThreads];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            cy[st] = eyrow[k0 * 16 + 4 * st];
+            cz[st] = ezrow[k0 * 16 + 4 * st];
+        }
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int64_t ks = k0; ks < k1; ++ks) {
+        double br[4], bi[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+            br[st] = cy[st].x * cz[st].x - cy[st].y * cz[st].y;
+            bi[st] = cy[st].x * cz[st].y + cy[st].y * cz[st].x;
+        }
+        const bool has_next = ks + 1 < k1;
+        if (has_next) {
+            const double *src = E + ((ks + 1) * nrb + rt * RB) * 2 * 4 * 64;
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) pre[e] = src[tid + e * kGemmThreads];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                ny[st] = eyrow[(ks + 1) * 16 + 4 * st];
+                nz[st] = ezrow[(ks + 1) * 16 + 4 * st];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const double *t = &sE[buf][r * 2 * 4 * 64];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const double er = t[(0 * 4 + st) * 64 + lane];
+                const double ei = t[(1 * 4 + st) * 64 + lane];
+                acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, br[st], acc_re[r], 0, 0, 0);
+                acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, bi[st], acc_im[r], 0, 0, 0);
+                acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ei, bi[st], acc_re[r], 0, 0, 0);
+                acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, br[st], acc_im[r], 0, 0, 0);
+            }
+        }
+        if (has_next) {
+#pragma unroll
+            for (int e = 0; e < kPer; ++e) sE[buf ^ 1][tid + e * kGemmThreads] = pre[e];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                cy[st] = ny[st];
+                cz[st] = nz[st];
+            }
+        }
+        __syncthreads();
+        buf ^= 1;
+    }
+    // acc register q of lane l holds row l/16 + 4q, column l%16
+    // (v_mfma_f64_16x16x4_f64 result layout, checked on gfx950)
+    if (col < NC) {
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t row = (rt * RB + r) * 16 + (lane / 16) + 4 * q;
+                if (row < M) part[(split * M + row) * NC + col] = make_double2(acc_re[r][q], acc_im[r][q]);
+            }
+    }
+}
+
+template <int RB>
+static void launch_f64(const GemmShape &s, char *ws, int64_t n_kzV, hipStream_t st)
+{
+    const unsigned grid = (unsigned)(s.n_rt * s.n_ct * s.n_splits);
+    k_cart_gemm_f64<RB><<<grid, kGemmThreads, 0, st>>>(
+        reinterpret_cast<const double *>(ws + s.off_E), reinterpret_cast<const double2 *>(ws + s.off_Ey),
+        reinterpret_cast<const double2 *>(ws + s.off_Ez), s.Np, s.Mp, s.M, s.NC, n_kzV, s.n_rt, s.n_ct, s.n_splits,
+        s.steps_per_split, s.ksteps, reinterpret_cast<double2 *>(ws + s.off_part));
+}
+
+void launch_cart_gemm_f64(const GemmShape &s, char *ws, int64_t n_kzV, hipStream_t st)
+{
+    switch (s.rb_per_wave) {
+    case 1: launch_f64<1>(s, ws, n_kzV, st); break;
+    case 2: launch_f64<2>(s, ws, n_kzV, st); break;
+    case 3: launch_f64<3>(s, ws, n_kzV, st); break;
+    case 4: launch_f64<4>(s, ws, n_kzV, st); break;
+    case 5: launch_f64<5>(s, ws, n_kzV, st); break;
+    case 6: launch_f64<6>(s, ws, n_kzV, st); break;
+    case 7: launch_f64<7>(s, ws, n_kzV, st); break;
+    default: launch_f64<8>(s, ws, n_kzV, st); break;
+    }
+}
+
+}  // namespace fm

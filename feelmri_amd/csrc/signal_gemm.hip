@@ -28,16 +28,10 @@
 #include <cmath>
 #include <vector>
 
-#include "signal_plan.h"
+#include "cart_gemm.h"
 
 namespace fm {
 
-typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
-typedef __attribute__((ext_vector_type(16))) float f32x16;
-typedef __attribute__((ext_vector_type(4))) double f64x4;
-
-constexpr int kGemmThreads = 256;  // 4 waves
-constexpr int kXcds = 8;
 
 // ---------------------------------------------------------------------------
 // Sampling analysis (host)
@@ -126,21 +120,6 @@ void cartesian_free(fm_signal_plan *p)
     c.d_F = c.d_G = c.d_H = c.d_t = nullptr;
 }
 
-// ---------------------------------------------------------------------------
-// Geometry of one execute
-// ---------------------------------------------------------------------------
-struct GemmShape {
-    bool mixed;
-    int64_t N, Np;         // nodes, padded to the K step of a stage
-    int64_t M, Mp;         // readout rows, padded to the row block
-    int64_t NC, NCp;       // output columns lines*kz*V, padded to the column block
-    int rb_per_wave;       // row blocks per wave
-    int64_t n_rt, n_ct;    // row tiles, column tiles (per workgroup)
-    int64_t ksteps;        // K stages in total
-    int64_t n_splits, steps_per_split;
-    // workspace offsets (bytes)
-    size_t off_E, off_Ey, off_Ez, off_part, total;
-};
 
 // mixed: row block 32, K stage 16 nodes, column block 32 per wave (4 per WG)
 // f64:   row block 16, K stage 16 nodes (4 MFMA k-steps), column block 16 per wave
@@ -280,6 +259,57 @@ __global__ void k_cart_E(const double *__restrict__ packed, int64_t N, int64_t N
     }
 }
 
+// Mixed E table, one thread per lane fragment: the thread owns row i and
+// the 8 nodes n0..n0+7 that lane (i%32 + 32g) of tile (kstep, i/32) feeds to
+// the MFMA, and writes each of the 4 planes as one 16-byte store (threads of
+// a wave write 1 KB contiguous).  Phase in FP64 with range reduction, then
+// the hardware FP32 sine/cosine and exp2.
+__global__ void k_cart_E_bf16(const double *__restrict__ packed, int64_t N, int64_t Np, int64_t M, int64_t Mp,
+                              const double *__restrict__ F, const double *__restrict__ tr, uint4 *__restrict__ E)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over Mp * Np / 8
+    if (t >= Mp * (Np / 8)) return;
+    const int lane = (int)(t % 64);
+    const int64_t tile = t / 64;
+    const int64_t nrb = Mp / 32;
+    const int64_t rb = tile % nrb, kb = tile / nrb;
+    const int64_t i = rb * 32 + (lane % 32);
+    const int64_t n0 = kb * 16 + 8 * (lane / 32);
+    unsigned short h[4][8];
+    const bool row_ok = i < M;
+    const double fx = row_ok ? F[3 * i] : 0.0, fy = row_ok ? F[3 * i + 1] : 0.0, fz = row_ok ? F[3 * i + 2] : 0.0;
+    const double ti = row_ok ? tr[i] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int64_t n = n0 + q;
+        float er = 0.0f, ei = 0.0f;
+        if (row_ok && n < N) {
+            const double x = packed[n], y = packed[N + n], z = packed[2 * N + n];
+            const double phi = packed[3 * N + n], r2 = packed[4 * N + n];
+            const double u = fma(fx, x, fma(fy, y, fz * z)) - phi * ti * (1.0 / kTwoPi);
+            const float rf = (float)(u - __builtin_rint(u));
+            const float d = __builtin_amdgcn_exp2f((float)(-ti * r2 * 1.4426950408889634));
+            er = d * __builtin_amdgcn_cosf(rf);
+            ei = -d * __builtin_amdgcn_sinf(rf);
+        }
+        const __bf16 erh = (__bf16)er, eih = (__bf16)ei;
+        const __bf16 erl = (__bf16)(er - (float)erh), eil = (__bf16)(ei - (float)eih);
+        h[0][q] = __builtin_bit_cast(unsigned short, erh);
+        h[1][q] = __builtin_bit_cast(unsigned short, erl);
+        h[2][q] = __builtin_bit_cast(unsigned short, eih);
+        h[3][q] = __builtin_bit_cast(unsigned short, eil);
+    }
+#pragma unroll
+    for (int pl = 0; pl < 4; ++pl) {
+        uint4 v;
+        v.x = (unsigned)h[pl][0] | ((unsigned)h[pl][1] << 16);
+        v.y = (unsigned)h[pl][2] | ((unsigned)h[pl][3] << 16);
+        v.z = (unsigned)h[pl][4] | ((unsigned)h[pl][5] << 16);
+        v.w = (unsigned)h[pl][6] | ((unsigned)h[pl][7] << 16);
+        E[(tile * 4 + pl) * 64 + lane] = v;
+    }
+}
+
 // Ey[j][n] = exp(-2 pi i G_j . x_n) ; Ezb[k*V+v][n] = b[n,v] exp(-2 pi i H_k . x_n)
 template <bool MIXED>
 __global__ void k_cart_yz(const double *__restrict__ packed, int64_t N, int64_t Np, int V, int64_t n_ln,
@@ -318,257 +348,7 @@ __global__ void k_cart_yz(const double *__restrict__ packed, int64_t N, int64_t 
     }
 }
 
-// Map the hardware workgroup id to (split, row tile, column tile) so that
-// the workgroups of one split run on one XCD (ids are dealt round-robin over
-// the 8 XCDs).
-__device__ inline void gemm_tile_ids(int64_t n_rt, int64_t n_ct, int64_t n_splits, int64_t &split, int64_t &rt,
-                                     int64_t &ct)
-{
-    const int64_t total = n_rt * n_ct * n_splits;
-    const int64_t h = blockIdx.x;
-    int64_t L = h;
-    if (total % kXcds == 0) L = (h % kXcds) * (total / kXcds) + h / kXcds;
-    const int64_t per_split = n_rt * n_ct;
-    split = L / per_split;
-    const int64_t rem = L % per_split;
-    rt = rem / n_ct;
-    ct = rem % n_ct;
-}
 
-// ---------------------------------------------------------------------------
-// Mixed GEMM: bf16 hi+lo, v_mfma_f32_32x32x16_bf16
-// ---------------------------------------------------------------------------
-// Workgroup tile: RB*32 readout rows x 128 columns; wave w owns columns
-// [32w, 32w+32) of the tile and all RB row blocks.  Per 16-node step the E
-// tiles (RB x 4 planes, 64 lanes x 16 B each) are double-buffered in LDS;
-// the next step's E tiles and Ey/Ezb values are loaded into registers while
-// the current step's MFMAs run (one barrier per step).
-template <int RB>
-__global__ __launch_bounds__(kGemmThreads) void k_cart_gemm_bf16(
-    const uint4 *__restrict__ E, const float2 *__restrict__ Ey, const float2 *__restrict__ Ez, int64_t Np,
-    int64_t Mp, int64_t M, int64_t NC, int64_t n_kzV, int64_t n_rt, int64_t n_ct, int64_t n_splits,
-    int64_t steps_per_split, int64_t ksteps, float2 *__restrict__ part)
-{
-    constexpr int kStage = RB * 4 * 64;                // uint4 per stage
-    constexpr int kPer = kStage / kGemmThreads;        // = RB
-    __shared__ uint4 sE[2][kStage];
-    int64_t split, rt, ct;
-    gemm_tile_ids(n_rt, n_ct, n_splits, split, rt, ct);
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    const int64_t nrb = Mp / 32;
-    const int64_t k0 = split * steps_per_split;
-    const int64_t k1 = (k0 + steps_per_split < ksteps) ? k0 + steps_per_split : ksteps;
-
-    // this lane's output column (clamped for padding lanes; results discarded)
-    const int64_t col = ct * 128 + wave * 32 + (lane % 32);
-    const int64_t colc = col < NC ? col : NC - 1;
-    const int64_t j = colc / n_kzV, kv = colc % n_kzV;
-    const float4 *ey4 = reinterpret_cast<const float4 *>(Ey + j * Np) + 4 * (lane / 32);
-    const float4 *ez4 = reinterpret_cast<const float4 *>(Ez + kv * Np) + 4 * (lane / 32);
-
-    f32x16 acc_re[RB], acc_im[RB];
-#pragma unroll
-    for (int r = 0; r < RB; ++r)
-#pragma unroll
-        for (int q = 0; q < 16; ++q) acc_re[r][q] = acc_im[r][q] = 0.0f;
-
-    uint4 pre[kPer];
-    float4 cy[4], cz[4], ny[4], nz[4];
-    if (k0 < k1) {
-        const uint4 *src = E + (k0 * nrb + rt * RB) * 4 * 64;
-#pragma unroll
-        for (int e = 0; e < kPer; ++e) sE[0][tid + e * kGemmThreads] = src[tid + e * kGemmThreads];
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            cy[h] = ey4[k0 * 8 + h];   // 16 nodes = 8 float4 per step
-            cz[h] = ez4[k0 * 8 + h];
-        }
-    }
-    __syncthreads();
-    int buf = 0;
-    for (int64_t ks = k0; ks < k1; ++ks) {
-        // A operand for this lane's column and nodes ks*16 + 8*(lane/32) + q
-        bf16x8 brh, brl, bih, bil;
-#pragma unroll
-        for (int h = 0; h < 4; ++h) {
-            const float4 a = cy[h], b = cz[h];
-            const float r0 = a.x * b.x - a.y * b.y, i0 = a.x * b.y + a.y * b.x;
-            const float r1 = a.z * b.z - a.w * b.w, i1 = a.z * b.w + a.w * b.z;
-            const __bf16 r0h = (__bf16)r0, r1h = (__bf16)r1, i0h = (__bf16)i0, i1h = (__bf16)i1;
-            brh[2 * h] = r0h;
-            brh[2 * h + 1] = r1h;
-            bih[2 * h] = i0h;
-            bih[2 * h + 1] = i1h;
-            brl[2 * h] = (__bf16)(r0 - (float)r0h);
-            brl[2 * h + 1] = (__bf16)(r1 - (float)r1h);
-            bil[2 * h] = (__bf16)(i0 - (float)i0h);
-            bil[2 * h + 1] = (__bf16)(i1 - (float)i1h);
-        }
-        const bool has_next = ks + 1 < k1;
-        if (has_next) {
-            const uint4 *src = E + ((ks + 1) * nrb + rt * RB) * 4 * 64;
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) pre[e] = src[tid + e * kGemmThreads];
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                ny[h] = ey4[(ks + 1) * 8 + h];
-                nz[h] = ez4[(ks + 1) * 8 + h];
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-            const uint4 *t = &sE[buf][r * 4 * 64];
-            const uint4 urh = t[0 * 64 + lane], url = t[1 * 64 + lane];
-            const uint4 uih = t[2 * 64 + lane], uil = t[3 * 64 + lane];
-            const unsigned sg = 0x80008000u;
-            const uint4 unh = make_uint4(uih.x ^ sg, uih.y ^ sg, uih.z ^ sg, uih.w ^ sg);
-            const uint4 unl = make_uint4(uil.x ^ sg, uil.y ^ sg, uil.z ^ sg, uil.w ^ sg);
-            const bf16x8 erh = __builtin_bit_cast(bf16x8, urh), erl = __builtin_bit_cast(bf16x8, url);
-            const bf16x8 eih = __builtin_bit_cast(bf16x8, uih), eil = __builtin_bit_cast(bf16x8, uil);
-            const bf16x8 enh = __builtin_bit_cast(bf16x8, unh), enl = __builtin_bit_cast(bf16x8, unl);
-            // Re += Er*Ar - Ei*Ai ; Im += Er*Ai + Ei*Ar  (each as hh + hl + lh)
-            acc_re[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(erh, brh, acc_re[r], 0, 0, 0);
-            acc_im[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(erh, bih, acc_im[r], 0, 0, 0);
-            acc_re[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(erh, brl, acc_re[r], 0, 0, 0);
-            acc_im[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(erh, bil, acc_im[r], 0, 0, 0);
-            acc_re[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(erl, brh, acc_re[r], 0, 0, 0);
-            acc_im[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(erl, bih, acc_im[r], 0, 0, 0);
-            acc_re[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(enh, bih, acc_re[r], 0, 0, 0);
-            acc_im[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(eih, brh, acc_im[r], 0, 0, 0);
-            acc_re[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(enh, bil, acc_re[r], 0, 0, 0);
-            acc_im[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(eih, brl, acc_im[r], 0, 0, 0);
-            acc_re[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(enl, bih, acc_re[r], 0, 0, 0);
-            acc_im[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(eil, brh, acc_im[r], 0, 0, 0);
-        }
-        if (has_next) {
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) sE[buf ^ 1][tid + e * kGemmThreads] = pre[e];
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                cy[h] = ny[h];
-                cz[h] = nz[h];
-            }
-        }
-        __syncthreads();
-        buf ^= 1;
-    }
-
-    // acc register q of lane l holds row 8*(q/4) + 4*(l/32) + q%4 and
-    // column l%32 of its 32x32 block
-    if (col < NC) {
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int64_t row = (rt * RB + r) * 32 + 8 * (q / 4) + 4 * (lane / 32) + (q % 4);
-                if (row < M) part[(split * M + row) * NC + col] = make_float2(acc_re[r][q], acc_im[r][q]);
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// FP64 GEMM: v_mfma_f64_16x16x4_f64
-// ---------------------------------------------------------------------------
-// Workgroup tile: RB*16 rows x 64 columns; wave w owns columns [16w, 16w+16).
-// A 16-node step is 4 MFMA k-steps of 4 nodes; lane l supplies row/column
-// l%16 and node 4*st + l/16 of k-step st.
-template <int RB>
-__global__ __launch_bounds__(kGemmThreads) void k_cart_gemm_f64(
-    const double *__restrict__ E, const double2 *__restrict__ Ey, const double2 *__restrict__ Ez, int64_t Np,
-    int64_t Mp, int64_t M, int64_t NC, int64_t n_kzV, int64_t n_rt, int64_t n_ct, int64_t n_splits,
-    int64_t steps_per_split, int64_t ksteps, double2 *__restrict__ part)
-{
-    constexpr int kStage = RB * 2 * 4 * 64;            // doubles per stage
-    constexpr int kPer = kStage / kGemmThreads;        // = 2 RB
-    __shared__ double sE[2][kStage];
-    int64_t split, rt, ct;
-    gemm_tile_ids(n_rt, n_ct, n_splits, split, rt, ct);
-    const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-    const int64_t nrb = Mp / 16;
-    const int64_t k0 = split * steps_per_split;
-    const int64_t k1 = (k0 + steps_per_split < ksteps) ? k0 + steps_per_split : ksteps;
-
-    const int64_t col = ct * 64 + wave * 16 + (lane % 16);
-    const int64_t colc = col < NC ? col : NC - 1;
-    const int64_t j = colc / n_kzV, kv = colc % n_kzV;
-    const double2 *eyrow = Ey + j * Np + lane / 16;
-    const double2 *ezrow = Ez + kv * Np + lane / 16;
-
-    f64x4 acc_re[RB], acc_im[RB];
-#pragma unroll
-    for (int r = 0; r < RB; ++r)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) acc_re[r][q] = acc_im[r][q] = 0.0;
-
-    double pre[kPer];
-    double2 cy[4], cz[4], ny[4], nz[4];
-    if (k0 < k1) {
-        const double *src = E + (k0 * nrb + rt * RB) * 2 * 4 * 64;
-#pragma unroll
-        for (int e = 0; e < kPer; ++e) sE[0][tid + e * kGemmThreads] = src[tid + e * kGemmThreads];
-#pragma unroll
-        for (int st = 0; st < 4; ++st) {
-            cy[st] = eyrow[k0 * 16 + 4 * st];
-            cz[st] = ezrow[k0 * 16 + 4 * st];
-        }
-    }
-    __syncthreads();
-    int buf = 0;
-    for (int64_t ks = k0; ks < k1; ++ks) {
-        double br[4], bi[4];
-#pragma unroll
-        for (int st = 0; st < 4; ++st) {
-            br[st] = cy[st].x * cz[st].x - cy[st].y * cz[st].y;
-            bi[st] = cy[st].x * cz[st].y + cy[st].y * cz[st].x;
-        }
-        const bool has_next = ks + 1 < k1;
-        if (has_next) {
-            const double *src = E + ((ks + 1) * nrb + rt * RB) * 2 * 4 * 64;
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) pre[e] = src[tid + e * kGemmThreads];
-#pragma unroll
-            for (int st = 0; st < 4; ++st) {
-                ny[st] = eyrow[(ks + 1) * 16 + 4 * st];
-                nz[st] = ezrow[(ks + 1) * 16 + 4 * st];
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-            const double *t = &sE[buf][r * 2 * 4 * 64];
-#pragma unroll
-            for (int st = 0; st < 4; ++st) {
-                const double er = t[(0 * 4 + st) * 64 + lane];
-                const double ei = t[(1 * 4 + st) * 64 + lane];
-                acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, br[st], acc_re[r], 0, 0, 0);
-                acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, bi[st], acc_im[r], 0, 0, 0);
-                acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ei, bi[st], acc_re[r], 0, 0, 0);
-                acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, br[st], acc_im[r], 0, 0, 0);
-            }
-        }
-        if (has_next) {
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) sE[buf ^ 1][tid + e * kGemmThreads] = pre[e];
-#pragma unroll
-            for (int st = 0; st < 4; ++st) {
-                cy[st] = ny[st];
-                cz[st] = nz[st];
-            }
-        }
-        __syncthreads();
-        buf ^= 1;
-    }
-    // acc register q of lane l holds row 4*(l/16) + q, column l%16
-    if (col < NC) {
-#pragma unroll
-        for (int r = 0; r < RB; ++r)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int64_t row = (rt * RB + r) * 16 + 4 * (lane / 16) + q;
-                if (row < M) part[(split * M + row) * NC + col] = make_double2(acc_re[r][q], acc_im[r][q]);
-            }
-    }
-}
 
 template <typename P>
 __global__ void k_cart_reduce(const P *__restrict__ part, int64_t n_splits, int64_t MNC, double2 *__restrict__ out)
@@ -582,26 +362,6 @@ __global__ void k_cart_reduce(const P *__restrict__ part, int64_t n_splits, int6
         im += (double)v.y;
     }
     out[i] = make_double2(re, im);
-}
-
-template <int RB>
-static void launch_bf16(const GemmShape &s, char *ws, int64_t n_kzV, hipStream_t st)
-{
-    const unsigned grid = (unsigned)(s.n_rt * s.n_ct * s.n_splits);
-    k_cart_gemm_bf16<RB><<<grid, kGemmThreads, 0, st>>>(
-        reinterpret_cast<const uint4 *>(ws + s.off_E), reinterpret_cast<const float2 *>(ws + s.off_Ey),
-        reinterpret_cast<const float2 *>(ws + s.off_Ez), s.Np, s.Mp, s.M, s.NC, n_kzV, s.n_rt, s.n_ct, s.n_splits,
-        s.steps_per_split, s.ksteps, reinterpret_cast<float2 *>(ws + s.off_part));
-}
-
-template <int RB>
-static void launch_f64(const GemmShape &s, char *ws, int64_t n_kzV, hipStream_t st)
-{
-    const unsigned grid = (unsigned)(s.n_rt * s.n_ct * s.n_splits);
-    k_cart_gemm_f64<RB><<<grid, kGemmThreads, 0, st>>>(
-        reinterpret_cast<const double *>(ws + s.off_E), reinterpret_cast<const double2 *>(ws + s.off_Ey),
-        reinterpret_cast<const double2 *>(ws + s.off_Ez), s.Np, s.Mp, s.M, s.NC, n_kzV, s.n_rt, s.n_ct, s.n_splits,
-        s.steps_per_split, s.ksteps, reinterpret_cast<double2 *>(ws + s.off_part));
 }
 
 int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, const double *packed, void *wsv,
@@ -624,8 +384,8 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
     {
         const int64_t tot = s.Mp * s.Np;
         if (s.mixed)
-            k_cart_E<true><<<(unsigned)cdiv(tot, 256), 256, 0, st>>>(packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t,
-                                                                    ws + s.off_E);
+            k_cart_E_bf16<<<(unsigned)cdiv(tot / 8, 256), 256, 0, st>>>(packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t,
+                                                                       reinterpret_cast<uint4 *>(ws + s.off_E));
         else
             k_cart_E<false><<<(unsigned)cdiv(tot, 256), 256, 0, st>>>(packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t,
                                                                      ws + s.off_E);
@@ -640,27 +400,13 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
         FM_LAUNCH_CHECK("k_cart_yz");
     }
     if (s.mixed) {
-        switch (s.rb_per_wave) {
-        case 1: launch_bf16<1>(s, ws, n_kzV, st); break;
-        case 2: launch_bf16<2>(s, ws, n_kzV, st); break;
-        case 3: launch_bf16<3>(s, ws, n_kzV, st); break;
-        default: launch_bf16<4>(s, ws, n_kzV, st); break;
-        }
+        launch_cart_gemm_bf16(s, ws, n_kzV, st);
         FM_LAUNCH_CHECK("k_cart_gemm_bf16");
         const int64_t MNC = s.M * s.NC;
         k_cart_reduce<float2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(
             reinterpret_cast<const float2 *>(ws + s.off_part), s.n_splits, MNC, reinterpret_cast<double2 *>(out));
     } else {
-        switch (s.rb_per_wave) {
-        case 1: launch_f64<1>(s, ws, n_kzV, st); break;
-        case 2: launch_f64<2>(s, ws, n_kzV, st); break;
-        case 3: launch_f64<3>(s, ws, n_kzV, st); break;
-        case 4: launch_f64<4>(s, ws, n_kzV, st); break;
-        case 5: launch_f64<5>(s, ws, n_kzV, st); break;
-        case 6: launch_f64<6>(s, ws, n_kzV, st); break;
-        case 7: launch_f64<7>(s, ws, n_kzV, st); break;
-        default: launch_f64<8>(s, ws, n_kzV, st); break;
-        }
+        launch_cart_gemm_f64(s, ws, n_kzV, st);
         FM_LAUNCH_CHECK("k_cart_gemm_f64");
         const int64_t MNC = s.M * s.NC;
         k_cart_reduce<double2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(
