@@ -95,6 +95,7 @@ def time_form(plan, nodes, form, steps, warmup):
         plan.execute(nodes, precision=form, out=out, workspace=ws)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
+    plan.set_timing(steps)  # HIP events around the dominant kernel, on its stream
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record(stream)
@@ -104,7 +105,22 @@ def time_form(plan, nodes, form, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_event = ev0.elapsed_time(ev1) / steps
-    return out, ms_event, wall / steps * 1e3
+    k_ms, k_n = plan.kernel_ms()
+    plan.set_timing(0)
+    return out, ms_event, wall / steps * 1e3, (k_ms / k_n if k_n else None)
+
+
+def load_traffic(kernel: str, config: str, form: str):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel of
+    known size), or None when no summary matches."""
+    path = ROOT / "profiles" / "traffic.json"
+    try:
+        table = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+    rec = table.get(f"{kernel}/{config}/{form}")
+    return None if rec is None else rec.get("bytes_per_launch")
 
 
 def cpu_baseline(wl, w, mxy_host, target_s: float):
@@ -198,17 +214,18 @@ def main() -> int:
     log(f"plan: method {plan.method}, {plan.n_tgroups} distinct t")
     work = wl.node_ksamples
 
-    out, ms_ev, ms_wall = time_form(plan, nodes, args.form, args.steps, args.warmup)
+    out, ms_ev, ms_wall, ms_kernel = time_form(plan, nodes, args.form, args.steps, args.warmup)
     value = work / (ms_ev * 1e-3)
     log(f"{args.form}: {ms_ev:.3f} ms/step (events), {ms_wall:.3f} ms/step (wall), {value:.3e} node*k/s")
     out_h = out.cpu().numpy()
 
-    forms = {args.form: {"ms_per_step": ms_ev, "node_ksamples_per_s": value}}
+    forms = {args.form: {"ms_per_step": ms_ev, "node_ksamples_per_s": value, "kernel_ms": ms_kernel}}
     err_f64 = err_img = None
     if not args.no_f64 and args.form != "f64":
         ref_steps = 1 if plan.method == "direct" else max(1, args.steps)
-        out64, ms64, _ = time_form(plan, nodes, "f64", ref_steps, 1)
-        forms["f64"] = {"ms_per_step": ms64, "node_ksamples_per_s": work / (ms64 * 1e-3)}
+        out64, ms64, _, ms64_kernel = time_form(plan, nodes, "f64", ref_steps, 1)
+        forms["f64"] = {"ms_per_step": ms64, "node_ksamples_per_s": work / (ms64 * 1e-3),
+                        "kernel_ms": ms64_kernel}
         out64_h = out64.cpu().numpy()
         err_f64 = rel_l2(out_h, out64_h)
         if wl.name in ("c0", "c1"):
@@ -234,9 +251,11 @@ def main() -> int:
         flops = (8.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
         peak = PEAKS["fp64_valu_tflops"]
         dom = "k_signal_direct"
-    achieved = flops / (ms_ev * 1e-3) / 1e12
+    ms_dom = ms_kernel if ms_kernel else ms_ev
+    achieved = flops / (ms_dom * 1e-3) / 1e12
+    traffic = load_traffic(dom, wl.name, args.form)
     roof = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": None,
+            "frac": achieved / peak, "traffic": traffic, "kernel_ms": ms_dom,
             "per_unit": "8 flops per node*k-sample*encoding (complex MAC)" if plan.method == "cartesian"
             else "8 flops + 8 per encoding and 2 transcendentals per node*k-sample"}
 

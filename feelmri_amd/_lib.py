@@ -8,11 +8,15 @@ library is an error, never a fallback: build it with
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 import torch  # noqa: F401  (load torch's HIP runtime before the library)
 
-LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libfeelmri_hip.so"
+# FEELMRI_HIP_LIB selects another build of the same C ABI (e.g. to compare
+# kernel revisions); the default is the in-tree build.
+LIB_PATH = Path(os.environ.get("FEELMRI_HIP_LIB") or
+                Path(__file__).resolve().parent / "_lib" / "libfeelmri_hip.so")
 
 FM_OK, FM_ERR_ARG, FM_ERR_HIP, FM_ERR_UNSUPPORTED, FM_ERR_WORKSPACE = range(5)
 PREC_F64, PREC_MIXED, PREC_F32 = 0, 1, 2
@@ -35,6 +39,8 @@ _SIGNATURES = {
     "fm_signal_workspace_size": [_p, _int, _i64, _int, ctypes.POINTER(_sz)],
     "fm_signal_nodes_pack": [_i64, _int, _int, _p, _p, _p, _p, _p, _p, _p],
     "fm_signal_execute": [_p, _int, _i64, _int, _p, _p, _int, _p, _p, _sz, _p, _p],
+    "fm_signal_plan_set_timing": [_p, _int],
+    "fm_signal_plan_kernel_ms": [_p, ctypes.POINTER(_dbl), ctypes.POINTER(_int)],
     "fm_csr_spmv_c128": [_i64, _int, _p, _p, _p, _p, _p, _p],
     "fm_bloch_nodes_pack": [_i64, _int, _p, _p, _p, _p, _p, _p],
     "fm_bloch_execute": [_int, _i64, _i64, _p, _dbl, _dbl, _p, _p, _p, _p, _p, _p, _int, _p, _p,

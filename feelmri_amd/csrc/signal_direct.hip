@@ -379,6 +379,7 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
     double2 *dst = L.n_splits > 1 ? reinterpret_cast<double2 *>(ws) : reinterpret_cast<double2 *>(out);
     const int write_final = L.n_splits > 1 ? 0 : 1;
     dim3 grid((unsigned)direct_tiles(p, grouped), (unsigned)L.n_splits);
+    plan_timing_start(p, st);
     for (int v0 = 0; v0 < V; v0 += kMaxVB) {
         const int vb = (V - v0 < kMaxVB) ? V - v0 : kMaxVB;
         if (precision == FM_PREC_F64)
@@ -387,6 +388,7 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
             launch_direct<true>(p, grouped, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
         FM_LAUNCH_CHECK("k_signal_direct");
     }
+    plan_timing_stop(p, st);
     if (L.n_splits > 1) {
         const int64_t SV = S * V;
         k_reduce_splits<<<(unsigned)cdiv(SV, 256), 256, 0, st>>>(dst, L.n_splits, SV,
@@ -514,9 +516,47 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
     return FM_OK;
 }
 
+extern "C" int fm_signal_plan_set_timing(fm_signal_plan *p, int max_launches)
+{
+    FM_CHECK_ARG(p != nullptr, "plan is NULL");
+    FM_CHECK_ARG(max_launches >= 0 && max_launches <= 1 << 16, "max_launches out of range");
+    for (hipEvent_t e : p->ev_start) (void)hipEventDestroy(e);
+    for (hipEvent_t e : p->ev_stop) (void)hipEventDestroy(e);
+    p->ev_start.clear();
+    p->ev_stop.clear();
+    p->ev_used = 0;
+    for (int i = 0; i < max_launches; ++i) {
+        hipEvent_t a, b;
+        FM_HIP(hipEventCreate(&a));
+        FM_HIP(hipEventCreate(&b));
+        p->ev_start.push_back(a);
+        p->ev_stop.push_back(b);
+    }
+    return FM_OK;
+}
+
+extern "C" int fm_signal_plan_kernel_ms(fm_signal_plan *p, double *total_ms, int *n_timed)
+{
+    FM_CHECK_ARG(p != nullptr && total_ms != nullptr && n_timed != nullptr, "NULL argument");
+    double sum = 0.0;
+    const int n = p->ev_used;
+    if (n > 0) FM_HIP(hipEventSynchronize(p->ev_stop[(size_t)n - 1]));
+    for (int i = 0; i < n; ++i) {
+        float ms = 0.0f;
+        FM_HIP(hipEventElapsedTime(&ms, p->ev_start[(size_t)i], p->ev_stop[(size_t)i]));
+        sum += ms;
+    }
+    p->ev_used = 0;
+    *total_ms = sum;
+    *n_timed = n;
+    return FM_OK;
+}
+
 extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
 {
     if (!p) return FM_OK;
+    for (hipEvent_t e : p->ev_start) (void)hipEventDestroy(e);
+    for (hipEvent_t e : p->ev_stop) (void)hipEventDestroy(e);
     (void)hipFree(p->d_ks);
     (void)hipFree(p->d_perm);
     (void)hipFree(p->d_tiles_grouped);

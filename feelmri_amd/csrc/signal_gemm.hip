@@ -400,13 +400,17 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
         FM_LAUNCH_CHECK("k_cart_yz");
     }
     if (s.mixed) {
+        plan_timing_start(p, st);
         launch_cart_gemm_bf16(s, ws, n_kzV, st);
+        plan_timing_stop(p, st);
         FM_LAUNCH_CHECK("k_cart_gemm_bf16");
         const int64_t MNC = s.M * s.NC;
         k_cart_reduce<float2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(
             reinterpret_cast<const float2 *>(ws + s.off_part), s.n_splits, MNC, reinterpret_cast<double2 *>(out));
     } else {
+        plan_timing_start(p, st);
         launch_cart_gemm_f64(s, ws, n_kzV, st);
+        plan_timing_stop(p, st);
         FM_LAUNCH_CHECK("k_cart_gemm_f64");
         const int64_t MNC = s.M * s.NC;
         k_cart_reduce<double2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(

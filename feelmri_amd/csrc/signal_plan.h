@@ -52,7 +52,22 @@ struct fm_signal_plan {
     fm::SignalTile *d_tiles_consec = nullptr;  // offsets are sample indices
     double *d_tg_t = nullptr;
     fm::CartesianTables cart;
+    // optional timing of the dominant kernel (fm_signal_plan_set_timing)
+    std::vector<hipEvent_t> ev_start, ev_stop;
+    mutable int ev_used = 0;
 };
+
+namespace fm {
+// Record the start / stop event of the dominant kernel when timing is on.
+inline void plan_timing_start(const fm_signal_plan *p, hipStream_t st)
+{
+    if (p->ev_used < (int)p->ev_start.size()) (void)hipEventRecord(p->ev_start[(size_t)p->ev_used], st);
+}
+inline void plan_timing_stop(const fm_signal_plan *p, hipStream_t st)
+{
+    if (p->ev_used < (int)p->ev_stop.size()) (void)hipEventRecord(p->ev_stop[(size_t)p->ev_used++], st);
+}
+}  // namespace fm
 
 namespace fm {
 // Cartesian path (signal_gemm.hip).

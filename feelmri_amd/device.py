@@ -102,6 +102,17 @@ class SignalPlan:
                 pass
             self._h = None
 
+    def set_timing(self, max_launches: int) -> None:
+        """Record HIP events around the dominant kernel of the next
+        ``max_launches`` executes (0 disables)."""
+        _lib.call("fm_signal_plan_set_timing", self._h, int(max_launches))
+
+    def kernel_ms(self) -> tuple[float, int]:
+        """(summed ms, number of timed executes) since the last query."""
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _lib.call("fm_signal_plan_kernel_ms", self._h, ctypes.byref(ms), ctypes.byref(n))
+        return ms.value, n.value
+
     def workspace_bytes(self, precision, n_nodes: int, n_enc: int) -> int:
         out = ctypes.c_size_t()
         _lib.call("fm_signal_workspace_size", self._h, _prec(precision), int(n_nodes), int(n_enc),

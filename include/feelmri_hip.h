@@ -107,6 +107,15 @@ int fm_signal_execute(const fm_signal_plan *plan, int precision, int64_t n_nodes
                       const double *motion_coef, void *workspace, size_t workspace_bytes,
                       double *out, void *stream);
 
+/* Measurement support (no reference counterpart): when enabled, each
+ * fm_signal_execute records a pair of HIP events on its stream around the
+ * dominant kernel (the GEMM, or the direct-sum launches), for up to
+ * max_launches executes.  fm_signal_plan_kernel_ms waits for the last
+ * recorded event and returns the summed milliseconds and the number of
+ * timed executes since the previous query.  max_launches = 0 disables. */
+int fm_signal_plan_set_timing(fm_signal_plan *plan, int max_launches);
+int fm_signal_plan_kernel_ms(fm_signal_plan *plan, double *total_ms, int *n_timed);
+
 /* y = M x for a CSR matrix M (int64 indptr/indices, float64 data) and
  * x (n_cols, n_enc) complex128: the M * Mxy product of MRI.cpp:326 when M
  * is the consistent mass matrix (examples/free_running.py:177). */
