@@ -43,8 +43,13 @@ def _run(kloc, t, n, v, seed, method="cartesian"):
     return got, ref
 
 
+# Column tiles of 128 (lines x kz x encodings): the tiles whose distinct Ey
+# and Ezb rows fit the LDS staging take the staged kernel, the others (many
+# lines per tile with one kz, or more than 128 kz x encodings) the per-lane
+# kernel; both kinds are covered below.
 @pytest.mark.parametrize("res,n,v", [((16, 8, 2), 3000, 1), ((40, 12, 3), 2049, 2), ((128, 9, 1), 700, 3),
-                                     ((8, 6, 1), 1, 1), ((33, 17, 5), 5000, 1)])
+                                     ((8, 6, 1), 1, 1), ((33, 17, 5), 5000, 1), ((32, 16, 32), 4000, 1),
+                                     ((16, 200, 1), 3000, 1), ((12, 3, 50), 2000, 3)])
 def test_gemm_shapes(gpu, res, n, v):
     kloc, t = trajectories.cartesian((0.22, 0.22, 0.22), res, t_first=1.0, t_readout=3.0)
     _run(kloc, t, n, v, seed=sum(res) + n)
