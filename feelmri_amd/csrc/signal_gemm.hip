@@ -259,96 +259,116 @@ __global__ void k_cart_E(const double *__restrict__ packed, int64_t N, int64_t N
     }
 }
 
-// Mixed E table, one thread per lane fragment: the thread owns row i and
-// the 8 nodes n0..n0+7 that lane (i%32 + 32g) of tile (kstep, i/32) feeds to
-// the MFMA, and writes each of the 4 planes as one 16-byte store (threads of
-// a wave write 1 KB contiguous).  Phase in FP64 with range reduction, then
-// the hardware FP32 sine/cosine and exp2.
-__global__ void k_cart_E_bf16(const double *__restrict__ packed, int64_t N, int64_t Np, int64_t M, int64_t Mp,
-                              const double *__restrict__ F, const double *__restrict__ tr, uint4 *__restrict__ E)
+// Mixed E table, one thread per lane fragment column: the thread owns the 8
+// nodes n0..n0+7 that lane l (row l%32, node group l/32) of k-step kb feeds
+// to the MFMA, keeps their attributes in registers and walks the row blocks,
+// writing each of the 4 planes of a tile as one 16-byte store (a wave writes
+// 1 KB contiguous per plane).  Phase in FP64 with range reduction, then the
+// hardware FP32 sine/cosine and exp2.
+__global__ __launch_bounds__(256) void k_cart_E_bf16(const double *__restrict__ packed, int64_t N, int64_t Np,
+                                                     int64_t M, int64_t Mp, const double *__restrict__ F,
+                                                     const double *__restrict__ tr, uint4 *__restrict__ E)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over Mp * Np / 8
-    if (t >= Mp * (Np / 8)) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over (Np / 16) * 64
+    if (t >= (Np / 16) * 64) return;
     const int lane = (int)(t % 64);
-    const int64_t tile = t / 64;
+    const int64_t kb = t / 64;
     const int64_t nrb = Mp / 32;
-    const int64_t rb = tile % nrb, kb = tile / nrb;
-    const int64_t i = rb * 32 + (lane % 32);
     const int64_t n0 = kb * 16 + 8 * (lane / 32);
-    unsigned short h[4][8];
-    const bool row_ok = i < M;
-    const double fx = row_ok ? F[3 * i] : 0.0, fy = row_ok ? F[3 * i + 1] : 0.0, fz = row_ok ? F[3 * i + 2] : 0.0;
-    const double ti = row_ok ? tr[i] : 0.0;
+    double x[8], y[8], z[8], ph[8], r2[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int64_t n = n0 + q;
-        float er = 0.0f, ei = 0.0f;
-        if (row_ok && n < N) {
-            const double x = packed[n], y = packed[N + n], z = packed[2 * N + n];
-            const double phi = packed[3 * N + n], r2 = packed[4 * N + n];
-            const double u = fma(fx, x, fma(fy, y, fz * z)) - phi * ti * (1.0 / kTwoPi);
-            const float rf = (float)(u - __builtin_rint(u));
-            const float d = __builtin_amdgcn_exp2f((float)(-ti * r2 * 1.4426950408889634));
-            er = d * __builtin_amdgcn_cosf(rf);
-            ei = -d * __builtin_amdgcn_sinf(rf);
-        }
-        const __bf16 erh = (__bf16)er, eih = (__bf16)ei;
-        const __bf16 erl = (__bf16)(er - (float)erh), eil = (__bf16)(ei - (float)eih);
-        h[0][q] = __builtin_bit_cast(unsigned short, erh);
-        h[1][q] = __builtin_bit_cast(unsigned short, erl);
-        h[2][q] = __builtin_bit_cast(unsigned short, eih);
-        h[3][q] = __builtin_bit_cast(unsigned short, eil);
+        const bool ok = n < N;
+        x[q] = ok ? packed[n] : 0.0;
+        y[q] = ok ? packed[N + n] : 0.0;
+        z[q] = ok ? packed[2 * N + n] : 0.0;
+        ph[q] = ok ? packed[3 * N + n] : 0.0;
+        r2[q] = ok ? packed[4 * N + n] : 0.0;
     }
+    const int nvalid = (int)(N - n0 < 8 ? (N - n0 > 0 ? N - n0 : 0) : 8);
+    for (int64_t rb = 0; rb < nrb; ++rb) {
+        const int64_t i = rb * 32 + (lane % 32);
+        const bool row_ok = i < M;
+        const double fx = row_ok ? F[3 * i] : 0.0, fy = row_ok ? F[3 * i + 1] : 0.0,
+                     fz = row_ok ? F[3 * i + 2] : 0.0;
+        const double ti = row_ok ? tr[i] : 0.0;
+        unsigned short h[4][8];
 #pragma unroll
-    for (int pl = 0; pl < 4; ++pl) {
-        uint4 v;
-        v.x = (unsigned)h[pl][0] | ((unsigned)h[pl][1] << 16);
-        v.y = (unsigned)h[pl][2] | ((unsigned)h[pl][3] << 16);
-        v.z = (unsigned)h[pl][4] | ((unsigned)h[pl][5] << 16);
-        v.w = (unsigned)h[pl][6] | ((unsigned)h[pl][7] << 16);
-        E[(tile * 4 + pl) * 64 + lane] = v;
-    }
-}
-
-// Ey[j][n] = exp(-2 pi i G_j . x_n) ; Ezb[k*V+v][n] = b[n,v] exp(-2 pi i H_k . x_n)
-template <bool MIXED>
-__global__ void k_cart_yz(const double *__restrict__ packed, int64_t N, int64_t Np, int V, int64_t n_ln,
-                          int64_t n_kz, const double *__restrict__ G, const double *__restrict__ H,
-                          void *__restrict__ Ey, void *__restrict__ Ez)
-{
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over (n_ln + n_kz) * Np
-    if (idx >= (n_ln + n_kz) * Np) return;
-    const int64_t n = idx % Np, row = idx / Np;
-    const bool is_y = row < n_ln;
-    const double *kv = is_y ? &G[3 * row] : &H[3 * (row - n_ln)];
-    double c = 0.0, s = 0.0;
-    if (n < N) {
-        const double x = packed[n], y = packed[N + n], z = packed[2 * N + n];
-        cis_neg_cycles<MIXED>(fma(kv[0], x, fma(kv[1], y, kv[2] * z)), c, s);
-    }
-    if (is_y) {
-        if (MIXED)
-            reinterpret_cast<float2 *>(Ey)[row * Np + n] = make_float2((float)c, (float)s);
-        else
-            reinterpret_cast<double2 *>(Ey)[row * Np + n] = make_double2(c, s);
-    } else {
-        const int64_t k = row - n_ln;
-        for (int v = 0; v < V; ++v) {
-            double br = 0.0, bi = 0.0;
-            if (n < N) {
-                br = packed[(int64_t)(5 + 2 * v) * N + n];
-                bi = packed[(int64_t)(6 + 2 * v) * N + n];
+        for (int q = 0; q < 8; ++q) {
+            float er = 0.0f, ei = 0.0f;
+            if (row_ok && q < nvalid) {
+                const double u = fma(fx, x[q], fma(fy, y[q], fz * z[q])) - ph[q] * ti * (1.0 / kTwoPi);
+                const float rf = (float)(u - __builtin_rint(u));
+                const float d = __builtin_amdgcn_exp2f((float)(-ti * r2[q] * 1.4426950408889634));
+                er = d * __builtin_amdgcn_cosf(rf);
+                ei = -d * __builtin_amdgcn_sinf(rf);
             }
-            const double ar = br * c - bi * s, ai = br * s + bi * c;
-            if (MIXED)
-                reinterpret_cast<float2 *>(Ez)[(k * V + v) * Np + n] = make_float2((float)ar, (float)ai);
-            else
-                reinterpret_cast<double2 *>(Ez)[(k * V + v) * Np + n] = make_double2(ar, ai);
+            const __bf16 erh = (__bf16)er, eih = (__bf16)ei;
+            const __bf16 erl = (__bf16)(er - (float)erh), eil = (__bf16)(ei - (float)eih);
+            h[0][q] = __builtin_bit_cast(unsigned short, erh);
+            h[1][q] = __builtin_bit_cast(unsigned short, erl);
+            h[2][q] = __builtin_bit_cast(unsigned short, eih);
+            h[3][q] = __builtin_bit_cast(unsigned short, eil);
+        }
+        const int64_t tile = kb * nrb + rb;
+#pragma unroll
+        for (int pl = 0; pl < 4; ++pl) {
+            uint4 v;
+            v.x = (unsigned)h[pl][0] | ((unsigned)h[pl][1] << 16);
+            v.y = (unsigned)h[pl][2] | ((unsigned)h[pl][3] << 16);
+            v.z = (unsigned)h[pl][4] | ((unsigned)h[pl][5] << 16);
+            v.w = (unsigned)h[pl][6] | ((unsigned)h[pl][7] << 16);
+            E[(tile * 4 + pl) * 64 + lane] = v;
         }
     }
 }
 
+// Ey rows exp(-2 pi i (G_j . x_n)) and Ezb rows exp(-2 pi i (H_k . x_n)) b_v[n]
+// (the encoding's weighted magnetisation folded into the kz factor).  One
+// thread per node keeps its position in registers and walks a block of
+// kYZRowsPerBlock rows (Ey rows first, then Ezb).
+constexpr int kYZRowsPerBlock = 16;
 
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_cart_yz(const double *__restrict__ packed, int64_t N, int64_t Np, int V,
+                                                 int64_t n_ln, int64_t n_kz, const double *__restrict__ G,
+                                                 const double *__restrict__ H, void *__restrict__ Ey,
+                                                 void *__restrict__ Ez)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= Np) return;
+    const int64_t r0 = (int64_t)blockIdx.y * kYZRowsPerBlock;
+    const int64_t r1 = (r0 + kYZRowsPerBlock < n_ln + n_kz) ? r0 + kYZRowsPerBlock : n_ln + n_kz;
+    const bool ok = n < N;
+    const double x = ok ? packed[n] : 0.0, y = ok ? packed[N + n] : 0.0, z = ok ? packed[2 * N + n] : 0.0;
+    for (int64_t row = r0; row < r1; ++row) {
+        const bool is_y = row < n_ln;
+        const double *kv = is_y ? &G[3 * row] : &H[3 * (row - n_ln)];
+        double c = 0.0, s = 0.0;
+        if (ok) cis_neg_cycles<MIXED>(fma(kv[0], x, fma(kv[1], y, kv[2] * z)), c, s);
+        if (is_y) {
+            if (MIXED)
+                reinterpret_cast<float2 *>(Ey)[row * Np + n] = make_float2((float)c, (float)s);
+            else
+                reinterpret_cast<double2 *>(Ey)[row * Np + n] = make_double2(c, s);
+        } else {
+            const int64_t k = row - n_ln;
+            for (int v = 0; v < V; ++v) {
+                double br = 0.0, bi = 0.0;
+                if (ok) {
+                    br = packed[(int64_t)(5 + 2 * v) * N + n];
+                    bi = packed[(int64_t)(6 + 2 * v) * N + n];
+                }
+                const double ar = br * c - bi * s, ai = br * s + bi * c;
+                if (MIXED)
+                    reinterpret_cast<float2 *>(Ez)[(k * V + v) * Np + n] = make_float2((float)ar, (float)ai);
+                else
+                    reinterpret_cast<double2 *>(Ez)[(k * V + v) * Np + n] = make_double2(ar, ai);
+            }
+        }
+    }
+}
 
 template <typename P>
 __global__ void k_cart_reduce(const P *__restrict__ part, int64_t n_splits, int64_t MNC, double2 *__restrict__ out)
@@ -384,19 +404,24 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
     {
         const int64_t tot = s.Mp * s.Np;
         if (s.mixed)
-            k_cart_E_bf16<<<(unsigned)cdiv(tot / 8, 256), 256, 0, st>>>(packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t,
-                                                                       reinterpret_cast<uint4 *>(ws + s.off_E));
+            k_cart_E_bf16<<<(unsigned)cdiv(s.Np / 16 * 64, 256), 256, 0, st>>>(
+                packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t, reinterpret_cast<uint4 *>(ws + s.off_E));
         else
             k_cart_E<false><<<(unsigned)cdiv(tot, 256), 256, 0, st>>>(packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t,
                                                                      ws + s.off_E);
         FM_LAUNCH_CHECK("k_cart_E");
-        const int64_t tot2 = (c.n_ln + c.n_kz) * s.Np;
+        if (cdiv(c.n_ln + c.n_kz, kYZRowsPerBlock) > 65535) {
+            set_error("Cartesian plan with %lld lines + kz rows exceeds the operand-table grid",
+                      (long long)(c.n_ln + c.n_kz));
+            return FM_ERR_UNSUPPORTED;
+        }
+        const dim3 gyz((unsigned)cdiv(s.Np, 256), (unsigned)cdiv(c.n_ln + c.n_kz, kYZRowsPerBlock));
         if (s.mixed)
-            k_cart_yz<true><<<(unsigned)cdiv(tot2, 256), 256, 0, st>>>(packed, N, s.Np, V, c.n_ln, c.n_kz, c.d_G,
-                                                                      c.d_H, ws + s.off_Ey, ws + s.off_Ez);
+            k_cart_yz<true><<<gyz, 256, 0, st>>>(packed, N, s.Np, V, c.n_ln, c.n_kz, c.d_G, c.d_H, ws + s.off_Ey,
+                                                 ws + s.off_Ez);
         else
-            k_cart_yz<false><<<(unsigned)cdiv(tot2, 256), 256, 0, st>>>(packed, N, s.Np, V, c.n_ln, c.n_kz, c.d_G,
-                                                                       c.d_H, ws + s.off_Ey, ws + s.off_Ez);
+            k_cart_yz<false><<<gyz, 256, 0, st>>>(packed, N, s.Np, V, c.n_ln, c.n_kz, c.d_G, c.d_H, ws + s.off_Ey,
+                                                  ws + s.off_Ez);
         FM_LAUNCH_CHECK("k_cart_yz");
     }
     if (s.mixed) {
