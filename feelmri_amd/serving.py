@@ -167,7 +167,8 @@ class Scheduler:
                     end = clock() if clock else virtual
                     req = st.req
                     rep.results[req.rid] = RequestResult(req.rid, end - req.arrival, st.t_admit, end,
-                                                         kspace if keep_kspace else None, req.work)
+                                                         kspace if keep_kspace else None,
+                                                         getattr(st, "work", req.work))
                     active.remove(st)
                     used -= st.fp
                     progressed = True
@@ -202,19 +203,35 @@ class _State:
     copy_event: object = None
     t_admit: float = 0.0
     fp: int = 0
+    work: int = 0                 # node*k-samples actually computed (phantom class size)
+
+
+# Cube sides of the resident phantom classes: k^3 nodes from 10.6 k to 2.25 M
+# in steps of 1.25 in k (about 2x in nodes).
+PHANTOM_CLASSES = (22, 27, 34, 43, 54, 67, 84, 105, 131)
 
 
 class PhantomStore:
-    """Resident phantoms by size class: mesh, tissue, lumped weights, Bloch nodes."""
+    """Resident phantoms by size class: mesh, tissue, lumped weights, Bloch nodes.
 
-    def __init__(self, dev, classes=None):
+    A request's node count is served by the class whose k^3 is nearest in
+    log scale (``classes=None``: a class per cube side, i.e. the nearest
+    k^3)."""
+
+    def __init__(self, dev, classes=PHANTOM_CLASSES):
         self.dev = dev
         self.cache = {}
-        self.classes = classes
+        self.classes = tuple(classes) if classes else None
 
     def key(self, n_nodes: int) -> int:
         # cube side of a box mesh with about n_nodes nodes
-        return max(4, int(round(n_nodes ** (1.0 / 3.0))))
+        k = max(4, int(round(n_nodes ** (1.0 / 3.0))))
+        if not self.classes:
+            return k
+        return min(self.classes, key=lambda c: abs(math.log(c) - math.log(k)))
+
+    def n_nodes(self, n_nodes: int) -> int:
+        return self.key(n_nodes) ** 3
 
     def get(self, n_nodes: int):
         import torch
@@ -283,8 +300,14 @@ class GpuBackend:
             self.plans[key] = (plans, rows)
         return self.plans[key]
 
+    def prepare(self, requests) -> None:
+        """Make the phantoms and plans of ``requests`` resident (service start-up)."""
+        for req in requests:
+            self.store.get(req.n_nodes)
+            self._chunks(req)
+
     def footprint(self, req: ScanRequest) -> int:
-        N = self.store.key(req.n_nodes) ** 3
+        N = self.store.n_nodes(req.n_nodes)
         S = req.n_samples
         plans, _ = self._chunks(req)
         ws = max(p.workspace_bytes(req.precision, N, 1) for p in plans)
@@ -300,6 +323,7 @@ class GpuBackend:
         plans, rows = self._chunks(req)
         st = _State(req=req, stream=stream, phantom=ph, plans=plans, rows=rows)
         N = ph["mesh"].n_nodes
+        st.work = N * req.n_samples
         with torch.cuda.stream(stream):
             # spoiled-GRE steady state for this request (its own state buffers)
             blocks = sequences.spoiled_gre(flip_deg=15.0, TR=5.0, n_dummy=self.n_dummy, n_lines=1, rf_dur=0.2)
