@@ -5,8 +5,9 @@
 // reference: cpp/feelmri/MRI.cpp:279-381 (static nodes), :121-242 (moving).
 //
 // Work decomposition.  Samples of equal t are grouped (the reference's t
-// cache) and cut into tiles of up to 512 samples; a workgroup owns one tile
-// (2 samples per lane) and one contiguous split of the nodes.  Nodes stream
+// cache) and cut into slots of up to 256 samples; a workgroup owns a tile of
+// two slots (2 samples per lane, the slots possibly of different t) and one
+// contiguous split of the nodes.  Nodes stream
 // through LDS in chunks of 256: each lane of the workgroup stages one node,
 // folding the t-dependent factors (T2* decay, off-resonance, and the POD
 // position update for moving nodes) into the staged coefficient, so the
@@ -39,17 +40,25 @@ struct CoefT<false> {
     using T = double;
 };
 
-template <int VB, bool MIXED, bool SHARED_T>
+// A node staged in LDS.  Shared-t tiles carry one coefficient set per slot
+// (the slots of a tile may belong to different t-groups), and, for moving
+// nodes, one position per slot.
+template <int VB, bool MIXED, bool SHARED_T, bool MOTION>
 struct alignas(16) StagedNode {
     using CT = typename CoefT<MIXED>::T;
-    double x, y, z;
-    double dwc;                  // phi/(2 pi), cycles/ms   (general kernel only)
-    double r2;                   // 1/T2, 1/ms              (general kernel only)
-    CT c[2 * VB];
+    static constexpr int kPos = MOTION ? kDirectSPL : 1;
+    static constexpr int kCoef = SHARED_T ? kDirectSPL : 1;
+    double x[kPos], y[kPos], z[kPos];
+    double dwc;                  // phi/(2 pi), cycles/ms   (per-sample-t kernel only)
+    double r2;                   // 1/T2, 1/ms              (per-sample-t kernel only)
+    CT c[kCoef][2 * VB];
 };
-// The shared-t kernel does not need dwc / r2; keep one layout for simplicity
-// of the loaders (LDS is not the constraint here).
 
+// Tiles.  Shared-t ("grouped") tiles are kDirectSPL slot records
+// tiles[kDirectSPL * b + j]: slot j holds up to 256 samples of one t-group
+// (lane l takes sample perm[offset + l]).  Per-sample-t ("consecutive")
+// tiles are one record of up to 512 consecutive samples (lane l takes
+// offset + l + 256 j).
 template <int VB, bool MIXED, bool SHARED_T, bool MOTION>
 __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     const double *__restrict__ packed, int64_t N, int V, int v0,
@@ -59,11 +68,11 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S, int write_final)
 {
     using CT = typename CoefT<MIXED>::T;
-    using Node = StagedNode<VB, MIXED, SHARED_T>;
+    using Node = StagedNode<VB, MIXED, SHARED_T, MOTION>;
+    constexpr int kSlots = SHARED_T ? kDirectSPL : 1;
     __shared__ Node snode[kDirectThreads];
-    __shared__ double scoef[kMaxModes + 3];
+    __shared__ double scoef[MOTION ? kDirectSPL : 1][kMaxModes + 3];
 
-    const SignalTile tile = tiles[blockIdx.x];
     const int64_t split = blockIdx.y;
     const int64_t n_begin = split * nodes_per_split;
     const int64_t n_end = (n_begin + nodes_per_split < N) ? n_begin + nodes_per_split : N;
@@ -76,10 +85,22 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     const double *pr2 = packed + 4 * N;
     const double *pb = packed + 5 * N;  // rows 2v, 2v+1
 
-    double tg = 0.0;
-    if (SHARED_T) tg = tg_t[tile.group];
+    SignalTile tl[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) tl[j] = tiles[(int64_t)blockIdx.x * kSlots + j];
+    double tg[kDirectSPL];
+    bool same_group = true;  // both slots share one t-group (stage once)
+    if (SHARED_T) {
+#pragma unroll
+        for (int j = 0; j < kDirectSPL; ++j) tg[j] = tl[j].count > 0 ? tg_t[tl[j].group] : 0.0;
+#pragma unroll
+        for (int j = 1; j < kDirectSPL; ++j) same_group = same_group && (tl[j].group == tl[0].group || tl[j].count == 0);
+    }
     if (MOTION) {
-        for (int q = tid; q < R + 3; q += kDirectThreads) scoef[q] = mcoef[(int64_t)tile.group * (R + 3) + q];
+#pragma unroll
+        for (int j = 0; j < kDirectSPL; ++j)
+            for (int q = tid; q < R + 3; q += kDirectThreads)
+                scoef[j][q] = tl[j].count > 0 ? mcoef[(int64_t)tl[j].group * (R + 3) + q] : 0.0;
     }
 
     int sid[kDirectSPL];
@@ -87,11 +108,14 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     double kxs[kDirectSPL], kys[kDirectSPL], kzs[kDirectSPL], ts[kDirectSPL];
 #pragma unroll
     for (int j = 0; j < kDirectSPL; ++j) {
-        const int idx = tid + j * kDirectThreads;
-        valid[j] = idx < tile.count;
-        // grouped tiles index the t-ordered permutation; consecutive tiles
-        // cover samples [offset, offset + count) directly
-        sid[j] = valid[j] ? (SHARED_T ? perm[tile.offset + idx] : tile.offset + idx) : 0;
+        if (SHARED_T) {
+            valid[j] = tid < tl[j].count;
+            sid[j] = valid[j] ? perm[tl[j].offset + tid] : 0;
+        } else {
+            const int idx = tid + j * kDirectThreads;
+            valid[j] = idx < tl[0].count;
+            sid[j] = valid[j] ? tl[0].offset + idx : 0;
+        }
         const double4 k = ks[sid[j]];
         kxs[j] = k.x;
         kys[j] = k.y;
@@ -111,50 +135,71 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
             const int64_t n = base + tid;
             Node nd;
             if (n < n_end) {
-                double x = px[n], y = py[n], z = pz[n];
-                if (MOTION) {
-                    double dx = scoef[R], dy = scoef[R + 1], dz = scoef[R + 2];
-                    for (int r = 0; r < R; ++r) {
-                        const double w = scoef[r];
-                        dx += (double)modes[(int64_t)(0 * R + r) * N + n] * w;
-                        dy += (double)modes[(int64_t)(1 * R + r) * N + n] * w;
-                        dz += (double)modes[(int64_t)(2 * R + r) * N + n] * w;
+                const double x0 = px[n], y0 = py[n], z0 = pz[n];
+#pragma unroll
+                for (int j = 0; j < Node::kPos; ++j) {
+                    double x = x0, y = y0, z = z0;
+                    if (MOTION) {
+                        if (j > 0 && same_group) {
+                            x = nd.x[0];
+                            y = nd.y[0];
+                            z = nd.z[0];
+                        } else {
+                            double dx = scoef[j][R], dy = scoef[j][R + 1], dz = scoef[j][R + 2];
+                            for (int r = 0; r < R; ++r) {
+                                const double w = scoef[j][r];
+                                dx += (double)modes[(int64_t)(0 * R + r) * N + n] * w;
+                                dy += (double)modes[(int64_t)(1 * R + r) * N + n] * w;
+                                dz += (double)modes[(int64_t)(2 * R + r) * N + n] * w;
+                            }
+                            x += dx;
+                            y += dy;
+                            z += dz;
+                        }
                     }
-                    x += dx;
-                    y += dy;
-                    z += dz;
+                    nd.x[j] = x;
+                    nd.y[j] = y;
+                    nd.z[j] = z;
                 }
-                nd.x = x;
-                nd.y = y;
-                nd.z = z;
                 const double phi = pphi[n], r2 = pr2[n];
                 nd.dwc = phi * (1.0 / kTwoPi);
                 nd.r2 = r2;
-                if (SHARED_T) {
-                    // c = b exp(-t/T2) exp(i phi t)   (MRI.cpp:347-354)
-                    const double d = exp(-tg * r2);
-                    double so, co;
-                    sincos(phi * tg, &so, &co);
-                    const double fr = d * co, fi = d * so;
 #pragma unroll
-                    for (int v = 0; v < VB; ++v) {
-                        const double br = pb[(int64_t)(2 * (v0 + v)) * N + n];
-                        const double bi = pb[(int64_t)(2 * (v0 + v) + 1) * N + n];
-                        nd.c[2 * v] = (CT)(br * fr - bi * fi);
-                        nd.c[2 * v + 1] = (CT)(br * fi + bi * fr);
-                    }
-                } else {
+                for (int j = 0; j < Node::kCoef; ++j) {
+                    if (SHARED_T) {
+                        if (j > 0 && same_group) {
 #pragma unroll
-                    for (int v = 0; v < VB; ++v) {
-                        nd.c[2 * v] = (CT)pb[(int64_t)(2 * (v0 + v)) * N + n];
-                        nd.c[2 * v + 1] = (CT)pb[(int64_t)(2 * (v0 + v) + 1) * N + n];
+                            for (int v = 0; v < 2 * VB; ++v) nd.c[j][v] = nd.c[0][v];
+                            continue;
+                        }
+                        // c = b exp(-t/T2) exp(i phi t)   (MRI.cpp:347-354)
+                        const double d = exp(-tg[j] * r2);
+                        double so, co;
+                        sincos(phi * tg[j], &so, &co);
+                        const double fr = d * co, fi = d * so;
+#pragma unroll
+                        for (int v = 0; v < VB; ++v) {
+                            const double br = pb[(int64_t)(2 * (v0 + v)) * N + n];
+                            const double bi = pb[(int64_t)(2 * (v0 + v) + 1) * N + n];
+                            nd.c[j][2 * v] = (CT)(br * fr - bi * fi);
+                            nd.c[j][2 * v + 1] = (CT)(br * fi + bi * fr);
+                        }
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < VB; ++v) {
+                            nd.c[j][2 * v] = (CT)pb[(int64_t)(2 * (v0 + v)) * N + n];
+                            nd.c[j][2 * v + 1] = (CT)pb[(int64_t)(2 * (v0 + v) + 1) * N + n];
+                        }
                     }
                 }
             } else {
-                nd.x = nd.y = nd.z = 0.0;
+#pragma unroll
+                for (int j = 0; j < Node::kPos; ++j) nd.x[j] = nd.y[j] = nd.z[j] = 0.0;
                 nd.dwc = nd.r2 = 0.0;
 #pragma unroll
-                for (int v = 0; v < 2 * VB; ++v) nd.c[v] = (CT)0;
+                for (int j = 0; j < Node::kCoef; ++j)
+#pragma unroll
+                    for (int v = 0; v < 2 * VB; ++v) nd.c[j][v] = (CT)0;
             }
             snode[tid] = nd;
         }
@@ -172,7 +217,8 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
             const Node &nd = snode[m];
 #pragma unroll
             for (int j = 0; j < kDirectSPL; ++j) {
-                double u = fma(kxs[j], nd.x, fma(kys[j], nd.y, kzs[j] * nd.z));
+                const int jp = MOTION ? j : 0;
+                double u = fma(kxs[j], nd.x[jp], fma(kys[j], nd.y[jp], kzs[j] * nd.z[jp]));
                 if (!SHARED_T) u = fma(-nd.dwc, ts[j], u);  // phase - phi t, in cycles
                 const double rr = u - __builtin_rint(u);
                 CT cs, sn;
@@ -198,9 +244,10 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                     sn *= d;
                 }
                 // e = cs - i sn ; acc += e * c
+                const int jc = SHARED_T ? j : 0;
 #pragma unroll
                 for (int v = 0; v < VB; ++v) {
-                    const CT cr = nd.c[2 * v], ci = nd.c[2 * v + 1];
+                    const CT cr = nd.c[jc][2 * v], ci = nd.c[jc][2 * v + 1];
                     acc_re[j][v] = fma(cs, cr, fma(sn, ci, acc_re[j][v]));
                     acc_im[j][v] = fma(cs, ci, fma(-sn, cr, acc_im[j][v]));
                 }
@@ -469,18 +516,21 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         for (int64_t g = 0; g < p->n_tgroups; ++g) count[(size_t)g + 1] += count[(size_t)g];
         std::vector<int64_t> fill(count.begin(), count.end() - 1);
         for (int64_t s = 0; s < S; ++s) perm[(size_t)fill[(size_t)p->sample_group[(size_t)s]]++] = (int32_t)s;
+        // grouped: slot records of up to 256 samples of one group, packed
+        // kDirectSPL to a tile (slots of one tile may hold different groups)
         for (int64_t g = 0; g < p->n_tgroups; ++g) {
-            for (int64_t o = count[(size_t)g]; o < count[(size_t)g + 1]; o += kDirectTile) {
-                const int64_t c = std::min<int64_t>(kDirectTile, count[(size_t)g + 1] - o);
+            for (int64_t o = count[(size_t)g]; o < count[(size_t)g + 1]; o += kDirectThreads) {
+                const int64_t c = std::min<int64_t>(kDirectThreads, count[(size_t)g + 1] - o);
                 tiles_g.push_back({(int32_t)g, (int32_t)o, (int32_t)c, 0});
             }
         }
+        while (tiles_g.size() % kDirectSPL) tiles_g.push_back({0, 0, 0, 0});
         for (int64_t o = 0; o < S; o += kDirectTile)
             tiles_c.push_back({-1, (int32_t)o, (int32_t)std::min<int64_t>(kDirectTile, S - o), 0});
     }
     const double avg_group = p->n_tgroups ? (double)S / (double)p->n_tgroups : 0.0;
     p->prefer_grouped = avg_group >= 64.0;
-    p->n_tiles_grouped = (int64_t)tiles_g.size();
+    p->n_tiles_grouped = (int64_t)tiles_g.size() / kDirectSPL;
     p->n_tiles_consec = (int64_t)tiles_c.size();
 
     std::vector<double4> ks((size_t)S);
