@@ -21,6 +21,8 @@
 // [-1/2, 1/2].  FP64 form: sincospi(2r).  Mixed form: r rounded to FP32 and
 // fed to the hardware v_sin_f32 / v_cos_f32, which take revolutions;
 // partial sums over a 256-node chunk are kept in FP32 and added into FP64.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <type_traits>
 #include <unordered_map>
@@ -271,6 +273,134 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     }
 }
 
+// Mixed form, static nodes, readout runs.  Lane l of a workgroup owns run
+// blockIdx.x * 256 + l (kRunLen samples of one readout line).  Per staged
+// node the lane forms
+//   e0 = exp(-t0/T2) exp(-2 pi i (k0.x - phi t0 / 2 pi))      (FP64 phase,
+//        range-reduced, FP32 hardware sine/cosine)
+//   q  = exp(-dt/T2) exp(-2 pi i (dk.x - phi dt / 2 pi))       (FP32)
+// and accumulates e0 q^m c for m = 0 .. kRunLen-1, advancing e by one
+// complex multiply per sample.  FP32 partial sums over a 256-node chunk are
+// added into FP64, as in k_signal_direct.
+template <int VB>
+struct alignas(16) RunNode {
+    double x, y, z, dwc, r2;
+    float xf, yf, zf, dwcf, r2f;
+    float c[2 * VB];
+};
+
+template <int VB>
+__global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
+    const double *__restrict__ packed, int64_t N, int V, int v0, const RunDesc *__restrict__ runs,
+    int64_t n_runs, int64_t stride, int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S,
+    int write_final)
+{
+    __shared__ RunNode<VB> snode[kDirectThreads];
+    const int64_t split = blockIdx.y;
+    const int64_t n_begin = split * nodes_per_split;
+    const int64_t n_end = (n_begin + nodes_per_split < N) ? n_begin + nodes_per_split : N;
+    const int tid = threadIdx.x;
+    const int64_t rid = (int64_t)blockIdx.x * kDirectThreads + tid;
+    const bool valid = rid < n_runs;
+    const RunDesc rd = runs[valid ? rid : 0];
+
+    const double *px = packed;
+    const double *py = packed + N;
+    const double *pz = packed + 2 * N;
+    const double *pphi = packed + 3 * N;
+    const double *pr2 = packed + 4 * N;
+    const double *pb = packed + 5 * N;
+
+    double tot_re[kRunLen][VB], tot_im[kRunLen][VB];
+#pragma unroll
+    for (int m = 0; m < kRunLen; ++m)
+#pragma unroll
+        for (int v = 0; v < VB; ++v) tot_re[m][v] = tot_im[m][v] = 0.0;
+
+    for (int64_t base = n_begin; base < n_end; base += kDirectThreads) {
+        __syncthreads();
+        {
+            const int64_t n = base + tid;
+            RunNode<VB> nd;
+            if (n < n_end) {
+                nd.x = px[n];
+                nd.y = py[n];
+                nd.z = pz[n];
+                nd.dwc = pphi[n] * (1.0 / kTwoPi);
+                nd.r2 = pr2[n];
+#pragma unroll
+                for (int v = 0; v < VB; ++v) {
+                    nd.c[2 * v] = (float)pb[(int64_t)(2 * (v0 + v)) * N + n];
+                    nd.c[2 * v + 1] = (float)pb[(int64_t)(2 * (v0 + v) + 1) * N + n];
+                }
+            } else {
+                nd.x = nd.y = nd.z = nd.dwc = nd.r2 = 0.0;
+#pragma unroll
+                for (int v = 0; v < 2 * VB; ++v) nd.c[v] = 0.0f;
+            }
+            nd.xf = (float)nd.x;
+            nd.yf = (float)nd.y;
+            nd.zf = (float)nd.z;
+            nd.dwcf = (float)nd.dwc;
+            nd.r2f = (float)nd.r2;
+            snode[tid] = nd;
+        }
+        __syncthreads();
+        const int cnt = (n_end - base < kDirectThreads) ? (int)(n_end - base) : kDirectThreads;
+
+        float acc_re[kRunLen][VB], acc_im[kRunLen][VB];
+#pragma unroll
+        for (int m = 0; m < kRunLen; ++m)
+#pragma unroll
+            for (int v = 0; v < VB; ++v) acc_re[m][v] = acc_im[m][v] = 0.0f;
+
+        for (int i = 0; i < cnt; ++i) {
+            const RunNode<VB> &nd = snode[i];
+            // anchor: FP64 phase in cycles, range-reduced
+            const double u = fma(rd.k0x, nd.x, fma(rd.k0y, nd.y, fma(rd.k0z, nd.z, -nd.dwc * rd.t0)));
+            const float r = (float)(u - __builtin_rint(u));
+            const float d0 = __builtin_amdgcn_exp2f((float)(-rd.t0 * nd.r2) * 1.44269504f);
+            float er = d0 * __builtin_amdgcn_cosf(r);
+            float ei = -d0 * __builtin_amdgcn_sinf(r);
+            // step factor
+            float du = fmaf(rd.dkx, nd.xf, fmaf(rd.dky, nd.yf, fmaf(rd.dkz, nd.zf, -nd.dwcf * rd.dt)));
+            du -= __builtin_rintf(du);
+            const float ds = __builtin_amdgcn_exp2f(-rd.dt * nd.r2f * 1.44269504f);
+            const float qr = ds * __builtin_amdgcn_cosf(du);
+            const float qi = -ds * __builtin_amdgcn_sinf(du);
+#pragma unroll
+            for (int m = 0; m < kRunLen; ++m) {
+#pragma unroll
+                for (int v = 0; v < VB; ++v) {
+                    const float cr = nd.c[2 * v], ci = nd.c[2 * v + 1];
+                    acc_re[m][v] = fmaf(er, cr, fmaf(-ei, ci, acc_re[m][v]));
+                    acc_im[m][v] = fmaf(er, ci, fmaf(ei, cr, acc_im[m][v]));
+                }
+                const float nr = er * qr - ei * qi;
+                ei = fmaf(er, qi, ei * qr);
+                er = nr;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < kRunLen; ++m)
+#pragma unroll
+            for (int v = 0; v < VB; ++v) {
+                tot_re[m][v] += (double)acc_re[m][v];
+                tot_im[m][v] += (double)acc_im[m][v];
+            }
+    }
+
+    if (!valid) return;
+#pragma unroll
+    for (int m = 0; m < kRunLen; ++m) {
+        if (m >= rd.len) continue;
+        const int64_t sid = (int64_t)rd.s0 + m * stride;
+        const int64_t row = write_final ? sid : split * S + sid;
+#pragma unroll
+        for (int v = 0; v < VB; ++v) dst[row * V + v0 + v] = make_double2(tot_re[m][v], tot_im[m][v]);
+    }
+}
+
 // out[s, v] = sum over splits, in split order.
 __global__ void k_reduce_splits(const double2 *__restrict__ part, int64_t n_splits, int64_t SV,
                                 double2 *__restrict__ out)
@@ -332,17 +462,19 @@ struct DirectLaunch {
     int64_t nodes_per_split;
 };
 
-static int64_t direct_tiles(const fm_signal_plan *p, bool grouped)
+// Tilings of the direct path: 0 consecutive, 1 t-grouped, 2 readout runs.
+static int64_t direct_tiles(const fm_signal_plan *p, int tiling)
 {
-    return grouped ? p->n_tiles_grouped : p->n_tiles_consec;
+    if (tiling == 2) return cdiv(p->n_runs, kDirectThreads);
+    return tiling == 1 ? p->n_tiles_grouped : p->n_tiles_consec;
 }
 
-static DirectLaunch direct_launch_shape(const fm_signal_plan *p, int64_t N, bool grouped)
+static DirectLaunch direct_launch_shape(const fm_signal_plan *p, int64_t N, int tiling)
 {
     // Aim for ~8 workgroups per CU over the whole grid; keep >= 2048 nodes
     // per split so the staging cost stays amortised.
     const int64_t target = 8 * (int64_t)device_cu_count();
-    const int64_t nt = direct_tiles(p, grouped);
+    const int64_t nt = direct_tiles(p, tiling);
     int64_t splits = cdiv(target, nt > 0 ? nt : 1);
     const int64_t max_splits = cdiv(N, 2048);
     if (splits > max_splits) splits = max_splits;
@@ -394,10 +526,11 @@ static void launch_direct(const fm_signal_plan *p, bool grouped, bool motion, di
 static size_t direct_workspace(const fm_signal_plan *p, int64_t N, int V)
 {
     if (N <= 0 || p->n_samples == 0) return 0;
-    // the larger of the two tilings' needs (motion forces the grouped one)
+    // the largest of the tilings' needs (motion forces the grouped one)
     size_t need = 0;
-    for (int g = 0; g < 2; ++g) {
-        const DirectLaunch L = direct_launch_shape(p, N, g == 1);
+    for (int g = 0; g < 3; ++g) {
+        if (g == 2 && p->n_runs == 0) continue;
+        const DirectLaunch L = direct_launch_shape(p, N, g);
         if (L.n_splits > 1)
             need = std::max(need, (size_t)L.n_splits * (size_t)p->n_samples * (size_t)V * sizeof(double2));
     }
@@ -415,9 +548,12 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
         return FM_OK;
     }
     const bool motion = modes != nullptr && R > 0;
-    // positions depend on t, so moving nodes always take the t-grouped tiles
+    // positions depend on t, so moving nodes always take the t-grouped tiles;
+    // static nodes in the mixed form take the readout runs when the plan has them
+    const bool use_runs = !motion && precision != FM_PREC_F64 && p->n_runs > 0;
     const bool grouped = motion || p->prefer_grouped;
-    const DirectLaunch L = direct_launch_shape(p, N, grouped);
+    const int tiling = use_runs ? 2 : (grouped ? 1 : 0);
+    const DirectLaunch L = direct_launch_shape(p, N, tiling);
     const size_t need = L.n_splits > 1 ? (size_t)L.n_splits * (size_t)S * (size_t)V * sizeof(double2) : 0;
     if (need > ws_bytes) {
         set_error("signal workspace too small: need %zu bytes, got %zu", need, ws_bytes);
@@ -425,9 +561,19 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
     }
     double2 *dst = L.n_splits > 1 ? reinterpret_cast<double2 *>(ws) : reinterpret_cast<double2 *>(out);
     const int write_final = L.n_splits > 1 ? 0 : 1;
-    dim3 grid((unsigned)direct_tiles(p, grouped), (unsigned)L.n_splits);
+    dim3 grid((unsigned)direct_tiles(p, tiling), (unsigned)L.n_splits);
     plan_timing_start(p, st);
-    for (int v0 = 0; v0 < V; v0 += kMaxVB) {
+    if (use_runs) {
+        // one encoding per launch: 16 samples x 1 encoding of accumulators
+        // keeps the kernel at ~130 VGPRs (3 waves per SIMD)
+        for (int v0 = 0; v0 < V; ++v0) {
+            k_signal_runs<1><<<grid, kDirectThreads, 0, st>>>(packed, N, V, v0, p->d_runs, p->n_runs,
+                                                              p->run_stride, L.nodes_per_split, dst, S,
+                                                              write_final);
+            FM_LAUNCH_CHECK("k_signal_runs");
+        }
+    }
+    for (int v0 = 0; v0 < V && !use_runs; v0 += kMaxVB) {
         const int vb = (V - v0 < kMaxVB) ? V - v0 : kMaxVB;
         if (precision == FM_PREC_F64)
             launch_direct<false>(p, grouped, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
@@ -537,6 +683,57 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
     for (int64_t s = 0; s < S; ++s)
         ks[(size_t)s] = make_double4(kx[s] * kPiFOverPi, ky[s] * kPiFOverPi, kz[s] * kPiFOverPi, t[s]);
 
+    // Readout runs: kRunLen consecutive readout samples of a line (stride
+    // lines * kz in the sample index) whose k' and t are linear in the
+    // readout index to 1e-12 of the largest |k'| / |t|.  All runs or none.
+    std::vector<RunDesc> runs;
+    if (n_meas >= 2 && S > 0) {
+        const int64_t stride = n_lines * n_kz;
+        double kmax = 0.0, tmax = 0.0;
+        for (int64_t s = 0; s < S; ++s) {
+            kmax = std::max({kmax, std::fabs(ks[(size_t)s].x), std::fabs(ks[(size_t)s].y), std::fabs(ks[(size_t)s].z)});
+            tmax = std::max(tmax, std::fabs(ks[(size_t)s].w));
+        }
+        const double tolk = 1e-12 * (kmax > 0.0 ? kmax : 1.0), tolt = 1e-12 * (tmax > 0.0 ? tmax : 1.0);
+        bool linear = true;
+        runs.reserve((size_t)(stride * cdiv(n_meas, kRunLen)));
+        for (int64_t line = 0; line < stride && linear; ++line) {
+            for (int64_t i0 = 0; i0 < n_meas && linear; i0 += kRunLen) {
+                const int64_t len = std::min<int64_t>(kRunLen, n_meas - i0);
+                const int64_t s0 = i0 * stride + line;
+                const double4 a = ks[(size_t)s0];
+                double4 d = make_double4(0.0, 0.0, 0.0, 0.0);
+                if (len > 1) {
+                    const double4 b = ks[(size_t)(s0 + stride)];
+                    d = make_double4(b.x - a.x, b.y - a.y, b.z - a.z, b.w - a.w);
+                }
+                for (int64_t m = 2; m < len && linear; ++m) {
+                    const double4 c = ks[(size_t)(s0 + m * stride)];
+                    linear = std::fabs(c.x - a.x - m * d.x) <= tolk && std::fabs(c.y - a.y - m * d.y) <= tolk &&
+                             std::fabs(c.z - a.z - m * d.z) <= tolk && std::fabs(c.w - a.w - m * d.w) <= tolt;
+                }
+                RunDesc r{};
+                r.k0x = a.x;
+                r.k0y = a.y;
+                r.k0z = a.z;
+                r.t0 = a.w;
+                r.dkx = (float)d.x;
+                r.dky = (float)d.y;
+                r.dkz = (float)d.z;
+                r.dt = (float)d.w;
+                r.s0 = (int32_t)s0;
+                r.len = (int32_t)len;
+                runs.push_back(r);
+            }
+        }
+        if (linear) {
+            p->n_runs = (int64_t)runs.size();
+            p->run_stride = stride;
+        } else {
+            runs.clear();
+        }
+    }
+
     auto fail = [&](hipError_t e, const char *what) {
         set_error("%s failed: %s", what, hipGetErrorString(e));
         fm_signal_plan_destroy(p);
@@ -554,6 +751,10 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         if ((e = hipMemcpy(p->d_tiles_grouped, tiles_g.data(), sizeof(SignalTile) * tiles_g.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if ((e = hipMemcpy(p->d_tiles_consec, tiles_c.data(), sizeof(SignalTile) * tiles_c.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if ((e = hipMemcpy(p->d_tg_t, p->tg_t.data(), sizeof(double) * (size_t)p->n_tgroups, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        if (p->n_runs > 0) {
+            if ((e = hipMalloc(&p->d_runs, sizeof(RunDesc) * runs.size())) != hipSuccess) return fail(e, "hipMalloc");
+            if ((e = hipMemcpy(p->d_runs, runs.data(), sizeof(RunDesc) * runs.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        }
         if (p->method == FM_METHOD_CARTESIAN) {
             const int rc = cartesian_upload(p);
             if (rc != FM_OK) {
@@ -610,6 +811,7 @@ extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
     (void)hipFree(p->d_ks);
     (void)hipFree(p->d_perm);
     (void)hipFree(p->d_tiles_grouped);
+    (void)hipFree(p->d_runs);
     (void)hipFree(p->d_tiles_consec);
     (void)hipFree(p->d_tg_t);
     cartesian_free(p);

@@ -15,6 +15,20 @@ constexpr int kMaxVB = 4;
 // Upper bound on POD modes fused into the loaders.
 constexpr int kMaxModes = 32;
 
+// Readout runs (mixed-form direct kernel for static nodes): kRunLen
+// consecutive readout samples of one line whose k and t are linear in the
+// readout index, so exp(-2 pi i k.x) exp((i phi - 1/T2) t) advances by one
+// complex factor per sample.
+constexpr int kRunLen = 16;
+
+struct RunDesc {
+    double k0x, k0y, k0z, t0;  // first sample (k' in cycles/m, t in ms)
+    float dkx, dky, dkz, dt;   // step per sample
+    int32_t s0;                // first sample index; sample m is s0 + m * run_stride
+    int32_t len;               // samples in the run (<= kRunLen)
+    int32_t pad[2];
+};
+
 struct SignalTile {
     int32_t group;   // t-group index (shared-t kernels) or -1
     int32_t offset;  // first entry in perm
@@ -51,6 +65,9 @@ struct fm_signal_plan {
     fm::SignalTile *d_tiles_grouped = nullptr;
     fm::SignalTile *d_tiles_consec = nullptr;  // offsets are sample indices
     double *d_tg_t = nullptr;
+    // readout runs (empty when some run is not linear in k and t)
+    int64_t n_runs = 0, run_stride = 0;
+    fm::RunDesc *d_runs = nullptr;
     fm::CartesianTables cart;
     // optional timing of the dominant kernel (fm_signal_plan_set_timing)
     std::vector<hipEvent_t> ev_start, ev_stop;
