@@ -110,6 +110,33 @@ def time_form(plan, nodes, form, steps, warmup):
     return out, ms_event, wall / steps * 1e3, (k_ms / k_n if k_n else None)
 
 
+def time_host_io(plan, wl, w_h, mxy_h, form, steps):
+    """Whole call from host arrays: node arrays (numpy) to the device, pack,
+    execute, k-space back to host memory; the plan is reused."""
+    import numpy as np
+    import torch
+
+    from feelmri_amd import device
+
+    host = (np.ascontiguousarray(wl.mesh.nodes), np.ascontiguousarray(wl.tissue.delta_omega),
+            np.ascontiguousarray(wl.tissue.T2star), np.ascontiguousarray(mxy_h), np.ascontiguousarray(w_h))
+    ws = torch.empty(max(plan.workspace_bytes(form, wl.n_nodes, mxy_h.shape[1]), 1), dtype=torch.uint8,
+                     device=plan.device)
+
+    def one():
+        nodes = device.SignalNodes.from_arrays(*host[:4], weights=host[4], device=plan.device)
+        out = torch.empty((plan.n_samples, nodes.n_enc), dtype=torch.complex128, device=plan.device)
+        plan.execute(nodes, precision=form, out=out, workspace=ws)
+        return out.cpu().numpy()
+
+    one()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        one()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
 def load_traffic(kernel: str, config: str, form: str):
     """HBM bytes per launch of ``kernel`` from the committed counter summary
     (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel of
@@ -287,6 +314,10 @@ def main() -> int:
             f"speed-up {ms64 / ms_ev:.2f}x")
     w_h = w.cpu().numpy()
     mxy_h = mxy.cpu().numpy()
+    ms_host = time_host_io(plan, wl, w_h, mxy_h, args.form, max(3, args.steps // 4))
+    log(f"{args.form} from host arrays (H2D nodes + pack + execute + D2H k-space): {ms_host:.3f} ms/step")
+    forms[args.form]["host_io_ms_per_step"] = ms_host
+    forms[args.form]["host_io_node_ksamples_per_s"] = work / (ms_host * 1e-3)
     err_oracle = oracle_subset_error(wl, w_h, mxy_h, out_h)
     log(f"relL2 vs FP64 oracle on sampled k-points: {err_oracle:.2e}")
 
@@ -296,10 +327,17 @@ def main() -> int:
         flops = 8.0 * work
         peak = PEAKS["bf16_mfma_tflops"] if args.form == "mixed" else PEAKS["fp64_mfma_tflops"]
         dom = "k_cart_gemm"
+    elif args.form == "mixed" and plan.n_runs > 0:
+        # one complex step (6 flops) and one complex MAC per encoding (8) per
+        # node*k-sample, in FP32
+        bound, unit = "valu", "TFLOP/s"
+        flops = (6.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
+        peak = PEAKS["fp32_valu_tflops"]
+        dom = "k_signal_runs"
     else:
         bound, unit = "valu", "TFLOP/s"
         flops = (8.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
-        peak = PEAKS["fp64_valu_tflops"]
+        peak = PEAKS["fp64_valu_tflops"] if args.form == "f64" else PEAKS["fp32_valu_tflops"]
         dom = "k_signal_direct"
     ms_dom = ms_kernel if ms_kernel else ms_ev
     achieved = flops / (ms_dom * 1e-3) / 1e12
@@ -309,7 +347,9 @@ def main() -> int:
             "executed": (3.0 * achieved if (plan.method == "cartesian" and args.form == "mixed") else achieved),
             "per_unit": "8 flops per node*k-sample*encoding (complex MAC); the mixed GEMM executes 3 bf16 "
                         "products (hi*hi + hi*lo + lo*hi) per real product, i.e. 24" if plan.method == "cartesian"
-            else "8 flops + 8 per encoding and 2 transcendentals per node*k-sample"}
+            else ("6 flops (complex step) + 8 per encoding (complex MAC) per node*k-sample, FP32; one FP64-anchored "
+                  "sincos per node and 16-sample readout run" if dom == "k_signal_runs"
+                  else "8 flops + 8 per encoding and 2 transcendentals per node*k-sample")}
 
     cpu = None
     if not args.no_cpu:

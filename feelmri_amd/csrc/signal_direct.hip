@@ -830,6 +830,14 @@ extern "C" int fm_signal_plan_info(const fm_signal_plan *p, int64_t *n_samples, 
     return FM_OK;
 }
 
+extern "C" int fm_signal_plan_runs(const fm_signal_plan *p, int64_t *n_runs, int *run_len)
+{
+    FM_CHECK_ARG(p != nullptr, "plan is NULL");
+    if (n_runs) *n_runs = p->n_runs;
+    if (run_len) *run_len = kRunLen;
+    return FM_OK;
+}
+
 extern "C" int fm_signal_workspace_size(const fm_signal_plan *p, int precision, int64_t n_nodes,
                                         int n_enc, size_t *bytes)
 {

@@ -86,6 +86,11 @@ class SignalPlan:
                   tg.ctypes.data_as(ctypes.c_void_p))
         self.tgroup_t = tg
         self._t = tt
+        nr, rl = ctypes.c_int64(), ctypes.c_int()
+        _lib.call("fm_signal_plan_runs", self._h, ctypes.byref(nr), ctypes.byref(rl))
+        # readout runs (k and t linear along every readout line): the mixed
+        # form of static nodes takes the run kernel when n_runs > 0
+        self.n_runs, self.run_len = nr.value, rl.value
 
     def sample_groups(self) -> np.ndarray:
         """Index into ``tgroup_t`` of every sample, in sample order."""

@@ -81,6 +81,10 @@ int fm_signal_plan_destroy(fm_signal_plan *plan);
  * coefficient table of fm_signal_execute must follow. */
 int fm_signal_plan_info(const fm_signal_plan *plan, int64_t *n_samples, int64_t *n_tgroups,
                         int *method, double *tgroup_t);
+/* Readout runs of the plan: 0 unless k and t are linear in the readout
+ * index along every line (then the mixed form of static nodes takes the
+ * run kernel).  No counterpart in the reference (plan introspection). */
+int fm_signal_plan_runs(const fm_signal_plan *plan, int64_t *n_runs, int *run_len);
 /* Workspace (device bytes) fm_signal_execute needs for this plan. */
 int fm_signal_workspace_size(const fm_signal_plan *plan, int precision, int64_t n_nodes,
                              int n_enc, size_t *bytes);

@@ -110,6 +110,21 @@ def test_direct_readout_runs_cartesian(gpu, shape, v):
     _check(kloc, t, n=2500, v=v, seed=sum(shape), method="direct")
 
 
+def test_readout_run_detection(gpu):
+    from feelmri_amd import device
+
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (21, 6, 3), t_first=0.7)
+    p = device.SignalPlan(kloc, t, method="direct")
+    assert p.n_runs == 6 * 3 * 2 and p.run_len == 16   # ceil(21 / 16) runs per line
+    kloc, t = trajectories.radial(0.24, 100, 7, t_first=0.3)
+    assert device.SignalPlan(kloc, t).n_runs == 7 * 7
+    kloc, t = trajectories.spiral(0.24, 64, 200, 8, t_first=0.2)
+    assert device.SignalPlan(kloc, t).n_runs == 0     # k not linear along the readout
+    rng = np.random.default_rng(5)
+    kl = tuple(rng.uniform(-200, 200, (30, 7, 2)) for _ in range(3))
+    assert device.SignalPlan(kl, rng.uniform(0.0, 4.0, (30, 7, 2))).n_runs == 0
+
+
 def test_direct_readout_runs_radial_short_last_run(gpu):
     kloc, t = trajectories.radial(0.24, 100, 7, t_first=0.3)
     _check(kloc, t, n=6000, seed=11)
