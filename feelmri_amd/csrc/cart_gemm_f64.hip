@@ -9,7 +9,10 @@ namespace fm {
 // ---------------------------------------------------------------------------
 // Workgroup tile: RB*16 rows x 64 columns; wave w owns columns [16w, 16w+16).
 // A 16-node step is 4 MFMA k-steps of 4 nodes; lane l supplies row/column
-// l%16 and node 4*st + l/16 of k-step st.
+// l%16 and node 4*st + l/16 of k-step st.  Within a k-step the MFMAs are
+// issued term by term across the RB row blocks, so an accumulator is
+// updated every RB-th MFMA.  RB <= 4 keeps the kernel within 256 registers
+// (two waves per SIMD).
 template <int RB>
 __global__ __launch_bounds__(kGemmThreads) void k_cart_gemm_f64(
     const double *__restrict__ E, const double2 *__restrict__ Ey, const double2 *__restrict__ Ez, int64_t Np,
@@ -71,17 +74,21 @@ __global__ __launch_bounds__(kGemmThreads) void k_cart_gemm_f64(
             }
         }
 #pragma unroll
-        for (int r = 0; r < RB; ++r) {
-            const double *t = &sE[buf][r * 2 * 4 * 64];
+        for (int st = 0; st < 4; ++st) {
+            double er[RB], ei[RB];
 #pragma unroll
-            for (int st = 0; st < 4; ++st) {
-                const double er = t[(0 * 4 + st) * 64 + lane];
-                const double ei = t[(1 * 4 + st) * 64 + lane];
-                acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, br[st], acc_re[r], 0, 0, 0);
-                acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, bi[st], acc_im[r], 0, 0, 0);
-                acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ei, bi[st], acc_re[r], 0, 0, 0);
-                acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, br[st], acc_im[r], 0, 0, 0);
+            for (int r = 0; r < RB; ++r) {
+                er[r] = sE[buf][(r * 2 * 4 + 0 * 4 + st) * 64 + lane];
+                ei[r] = sE[buf][(r * 2 * 4 + 1 * 4 + st) * 64 + lane];
             }
+#pragma unroll
+            for (int r = 0; r < RB; ++r) acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[r], br[st], acc_re[r], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[r], bi[st], acc_im[r], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ei[r], bi[st], acc_re[r], 0, 0, 0);
+#pragma unroll
+            for (int r = 0; r < RB; ++r) acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei[r], br[st], acc_im[r], 0, 0, 0);
         }
         if (has_next) {
 #pragma unroll

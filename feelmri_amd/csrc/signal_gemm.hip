@@ -130,7 +130,7 @@ static GemmShape gemm_shape(const fm_signal_plan *p, int precision, int64_t N, i
     s.mixed = precision != FM_PREC_F64;
     const int rowblk = s.mixed ? 32 : 16;
     const int colblk = s.mixed ? 32 : 16;
-    const int max_rb = s.mixed ? 4 : 8;
+    const int max_rb = s.mixed ? 4 : 4;
     s.N = N;
     s.Np = cdiv(std::max<int64_t>(N, 1), 16) * 16;
     s.M = c.n_ro;
@@ -196,66 +196,46 @@ __device__ inline void cis_neg_cycles(double u, double &c, double &s)
     }
 }
 
-__device__ inline void split_bf16(float x, uint16_t &hi, uint16_t &lo)
-{
-    const __bf16 h = (__bf16)x;
-    const float hf = (float)h;
-    const __bf16 l = (__bf16)(x - hf);
-    hi = __builtin_bit_cast(uint16_t, h);
-    lo = __builtin_bit_cast(uint16_t, l);
-}
-
-// E table.  Mixed: tiles of (32 rows x 16 nodes) per 4 planes (Er_h, Er_l,
+// E tables.  Mixed: tiles of (32 rows x 16 nodes) per 4 planes (Er_h, Er_l,
 // Ei_h, Ei_l), each plane in MFMA lane order: lane l holds row l%32 and
 // nodes 8*(l/32) .. +7 (8 bf16 = 16 bytes).  Tile index (kstep, rowblock).
 // FP64: tiles of (16 rows x 16 nodes) per 2 planes (Er, Ei), plane layout
 // [kstep4 (4)][lane (64)] with lane l holding row l%16, node l/16 of that
 // 4-node step.  Padding rows/nodes are zero.
-template <bool MIXED>
-__global__ void k_cart_E(const double *__restrict__ packed, int64_t N, int64_t Np, int64_t M, int64_t Mp,
-                         const double *__restrict__ F, const double *__restrict__ tr, void *__restrict__ Eout)
+//
+// FP64 E table, one thread per (k-step, 4-node step, lane): the thread owns
+// node n = 16 kb + 4 st + l/16 and walks the row blocks (row 16 rb + l%16),
+// so a wave writes 512 contiguous bytes per plane.
+__global__ __launch_bounds__(256) void k_cart_E_f64(const double *__restrict__ packed, int64_t N, int64_t Np,
+                                                    int64_t M, int64_t Mp, const double *__restrict__ F,
+                                                    const double *__restrict__ tr, double *__restrict__ E)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over Mp * Np
-    if (idx >= Mp * Np) return;
-    const int64_t n = idx % Np, i = idx / Np;
-    double er = 0.0, ei = 0.0;
-    if (n < N && i < M) {
-        const double x = packed[n], y = packed[N + n], z = packed[2 * N + n];
-        const double phi = packed[3 * N + n], r2 = packed[4 * N + n];
-        const double ti = tr[i];
-        // total phase in cycles: -(F.x) + phi t / (2 pi)
-        const double u = fma(F[3 * i], x, fma(F[3 * i + 1], y, F[3 * i + 2] * z)) - phi * ti * (1.0 / kTwoPi);
-        double c, s;
-        cis_neg_cycles<MIXED>(u, c, s);
-        const double d = exp(-ti * r2);
-        er = d * c;
-        ei = d * s;
-    }
-    if (MIXED) {
-        const int64_t kb = n / 16, q16 = n % 16;
-        const int64_t rb = i / 32, row = i % 32;
-        const int lane = (int)(row + 32 * (q16 / 8));
-        const int q = (int)(q16 % 8);
-        const int64_t nrb = Mp / 32;
-        uint16_t *E = reinterpret_cast<uint16_t *>(Eout);
-        const int64_t tile = (kb * nrb + rb) * 4;  // 4 planes per tile
-        uint16_t h, l;
-        split_bf16((float)er, h, l);
-        E[((tile + 0) * 64 + lane) * 8 + q] = h;
-        E[((tile + 1) * 64 + lane) * 8 + q] = l;
-        split_bf16((float)ei, h, l);
-        E[((tile + 2) * 64 + lane) * 8 + q] = h;
-        E[((tile + 3) * 64 + lane) * 8 + q] = l;
-    } else {
-        const int64_t kb = n / 16, q16 = n % 16;
-        const int64_t rb = i / 16, row = i % 16;
-        const int64_t step = q16 / 4;
-        const int lane = (int)(row + 16 * (q16 % 4));
-        const int64_t nrb = Mp / 16;
-        double *E = reinterpret_cast<double *>(Eout);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over (Np / 16) * 4 * 64
+    if (t >= (Np / 16) * 256) return;
+    const int lane = (int)(t % 64);
+    const int st = (int)((t / 64) % 4);
+    const int64_t kb = t / 256;
+    const int64_t nrb = Mp / 16;
+    const int64_t n = kb * 16 + 4 * st + lane / 16;
+    const bool ok = n < N;
+    const double x = ok ? packed[n] : 0.0, y = ok ? packed[N + n] : 0.0, z = ok ? packed[2 * N + n] : 0.0;
+    const double phi = ok ? packed[3 * N + n] : 0.0, r2 = ok ? packed[4 * N + n] : 0.0;
+    for (int64_t rb = 0; rb < nrb; ++rb) {
+        const int64_t i = rb * 16 + (lane % 16);
+        double er = 0.0, ei = 0.0;
+        if (ok && i < M) {
+            const double ti = tr[i];
+            // total phase in cycles: -(F.x) + phi t / (2 pi)
+            const double u = fma(F[3 * i], x, fma(F[3 * i + 1], y, F[3 * i + 2] * z)) - phi * ti * (1.0 / kTwoPi);
+            double c, s;
+            cis_neg_cycles<false>(u, c, s);
+            const double d = exp(-ti * r2);
+            er = d * c;
+            ei = d * s;
+        }
         const int64_t tile = (kb * nrb + rb) * 2;
-        E[((tile + 0) * 4 + step) * 64 + lane] = er;
-        E[((tile + 1) * 4 + step) * 64 + lane] = ei;
+        E[((tile + 0) * 4 + st) * 64 + lane] = er;
+        E[((tile + 1) * 4 + st) * 64 + lane] = ei;
     }
 }
 
@@ -402,13 +382,12 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
     char *ws = reinterpret_cast<char *>(wsv);
     const int64_t n_kzV = c.n_kz * V;
     {
-        const int64_t tot = s.Mp * s.Np;
         if (s.mixed)
             k_cart_E_bf16<<<(unsigned)cdiv(s.Np / 16 * 64, 256), 256, 0, st>>>(
                 packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t, reinterpret_cast<uint4 *>(ws + s.off_E));
         else
-            k_cart_E<false><<<(unsigned)cdiv(tot, 256), 256, 0, st>>>(packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t,
-                                                                     ws + s.off_E);
+            k_cart_E_f64<<<(unsigned)cdiv(s.Np / 16 * 256, 256), 256, 0, st>>>(
+                packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t, reinterpret_cast<double *>(ws + s.off_E));
         FM_LAUNCH_CHECK("k_cart_E");
         if (cdiv(c.n_ln + c.n_kz, kYZRowsPerBlock) > 65535) {
             set_error("Cartesian plan with %lld lines + kz rows exceeds the operand-table grid",
