@@ -98,6 +98,44 @@ def _smooth_modes(nodes: np.ndarray, n_modes: int, rng, amp: float = 5e-4, scale
     return phi
 
 
+def encoding_block(enc: dict, d: int, t0: float, store: bool = True) -> sequences.Block:
+    """Phase-contrast imaging block starting at t0 (ms): a hard pulse, then for
+    encoding d > 0 a bipolar gradient pair along axis d-1 (VENC set by the
+    lobe amplitude ``gmax``); d = 0 is the reference encoding."""
+    pulse = sequences.HardPulse(start=t0, dur=enc["rf_dur"], flip=np.deg2rad(enc["flip_deg"]))
+    grads = []
+    if d > 0:
+        grads = [sequences.Trapezoid(d - 1, t0 + 0.25, enc["gmax"], enc["ramp"], enc["flat"]),
+                 sequences.Trapezoid(d - 1, t0 + 0.25 + enc["sep"], -enc["gmax"], enc["ramp"], enc["flat"])]
+    return sequences.make_block(t0, enc["rf_dur"], pulses=[pulse], gradients=grads, dt_rf=0.01, dt_gr=0.01,
+                                dt=1.0, store=store)
+
+
+def flow_sequence(wl: "FlowWorkload", n_dummy: int = 8, TR: float = 5.0, n_phases: int | None = None) -> list:
+    """The 4-D flow acquisition as Bloch blocks (like examples/4dflow.py:
+    dummy TRs to steady state, then per cardiac phase one stored imaging block
+    per encoding, TR apart, and a delay to the next phase).  Stored columns
+    come out in (phase, encoding) order."""
+    enc = wl.info["encoding"]
+    n_phases = wl.n_phases if n_phases is None else n_phases
+    spacing = wl.period / wl.n_phases
+    blocks, t = [], 0.0
+    for k in range(n_dummy):
+        b = encoding_block(enc, k % 4, t, store=False)
+        blocks += [b, sequences.delay_block(b.info["extent"][1], TR - (b.info["extent"][1] - t), dt=1.0)]
+        t += TR
+    for p in range(n_phases):
+        start = n_dummy * TR + p * spacing
+        if start > t:
+            blocks.append(sequences.delay_block(t, start - t, dt=1.0))
+            t = start
+        for d in range(4):
+            b = encoding_block(enc, d, t, store=True)
+            blocks += [b, sequences.delay_block(b.info["extent"][1], TR - (b.info["extent"][1] - t), dt=1.0)]
+            t += TR
+    return blocks
+
+
 def c3(scale: float = 1.0, n_phases: int = 20, n_modes: int = 10, kspace=(96, 96, 24)) -> FlowWorkload:
     from . import Motion
 
@@ -134,19 +172,12 @@ def c3(scale: float = 1.0, n_phases: int = 20, n_modes: int = 10, kspace=(96, 96
     area_lobe = flat + ramp                       # ms, per unit amplitude
     sep = flat + 2 * ramp                         # lobe centre separation
     gmax = np.pi / (sequences.GAMMA * area_lobe * sep * venc * 1e-3)   # mT/m
-    enc_blocks = []
-    for d in range(4):
-        pulse = sequences.HardPulse(start=0.0, dur=0.2, flip=np.deg2rad(15.0))
-        grads = []
-        if d > 0:
-            grads = [sequences.Trapezoid(d - 1, 0.25, gmax, ramp, flat),
-                     sequences.Trapezoid(d - 1, 0.25 + sep, -gmax, ramp, flat)]
-        enc_blocks.append(sequences.make_block(0.0, 0.2, pulses=[pulse], gradients=grads, dt_rf=0.01,
-                                               dt_gr=0.01, dt=1.0, store=True))
+    enc = dict(gmax=gmax, ramp=ramp, flat=flat, sep=sep, flip_deg=15.0, rf_dur=0.2)
+    enc_blocks = [encoding_block(enc, d, 0.0) for d in range(4)]
     kloc, t = trajectories.cartesian((0.24, 0.24, 0.06), kspace, t_first=1.0, t_readout=2.0)
     wl = FlowWorkload("c3", m, tis, kloc, t, n_enc=4, n_phases=n_phases, period=period, venc=venc,
                       motion=motion, velocity=velocity, enc_blocks=enc_blocks,
-                      info=dict(desc=f"4-D flow, {m.n_nodes}-node curved tube, {n_modes} POD modes, 4 encodings, "
+                      info=dict(encoding=enc, desc=f"4-D flow, {m.n_nodes}-node curved tube, {n_modes} POD modes, 4 encodings, "
                                      f"{'x'.join(map(str, kspace))} x {n_phases} phases, direct per-line sum"))
     return wl
 
