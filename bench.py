@@ -316,6 +316,87 @@ def run_flow(args, dev):
     return 0
 
 
+def _time_cuda(fn, reps: int) -> float:
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def row_benchmarks(wl, dev) -> dict:
+    """Throughput and roofline of the other rows of the path on the workload's
+    mesh: the Bloch integrator (final state and full history), P1 lumped
+    mass, and the POD contraction at the 4-D flow size (2 M nodes, 10 modes)."""
+    import numpy as np
+    import torch
+
+    from feelmri_amd import device
+    from feelmri_amd.sequences import GAMMA
+
+    m, tis = wl.mesh, wl.tissue
+    N = m.n_nodes
+    rows = {}
+    # Bloch: 1000 events of 10 us (hard pulse over 20 events, then random gradients)
+    T = 1000
+    rng = np.random.default_rng(0)
+    rf = np.zeros(T, dtype=np.complex128)
+    rf[1:21] = 0.01 * np.exp(0.3j)
+    G = np.zeros((T, 3))
+    G[21:] = rng.normal(0.0, 5.0, (T - 21, 3))
+    ev = device.EventTable(rf, G, np.full(T, 0.01), device=dev)
+    bn = device.BlochNodes.from_arrays(m.nodes, tis.T1, tis.T2, tis.delta_B_mT, device=dev)
+    mxy0 = torch.zeros(N, dtype=torch.complex128, device=dev)
+    mz0 = torch.ones(N, dtype=torch.float64, device=dev)
+    for prec, peak, key in (("f64", PEAKS["fp64_valu_tflops"], "bloch_f64"), ("f32", PEAKS["fp32_valu_tflops"], "bloch_f32")):
+        ms = _time_cuda(lambda: device.bloch(bn, ev, 1.0, GAMMA, mxy0, mz0, precision=prec), 5)
+        ach = 70.0 * N * T / (ms * 1e-3) / 1e12
+        rows[key] = {"node_events_per_s": N * T / (ms * 1e-3), "ms": ms, "nodes": N, "events": T,
+                     "roofline": {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+                                  "frac": ach / peak,
+                                  "per_unit": "~70 flops + sqrt, rcp, sin, cos per node*event (final state kept)"}}
+    Th = 200
+    evh = device.EventTable(rf[:Th], G[:Th], np.full(Th, 0.01), device=dev)
+    ms = _time_cuda(lambda: device.bloch(bn, evh, 1.0, GAMMA, mxy0, mz0, precision="f32", history=True), 3)
+    gbs = 12.0 * N * Th / (ms * 1e-3) / 1e9
+    rows["bloch_f32_history"] = {"node_events_per_s": N * Th / (ms * 1e-3), "ms": ms, "nodes": N, "events": Th,
+                                 "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAKS["hbm_gbs"],
+                                              "unit": "GB/s", "frac": gbs / PEAKS["hbm_gbs"],
+                                              "per_unit": "12 B written per node*event (complex64 Mxy + float32 Mz "
+                                                          "column-major history)"}}
+    # P1 lumped mass
+    col = device.P1Colouring(m.elems, N, device=dev)
+    nodes_d = torch.from_numpy(m.nodes).to(dev)
+    out = torch.empty(N, dtype=torch.float64, device=dev)
+    ms = _time_cuda(lambda: device.lumped_mass(col, nodes_d, out=out), 5)
+    E = m.n_elems
+    by = E * (16 + 4 + 4 * 24) + N * 16          # connectivity, permutation, gathered coordinates; weights
+    rows["lumped_mass_p1"] = {"elements_per_s": E / (ms * 1e-3), "ms": ms, "elements": E, "colours": col.n_colours,
+                              "roofline": {"bound": "hbm", "achieved": by / (ms * 1e-3) / 1e9,
+                                           "peak": PEAKS["hbm_gbs"], "unit": "GB/s",
+                                           "frac": by / (ms * 1e-3) / 1e9 / PEAKS["hbm_gbs"],
+                                           "per_unit": "116 B per element (16 B nodes, 4 B colour permutation, "
+                                                       "4 x 24 B coordinates) + 16 B per node; one launch per colour"}}
+    # POD contraction at 2 M nodes, 10 modes (float32)
+    P, R = 2_000_000, 10
+    modes = torch.randn((P, 3, R), dtype=torch.float32, device=dev)
+    wts = torch.randn(R, dtype=torch.float32, device=dev)
+    ms = _time_cuda(lambda: device.pod_tensordot(modes, wts), 10)
+    by = P * 3 * R * 4 + P * 3 * 4
+    rows["pod_tensordot"] = {"nodes": P, "modes": R, "ms": ms,
+                             "roofline": {"bound": "hbm", "achieved": by / (ms * 1e-3) / 1e9, "peak": PEAKS["hbm_gbs"],
+                                          "unit": "GB/s", "frac": by / (ms * 1e-3) / 1e9 / PEAKS["hbm_gbs"],
+                                          "per_unit": "12R B read + 12 B written per node"}}
+    del modes
+    return rows
+
+
 def oracle_subset_error(wl, w, mxy_host, out, n_pick=96):
     import numpy as np
 
@@ -345,6 +426,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison form")
+    ap.add_argument("--no-rows", action="store_true", help="skip the Bloch / mass / POD row measurements")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests in the serving stream reported beside the throughput (0: none)")
     ap.add_argument("--rate", type=float, default=20.0, help="serving stream arrival rate (requests/s)")
@@ -355,7 +437,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "node-sharded k-space partial sums would be combined with one reduce, see DESIGN.md",
+              "node-sharded k-space partial sums would be combined with one reduce over the ranks",
               file=sys.stderr)
         return 2
 
@@ -462,6 +544,11 @@ def main() -> int:
         cpu = cpu_baseline(wl, w_h, mxy_h, args.cpu_seconds)
         log(f"CPU: {cpu['value']:.3e} node*k/s on {cpu['cores']} threads")
 
+    rows = None
+    if not args.no_rows:
+        rows = row_benchmarks(wl, dev)
+        log("rows: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['roofline']['frac']:.2f} of {v['roofline']['bound']})"
+                                  for k, v in rows.items()))
     srv = None
     if args.serve_requests > 0:
         srv = serving_run(dev, args.serve_requests, args.rate)
@@ -491,6 +578,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "serving": srv,
+        "rows": rows,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -28,22 +28,37 @@ template <typename T>
 __device__ inline void dev_sincos(T x, T *s, T *c);
 template <>
 __device__ inline void dev_sincos<double>(double x, double *s, double *c) { sincos(x, s, c); }
+// FP32 form: the hardware sine/cosine, which take revolutions, after
+// reducing the argument to [-1/2, 1/2] revolution.
 template <>
-__device__ inline void dev_sincos<float>(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ inline void dev_sincos<float>(float x, float *s, float *c)
+{
+    float r = x * 0.159154943091895336f;
+    r -= __builtin_rintf(r);
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
 
 template <typename T>
 __device__ inline T dev_exp(T x);
 template <>
 __device__ inline double dev_exp<double>(double x) { return exp(x); }
 template <>
-__device__ inline float dev_exp<float>(float x) { return expf(x); }
+__device__ inline float dev_exp<float>(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 
 template <typename T>
 __device__ inline T dev_sqrt(T x);
 template <>
 __device__ inline double dev_sqrt<double>(double x) { return sqrt(x); }
 template <>
-__device__ inline float dev_sqrt<float>(float x) { return sqrtf(x); }
+__device__ inline float dev_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+template <typename T>
+__device__ inline T dev_rcp(T x);
+template <>
+__device__ inline double dev_rcp<double>(double x) { return 1.0 / x; }
+template <>
+__device__ inline float dev_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <typename T>
 struct Cplx2;
@@ -162,7 +177,7 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                 const T Bz = px * ev.gx + py * ev.gy + pz * ev.gz + dB;
                 T Bn = dev_sqrt<T>(Bz * Bz + ev.rf2);
                 if (Bn < (T)1e-12) Bn = (T)1e-12;
-                const T inv = (T)1 / Bn;
+                const T inv = dev_rcp<T>(Bn);
                 const T nz = Bz * inv;
                 T s, c;
                 dev_sincos<T>(Bn * ev.kappa, &s, &c);
