@@ -4,6 +4,8 @@
 // trajectory evaluation in python/feelmri/Motion.py:240-254 (POD),
 // :272-290 (PODVelocity), :303-309 (PODSum), :58-68 (RespiratoryMotion).
 // Memory-bound: each output row reads its M mode values once.
+#include <cstdint>
+
 #include "fm_common.h"
 
 namespace fm {
@@ -18,6 +20,41 @@ __global__ void k_tensordot(int64_t rows, int64_t M, const T *__restrict__ modes
     T acc = 0;
     for (int64_t m = 0; m < M; ++m) acc += a[m] * w[m];
     out[r] = acc;
+}
+
+// Same contraction for M <= kTdMaxM: a workgroup stages its 256 rows (256*M
+// contiguous values) through LDS with 16-byte loads, so the HBM reads are
+// fully coalesced, then each lane sums its row in mode order.
+constexpr int kTdMaxM = 32;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tensordot_tiled(int64_t rows, int M, const T *__restrict__ modes,
+                                                        const T *__restrict__ w, T *__restrict__ out)
+{
+    __shared__ __align__(16) T tile[256 * kTdMaxM];
+    const int64_t r0 = (int64_t)blockIdx.x * 256;
+    const int64_t nrows = (rows - r0 < 256) ? rows - r0 : 256;
+    const int64_t nval = nrows * M;
+    const T *src = modes + r0 * M;
+    constexpr int kVec = 16 / sizeof(T);
+    // 16-byte loads where the tile start is 16-byte aligned, scalar otherwise
+    if ((reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        const int64_t nvec = nval / kVec;
+        for (int64_t i = threadIdx.x; i < nvec; i += 256) {
+            const uint4 v = reinterpret_cast<const uint4 *>(src)[i];
+            *reinterpret_cast<uint4 *>(&tile[i * kVec]) = v;
+        }
+        for (int64_t i = nvec * kVec + threadIdx.x; i < nval; i += 256) tile[i] = src[i];
+    } else {
+        for (int64_t i = threadIdx.x; i < nval; i += 256) tile[i] = src[i];
+    }
+    __syncthreads();
+    if (threadIdx.x < nrows) {
+        const T *a = &tile[threadIdx.x * M];
+        T acc = 0;
+        for (int m = 0; m < M; ++m) acc += a[m] * w[m];
+        out[r0 + threadIdx.x] = acc;
+    }
 }
 
 __global__ void k_positions(int64_t N, const double *__restrict__ xyz, const float *__restrict__ modes, int R,
@@ -56,6 +93,16 @@ extern "C" int fm_pod_tensordot(int dtype, int64_t P, int64_t C, int64_t M, cons
     FM_CHECK_ARG(out && (M == 0 || (modes && weights)), "NULL array");
     hipStream_t st = as_stream(stream);
     const unsigned blocks = (unsigned)cdiv(rows, 256);
+    if (M > 0 && M <= kTdMaxM) {
+        if (dtype == FM_DTYPE_F64)
+            k_tensordot_tiled<double><<<blocks, 256, 0, st>>>(rows, (int)M, (const double *)modes,
+                                                            (const double *)weights, (double *)out);
+        else
+            k_tensordot_tiled<float><<<blocks, 256, 0, st>>>(rows, (int)M, (const float *)modes,
+                                                           (const float *)weights, (float *)out);
+        FM_LAUNCH_CHECK("k_tensordot_tiled");
+        return FM_OK;
+    }
     if (dtype == FM_DTYPE_F64)
         k_tensordot<double><<<blocks, 256, 0, st>>>(rows, M, (const double *)modes, (const double *)weights,
                                                    (double *)out);

@@ -140,13 +140,16 @@ def test_consistent_mass_assembly(gpu, dtype):
     assert diff <= (1e-12 if dtype == np.float64 else 1e-6) * abs(ref).max()
 
 
+# M <= 32 modes take the LDS-tiled kernel, more modes the per-row one; row
+# counts that are not a multiple of the 256-row tile end in a partial tile
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_pod_tensordot(gpu, dtype):
+@pytest.mark.parametrize("P,C,M", [(5000, 3, 10), (777, 3, 1), (301, 2, 32), (129, 3, 33), (1, 1, 5)])
+def test_pod_tensordot(gpu, dtype, P, C, M):
     from feelmri_amd import POD
 
-    rng = np.random.default_rng(9)
-    modes = rng.normal(size=(5000, 3, 10)).astype(dtype)
-    w = rng.normal(size=10).astype(dtype)
+    rng = np.random.default_rng(9 + P + M)
+    modes = rng.normal(size=(P, C, M)).astype(dtype)
+    w = rng.normal(size=M).astype(dtype)
     got = POD.tensordot_modes_weights(modes, w)
     want = np.tensordot(modes.astype(np.float64), w.astype(np.float64), axes=([2], [0]))
     assert got.dtype == dtype
