@@ -60,6 +60,24 @@ __device__ inline double dev_rcp<double>(double x) { return 1.0 / x; }
 template <>
 __device__ inline float dev_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// a != b for values that are equal across the wave, decided on the scalar
+// unit (first lane's bits).
+__device__ inline bool uniform_ne(double a, double b)
+{
+    const unsigned long long ua = __builtin_bit_cast(unsigned long long, a);
+    const unsigned long long ub = __builtin_bit_cast(unsigned long long, b);
+    const unsigned alo = __builtin_amdgcn_readfirstlane((unsigned)ua);
+    const unsigned ahi = __builtin_amdgcn_readfirstlane((unsigned)(ua >> 32));
+    const unsigned blo = __builtin_amdgcn_readfirstlane((unsigned)ub);
+    const unsigned bhi = __builtin_amdgcn_readfirstlane((unsigned)(ub >> 32));
+    return alo != blo || ahi != bhi;
+}
+__device__ inline bool uniform_ne(float a, float b)
+{
+    return __builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, a)) !=
+           __builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, b));
+}
+
 template <typename T>
 struct Cplx2;
 template <>
@@ -153,7 +171,11 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
         if (active) {
             for (int e = 0; e < cnt; ++e) {
                 const Event<T> ev = sev[e];
-                if (ev.dt != dt_prev) {  // BlochSimulator.cpp:287-294
+                // BlochSimulator.cpp:287-294: e1/e2 only when dt changes.  dt
+                // is the same for every lane (LDS broadcast), so test it on
+                // the scalar unit: a real branch, not two exp per event
+                // under a lane mask.
+                if (uniform_ne(ev.dt, dt_prev)) {
                     e1 = dev_exp<T>(-ev.dt * invT1);
                     e2 = dev_exp<T>(-ev.dt * invT2);
                     dt_prev = ev.dt;
