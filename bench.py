@@ -193,7 +193,7 @@ def serving_run(dev, n_requests: int, rate: float, max_active: int = 8):
                                                      ("radial", (256, 64, 1), "mixed")])]
     be = GpuBackend(dev=dev)
     t0 = time.perf_counter()
-    be.prepare(reqs + warm)
+    be.prepare(reqs + warm, slots=max_active)
     prep = time.perf_counter() - t0
     Scheduler(be, max_active=max_active).run(warm, keep_kspace=False)
     log(f"serving: {len(reqs)} requests at {rate}/s (resident set-up {prep:.1f} s)")

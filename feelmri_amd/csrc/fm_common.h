@@ -55,4 +55,8 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 // Number of CUs on the current device (cached per device).
 int device_cu_count();
 
+// Zero n doubles on the stream with a kernel (not a memset node, so the
+// call can be captured into a HIP graph and replayed).
+int zero_f64_async(double *x, int64_t n, hipStream_t st);
+
 }  // namespace fm

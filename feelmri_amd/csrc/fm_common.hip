@@ -35,6 +35,20 @@ int device_cu_count()
     return cache[dev];
 }
 
+__global__ void k_zero_f64(int64_t n, double *__restrict__ x)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+int zero_f64_async(double *x, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return FM_OK;
+    k_zero_f64<<<(unsigned)cdiv(n, 256), 256, 0, st>>>(n, x);
+    FM_LAUNCH_CHECK("k_zero_f64");
+    return FM_OK;
+}
+
 }  // namespace fm
 
 extern "C" const char *fm_last_error(void) { return fm::g_last_error.c_str(); }

@@ -544,8 +544,7 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
     const int64_t S = p->n_samples;
     if (S == 0) return FM_OK;
     if (N == 0) {
-        FM_HIP(hipMemsetAsync(out, 0, (size_t)S * V * sizeof(double2), st));
-        return FM_OK;
+        return zero_f64_async(out, 2 * S * V, st);
     }
     const bool motion = modes != nullptr && R > 0;
     // positions depend on t, so moving nodes always take the t-grouped tiles;

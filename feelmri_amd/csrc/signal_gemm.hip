@@ -371,8 +371,7 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
     const int64_t S = p->n_samples;
     if (S == 0) return FM_OK;
     if (N == 0) {
-        FM_HIP(hipMemsetAsync(out, 0, (size_t)S * V * sizeof(double2), st));
-        return FM_OK;
+        return zero_f64_async(out, 2 * S * V, st);
     }
     const GemmShape s = gemm_shape(p, precision, N, V);
     if (s.total > ws_bytes) {

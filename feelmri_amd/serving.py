@@ -151,7 +151,10 @@ class Scheduler:
                 if used + fp > self.budget:
                     break  # wait for memory to free up
                 pending.pop(0)
-                st = self.backend.start(req, slot=len(active))
+                busy = {getattr(a, "slot", -1) for a in active}
+                slot = next(i for i in range(self.max_active + 1) if i not in busy)
+                st = self.backend.start(req, slot=slot)
+                st.slot = slot
                 st.t_admit = now
                 st.fp = fp
                 active.append(st)
@@ -204,6 +207,7 @@ class _State:
     t_admit: float = 0.0
     fp: int = 0
     work: int = 0                 # node*k-samples actually computed (phantom class size)
+    slot: int = -1
 
 
 # Cube sides of the resident phantom classes: k^3 nodes from 10.6 k to 2.25 M
@@ -253,6 +257,57 @@ class PhantomStore:
         return self.cache[k]
 
 
+class SteadyStateGraph:
+    """Spoiled-GRE steady state of one resident phantom, captured as a HIP graph.
+
+    The TR loop is recorded once per phantom class and stream slot, then
+    replayed per request as one launch instead of two per block:
+    - per TR one imaging block and one delay block;
+    - each block is an integrator launch and a store/spoil launch.
+
+    The graph owns its state and store buffers. The initial state
+    (Mxy = 0, Mz = M0 = 1) is written by fill kernels inside the graph, so
+    every replay starts from the same state. The stored Mxy of the last TR
+    is ``store_xy[0]``, valid once the replay has run on the slot's stream.
+    """
+
+    def __init__(self, ph: dict, blocks: list, dev, stream):
+        import torch
+
+        from . import device, sequences
+
+        N = ph["mesh"].n_nodes
+        self.events = [device.EventTable(b.rf, b.G, b.dt, device=dev) for b in blocks]
+        self.blocks = blocks
+        self.nodes = ph["bloch"]
+        self.mxy = torch.empty(N, dtype=torch.complex128, device=dev)
+        self.mz = torch.empty(N, dtype=torch.float64, device=dev)
+        self.store_xy = torch.empty((1, N), dtype=torch.complex64, device=dev)
+        self.store_z = torch.empty((1, N), dtype=torch.float32, device=dev)
+        self.gamma = sequences.GAMMA
+        with torch.cuda.stream(stream):
+            self._body()           # eager run: loads the kernels before capture
+        stream.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=stream):
+            self._body()
+
+    def _body(self):
+        from . import device
+
+        self.mxy.zero_()
+        self.mz.fill_(1.0)
+        self.store_xy.zero_()
+        self.store_z.zero_()
+        for b, ev in zip(self.blocks, self.events):
+            device.bloch(self.nodes, ev, 1.0, self.gamma, self.mxy, self.mz, mxy_out=self.mxy, mz_out=self.mz)
+            device.bloch_store_spoil(self.mxy, self.mz, self.store_xy, self.store_z, col=0 if b.store else -1,
+                                     spoil=not b.empty)
+
+    def replay(self):
+        self.graph.replay()
+
+
 def request_trajectory(req: ScanRequest):
     from . import trajectories
 
@@ -276,6 +331,8 @@ class GpuBackend:
         self.chunk_rows = chunk_rows
         self.n_dummy = n_dummy
         self.streams = []
+        self.graphs = {}          # (phantom class, slot) -> SteadyStateGraph
+        self._blocks = None
 
     def _stream(self, slot: int):
         import torch
@@ -300,11 +357,31 @@ class GpuBackend:
             self.plans[key] = (plans, rows)
         return self.plans[key]
 
-    def prepare(self, requests) -> None:
-        """Make the phantoms and plans of ``requests`` resident (service start-up)."""
+    def prepare(self, requests, slots: int = 0) -> None:
+        """Make the phantoms and plans of ``requests`` resident (service
+        start-up), and capture the steady-state graphs of their phantom
+        classes for stream slots 0 .. slots-1."""
         for req in requests:
             self.store.get(req.n_nodes)
             self._chunks(req)
+            for slot in range(slots):
+                self._graph(req.n_nodes, slot)
+
+    def _steady_blocks(self):
+        from . import sequences
+
+        if self._blocks is None:
+            self._blocks = sequences.spoiled_gre(flip_deg=15.0, TR=5.0, n_dummy=self.n_dummy, n_lines=1,
+                                                 rf_dur=0.2)
+        return self._blocks
+
+    def _graph(self, n_nodes: int, slot: int) -> SteadyStateGraph:
+        key = (self.store.key(n_nodes), slot)
+        g = self.graphs.get(key)
+        if g is None:
+            g = SteadyStateGraph(self.store.get(n_nodes), self._steady_blocks(), self.dev, self._stream(slot))
+            self.graphs[key] = g
+        return g
 
     def footprint(self, req: ScanRequest) -> int:
         N = self.store.n_nodes(req.n_nodes)
@@ -324,19 +401,13 @@ class GpuBackend:
         st = _State(req=req, stream=stream, phantom=ph, plans=plans, rows=rows)
         N = ph["mesh"].n_nodes
         st.work = N * req.n_samples
+        graph = self._graph(req.n_nodes, slot)
         with torch.cuda.stream(stream):
-            # spoiled-GRE steady state for this request (its own state buffers)
-            blocks = sequences.spoiled_gre(flip_deg=15.0, TR=5.0, n_dummy=self.n_dummy, n_lines=1, rf_dur=0.2)
-            mxy = torch.zeros(N, dtype=torch.complex128, device=self.dev)
-            mz = torch.ones(N, dtype=torch.float64, device=self.dev)
-            store_xy = torch.zeros((1, N), dtype=torch.complex64, device=self.dev)
-            store_z = torch.zeros((1, N), dtype=torch.float32, device=self.dev)
-            for b in blocks:
-                ev = device.EventTable(b.rf, b.G, b.dt, device=self.dev)
-                device.bloch(ph["bloch"], ev, 1.0, sequences.GAMMA, mxy, mz, mxy_out=mxy, mz_out=mz, stream=stream)
-                device.bloch_store_spoil(mxy, mz, store_xy, store_z, col=0 if b.store else -1, spoil=not b.empty,
-                                         stream=stream)
-            m = (store_xy[0].to(torch.complex128) * ph["M0"]).reshape(-1, 1)
+            # spoiled-GRE steady state: the phantom class's captured TR loop,
+            # replayed on this slot's stream, then copied out of the graph's
+            # buffers before the slot's next replay (same stream, in order)
+            graph.replay()
+            m = (graph.store_xy[0].to(torch.complex128) * ph["M0"]).reshape(-1, 1)
             st.nodes = device.SignalNodes.from_arrays(ph["nodes"], ph["phi"], ph["T2s"], m, weights=ph["w"],
                                                       device=self.dev, stream=stream)
             st.out = torch.empty((req.n_samples, 1), dtype=torch.complex128, device=self.dev)
