@@ -597,26 +597,21 @@ using namespace fm;
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int64_t n_lines,
-                                     int64_t n_kz, const double *kx, const double *ky,
-                                     const double *kz, const double *t, int method)
+namespace fm {
+// Host tables of a plan before upload.
+struct PlanHost {
+    std::vector<int32_t> perm;
+    std::vector<SignalTile> tiles_g, tiles_c;
+    std::vector<double4> ks;
+    std::vector<RunDesc> runs;
+};
+
+// Host analysis of a trajectory: distinct-t groups, Cartesian
+// factorisation, tilings, readout runs.  No device calls.
+static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, const double *kz,
+                        const double *t, int method, PlanHost &h)
 {
-    FM_CHECK_ARG(plan != nullptr, "plan pointer is NULL");
-    *plan = nullptr;
-    FM_CHECK_ARG(n_meas >= 0 && n_lines >= 0 && n_kz >= 0, "negative k-space dimension");
-    FM_CHECK_ARG(method == FM_METHOD_AUTO || method == FM_METHOD_DIRECT || method == FM_METHOD_CARTESIAN,
-                 "unknown method %d", method);
-    const int64_t S = n_meas * n_lines * n_kz;
-    FM_CHECK_ARG(S < (int64_t(1) << 31), "too many k-space samples (%lld)", (long long)S);
-    FM_CHECK_ARG(S == 0 || (kx && ky && kz && t), "NULL k-space array");
-
-    auto *p = new fm_signal_plan();
-    FM_HIP(hipGetDevice(&p->device));
-    p->n_meas = n_meas;
-    p->n_lines = n_lines;
-    p->n_kz = n_kz;
-    p->n_samples = S;
-
+    const int64_t S = p->n_samples, n_meas = p->n_meas, n_lines = p->n_lines, n_kz = p->n_kz;
     // distinct t in first-appearance order (equal values share decay/phase)
     std::unordered_map<uint64_t, int32_t> gid;
     p->sample_group.resize((size_t)S);
@@ -641,7 +636,6 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
     if (method != FM_METHOD_DIRECT && S > 0) {
         const bool ok = cartesian_analyse(p, kx, ky, kz, t);
         if (method == FM_METHOD_CARTESIAN && !ok) {
-            delete p;
             set_error("k-space sampling does not factorise as a Cartesian grid "
                       "(k not additively separable per axis, or t not a function of the readout index)");
             return FM_ERR_UNSUPPORTED;
@@ -653,8 +647,9 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
     // group, so the t-dependent node factors are staged once per tile.
     // Consecutive: plain runs of samples, per-sample t.  Static nodes prefer
     // the grouped tiles when groups fill most of a 512-sample tile.
-    std::vector<int32_t> perm((size_t)S);
-    std::vector<SignalTile> tiles_g, tiles_c;
+    std::vector<int32_t> &perm = h.perm;
+    perm.resize((size_t)S);
+    std::vector<SignalTile> &tiles_g = h.tiles_g, &tiles_c = h.tiles_c;
     {
         std::vector<int64_t> count((size_t)p->n_tgroups + 1, 0);
         for (int64_t s = 0; s < S; ++s) count[(size_t)p->sample_group[(size_t)s] + 1]++;
@@ -678,14 +673,15 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
     p->n_tiles_grouped = (int64_t)tiles_g.size() / kDirectSPL;
     p->n_tiles_consec = (int64_t)tiles_c.size();
 
-    std::vector<double4> ks((size_t)S);
+    std::vector<double4> &ks = h.ks;
+    ks.resize((size_t)S);
     for (int64_t s = 0; s < S; ++s)
         ks[(size_t)s] = make_double4(kx[s] * kPiFOverPi, ky[s] * kPiFOverPi, kz[s] * kPiFOverPi, t[s]);
 
     // Readout runs: kRunLen consecutive readout samples of a line (stride
     // lines * kz in the sample index) whose k' and t are linear in the
     // readout index to 1e-12 of the largest |k'| / |t|.  All runs or none.
-    std::vector<RunDesc> runs;
+    std::vector<RunDesc> &runs = h.runs;
     if (n_meas >= 2 && S > 0) {
         const int64_t stride = n_lines * n_kz;
         double kmax = 0.0, tmax = 0.0;
@@ -733,6 +729,49 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         }
     }
 
+    return FM_OK;
+}
+}  // namespace fm
+
+extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int64_t n_lines,
+                                     int64_t n_kz, const double *kx, const double *ky,
+                                     const double *kz, const double *t, int method)
+{
+    FM_CHECK_ARG(plan != nullptr, "plan pointer is NULL");
+    *plan = nullptr;
+    FM_CHECK_ARG(n_meas >= 0 && n_lines >= 0 && n_kz >= 0, "negative k-space dimension");
+    FM_CHECK_ARG(method == FM_METHOD_AUTO || method == FM_METHOD_DIRECT || method == FM_METHOD_CARTESIAN,
+                 "unknown method %d", method);
+    const int64_t S = n_meas * n_lines * n_kz;
+    FM_CHECK_ARG(S < (int64_t(1) << 31), "too many k-space samples (%lld)", (long long)S);
+    FM_CHECK_ARG(S == 0 || (kx && ky && kz && t), "NULL k-space array");
+
+    auto *p = new fm_signal_plan();
+    p->n_meas = n_meas;
+    p->n_lines = n_lines;
+    p->n_kz = n_kz;
+    p->n_samples = S;
+    {
+        const hipError_t e = hipGetDevice(&p->device);
+        if (e != hipSuccess) {
+            delete p;
+            set_error("hipGetDevice failed: %s", hipGetErrorString(e));
+            return FM_ERR_HIP;
+        }
+    }
+    PlanHost h;
+    {
+        const int rc = plan_analyse(p, kx, ky, kz, t, method, h);
+        if (rc != FM_OK) {
+            delete p;
+            return rc;
+        }
+    }
+    const std::vector<int32_t> &perm = h.perm;
+    const std::vector<SignalTile> &tiles_g = h.tiles_g, &tiles_c = h.tiles_c;
+    const std::vector<double4> &ks = h.ks;
+    const std::vector<RunDesc> &runs = h.runs;
+
     auto fail = [&](hipError_t e, const char *what) {
         set_error("%s failed: %s", what, hipGetErrorString(e));
         fm_signal_plan_destroy(p);
@@ -763,6 +802,32 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         }
     }
     *plan = p;
+    return FM_OK;
+}
+
+extern "C" int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n_kz, const double *kx,
+                                      const double *ky, const double *kz, const double *t, int method,
+                                      int *method_out, int64_t *n_tgroups, int64_t *n_runs,
+                                      int64_t *n_tiles_grouped)
+{
+    FM_CHECK_ARG(n_meas >= 0 && n_lines >= 0 && n_kz >= 0, "negative k-space dimension");
+    FM_CHECK_ARG(method == FM_METHOD_AUTO || method == FM_METHOD_DIRECT || method == FM_METHOD_CARTESIAN,
+                 "unknown method %d", method);
+    const int64_t S = n_meas * n_lines * n_kz;
+    FM_CHECK_ARG(S < (int64_t(1) << 31), "too many k-space samples (%lld)", (long long)S);
+    FM_CHECK_ARG(S == 0 || (kx && ky && kz && t), "NULL k-space array");
+    fm_signal_plan p;
+    p.n_meas = n_meas;
+    p.n_lines = n_lines;
+    p.n_kz = n_kz;
+    p.n_samples = S;
+    PlanHost h;
+    const int rc = plan_analyse(&p, kx, ky, kz, t, method, h);
+    if (rc != FM_OK) return rc;
+    if (method_out) *method_out = p.method;
+    if (n_tgroups) *n_tgroups = p.n_tgroups;
+    if (n_runs) *n_runs = p.n_runs;
+    if (n_tiles_grouped) *n_tiles_grouped = p.n_tiles_grouped;
     return FM_OK;
 }
 

@@ -1,0 +1,68 @@
+"""Host analysis of trajectories (runs on the CPU, no device): the Cartesian
+factorisation test, distinct-t groups, readout runs and t-grouped slots."""
+import numpy as np
+import pytest
+
+from feelmri_amd import trajectories
+from feelmri_amd._lib import FeelmriHipError
+from feelmri_amd.device import analyse_trajectory
+
+
+def test_cartesian_stack_factorises():
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (16, 12, 4), t_first=1.2, t_readout=2.0)
+    a = analyse_trajectory(kloc, t)
+    assert a["method"] == "cartesian"
+    assert a["n_tgroups"] == 16                      # t depends on the readout index only
+    assert a["n_runs"] == 12 * 4 * 1                 # one 16-sample run per line
+    # forcing the direct method keeps the same groups and runs
+    d = analyse_trajectory(kloc, t, method="direct")
+    assert d["method"] == "direct" and d["n_runs"] == a["n_runs"]
+
+
+def test_rotated_grid_factorises_by_general_split():
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (10, 6, 3), t_first=1.0)
+    c, s = np.cos(0.3), np.sin(0.3)
+    kx, ky, kz = kloc
+    rot = (c * kx - s * ky, s * kx + c * ky, kz)
+    assert analyse_trajectory(rot, t)["method"] == "cartesian"
+
+
+def test_time_along_lines_breaks_factorisation():
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (8, 8, 2), t_first=1.0)
+    t2 = t + np.arange(8)[None, :, None] * 5.0       # each line a shot later
+    assert analyse_trajectory(kloc, t2)["method"] == "direct"
+    with pytest.raises(FeelmriHipError, match="does not factorise"):
+        analyse_trajectory(kloc, t2, method="cartesian")
+
+
+def test_non_separable_k_is_direct():
+    rng = np.random.default_rng(1)
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (8, 6, 2), t_first=1.0)
+    kx = kloc[0] + rng.normal(0, 1e-3, kloc[0].shape)    # jitter breaks the additive split
+    assert analyse_trajectory((kx, kloc[1], kloc[2]), t)["method"] == "direct"
+
+
+def test_radial_runs_and_slots():
+    kloc, t = trajectories.radial(0.24, 100, 7, t_first=0.3)
+    a = analyse_trajectory(kloc, t)
+    assert a["method"] == "direct"
+    assert a["n_tgroups"] == 100                     # one group per readout index
+    assert a["n_runs"] == 7 * 7                      # ceil(100 / 16) runs per spoke
+    assert a["n_tiles_grouped"] == 50                # 100 slots of <= 256 samples, two per tile
+
+
+def test_spiral_and_random_have_no_runs():
+    kloc, t = trajectories.spiral(0.24, 64, 200, 8, t_first=0.2)
+    assert analyse_trajectory(kloc, t)["n_runs"] == 0
+    rng = np.random.default_rng(5)
+    kl = tuple(rng.uniform(-200, 200, (30, 7, 2)) for _ in range(3))
+    a = analyse_trajectory(kl, rng.uniform(0.0, 4.0, (30, 7, 2)))
+    assert a["n_runs"] == 0 and a["method"] == "direct" and a["n_tgroups"] == 30 * 7 * 2
+
+
+def test_groups_merge_signed_zero_and_bad_method():
+    kloc = tuple(np.zeros((2, 1, 1)) for _ in range(3))
+    t = np.array([[[0.0]], [[-0.0]]])
+    assert analyse_trajectory(kloc, t, method="direct")["n_tgroups"] == 1
+    with pytest.raises(KeyError):
+        analyse_trajectory(kloc, t, method="nufft")
