@@ -137,6 +137,32 @@ def time_host_io(plan, wl, w_h, mxy_h, form, steps):
     return (time.perf_counter() - t0) / steps * 1e3
 
 
+def time_shim_per_line(wl, w_h, mxy_h, form, reps: int = 3):
+    """The reference examples' call pattern: one ``MRI.Signal`` call per
+    phase-encoding line from host arrays (free_running.py:203-209), through
+    the drop-in shim (plans cached across calls).  Returns ms per full
+    k-space."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    from feelmri_amd import MRI
+
+    M = sp.diags(w_h)
+    lines = wl.t.shape[1]
+    args = (wl.mesh.nodes, wl.tissue.delta_omega, wl.tissue.T2star, mxy_h)
+
+    def once():
+        for j in range(lines):
+            MRI.Signal(0, M, [np.ascontiguousarray(k[:, j:j + 1, :]) for k in wl.kloc],
+                       np.ascontiguousarray(wl.t[:, j:j + 1, :]), *args, precision=form)
+
+    once()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        once()
+    return (time.perf_counter() - t0) / reps * 1e3, lines
+
+
 def load_traffic(kernel: str, config: str, form: str):
     """HBM bytes per launch of ``kernel`` from the committed counter summary
     (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel of
@@ -505,6 +531,12 @@ def main() -> int:
     log(f"{args.form} from host arrays (H2D nodes + pack + execute + D2H k-space): {ms_host:.3f} ms/step")
     forms[args.form]["host_io_ms_per_step"] = ms_host
     forms[args.form]["host_io_node_ksamples_per_s"] = work / (ms_host * 1e-3)
+    if wl.name == "c0":
+        ms_lines, n_lines = time_shim_per_line(wl, w_h, mxy_h, args.form)
+        log(f"{args.form} per-line drop-in calls ({n_lines} x MRI.Signal from host arrays): {ms_lines:.2f} ms "
+            f"per k-space, {ms_lines / n_lines:.3f} ms per call")
+        forms[args.form]["per_line_shim_ms_per_kspace"] = ms_lines
+        forms[args.form]["per_line_shim_calls"] = n_lines
     err_oracle = oracle_subset_error(wl, w_h, mxy_h, out_h)
     log(f"relL2 vs FP64 oracle on sampled k-points: {err_oracle:.2e}")
 

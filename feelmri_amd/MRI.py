@@ -23,6 +23,9 @@ only when t changes (MRI.cpp:192-217).
 """
 from __future__ import annotations
 
+import hashlib
+from collections import OrderedDict
+
 import numpy as np
 import scipy.sparse as sp
 import torch
@@ -103,6 +106,34 @@ def motion_tables(pod_trajectory, n_nodes: int, times: np.ndarray, dev):
     raise NotImplementedError("generic motion callables are evaluated through motion_tables_generic")
 
 
+# Plans of recently used trajectories, keyed by device, method, shape and a
+# digest of the sampling arrays: callers that evaluate the same trajectory
+# repeatedly (per line, per cardiac phase, per encoding) analyse and upload it
+# once.
+_PLAN_CACHE: "OrderedDict[tuple, device.SignalPlan]" = OrderedDict()
+_PLAN_CACHE_SIZE = 16
+
+
+def _plan_for(kloc, t_arr, method: str, dev) -> "device.SignalPlan":
+    arrs = [np.ascontiguousarray(np.asarray(k, dtype=np.float64)) for k in kloc]
+    tt = np.ascontiguousarray(np.asarray(t_arr, dtype=np.float64))
+    h = hashlib.blake2b(digest_size=16)
+    for a in (*arrs, tt):
+        if a.shape != tt.shape:
+            raise ValueError(f"kloc shape {a.shape} differs from t shape {tt.shape}")
+        h.update(memoryview(a).cast("B"))
+    key = (str(dev), method, tt.shape, h.digest())
+    plan = _PLAN_CACHE.get(key)
+    if plan is not None:
+        _PLAN_CACHE.move_to_end(key)
+        return plan
+    plan = device.SignalPlan(arrs, tt, method=method, device=dev)
+    _PLAN_CACHE[key] = plan
+    while len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
+        _PLAN_CACHE.popitem(last=False)
+    return plan
+
+
 def Signal(MPI_rank, M, kloc, t, r0, phi_dB0, T2, Mxy, pod_trajectory=None, *, precision=None,
            method: str = "auto", dev=None):
     """k-space signal (n_meas, n_lines, n_kz, n_enc); see the module docstring."""
@@ -122,7 +153,7 @@ def Signal(MPI_rank, M, kloc, t, r0, phi_dB0, T2, Mxy, pod_trajectory=None, *, p
 
     moving = pod_trajectory is not None
     generic = moving and not Motion.is_supported(pod_trajectory)
-    plan = device.SignalPlan(kloc, t_arr, method="direct" if moving else method, device=dev)
+    plan = _plan_for(kloc, t_arr, "direct" if moving else method, dev)
     with torch.cuda.device(dev):
         mmxy, diag = weighted_mxy(M, Mxy_arr, dev)
         nodes = device.SignalNodes.from_arrays(r0, phi_dB0, T2, mmxy, weights=diag,
