@@ -15,8 +15,9 @@
 //
 // Two kernels differ in where the A operand (Ey * Ezb per column, split into
 // bf16 hi and lo) comes from:
-//   * k_cart_gemm_bf16_lds: the distinct Ey and Ezb rows of the workgroup's
-//     128 columns are DMA'd into LDS two steps ahead (a 3-deep ring), and the
+//   * k_cart_gemm_bf16_lds: 8 waves (256 columns) per workgroup when RB is
+//     even, else 4; the distinct Ey and Ezb rows of the workgroup's
+//     columns are DMA'd into LDS two steps ahead (a 3-deep ring), and the
 //     A operand of step ks+1 is built from LDS in four chunks interleaved
 //     with the four MFMA phases of step ks.  Used when those rows fit the
 //     ring (kYZRows), e.g. 3-D stacks with up to ~64 kz x encodings.
@@ -215,17 +216,42 @@ __global__ __launch_bounds__(kGemmThreads) void k_cart_gemm_bf16(
 // Staged kernel.  Per step the workgroup's 128 columns need nJ distinct Ey
 // rows and nK = min(128, kz*V) distinct Ezb rows (16 nodes, 128 B each).
 constexpr int kYZRows = 72;
+// Waves per workgroup of the staged kernel: 8 (256 columns, one E stage
+// shared by twice the columns, halving E staging per MFMA) when the E stage's
+// RB*4 chunks divide over 8 waves, else 4.
+template <int RB>
+constexpr int lds_waves() { return (RB % 2 == 0) ? 8 : 4; }
 constexpr int kYZStride = 9;  // 16-byte pieces per staged row: 8 + 1 pad
 
-template <int RB>
-__global__ __launch_bounds__(kGemmThreads, 2) void k_cart_gemm_bf16_lds(
+// Copy one E stage (RB tiles of 4 KB) into LDS with WAVES waves: each wave
+// moves RB*4/WAVES chunks of 1 KB (64 lanes x 16 B per instruction).
+template <int RB, int WAVES>
+__device__ inline void stage_E_waves(const uint4 *__restrict__ src, uint4 *dst, int wave, int lane)
+{
+    static_assert((RB * 4) % WAVES == 0, "E stage chunks must divide over the waves");
+    constexpr int kPer = RB * 4 / WAVES;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) {
+        const int chunk = wave * kPer + e;
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void *)(src + chunk * 64 + lane),
+            (__attribute__((address_space(3))) void *)(dst + chunk * 64), 16, 0, 0);
+    }
+}
+
+// WAVES waves per workgroup, 32 columns each (the workgroup tile is
+// RB*32 rows x 32*WAVES columns); the E stage is shared by all of them.
+template <int RB, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 2) void k_cart_gemm_bf16_lds(
     const uint4 *__restrict__ E, const float2 *__restrict__ Ey, const float2 *__restrict__ Ez, int64_t Np,
     int64_t Mp, int64_t M, int64_t NC, int64_t n_kzV, int64_t n_rt, int64_t n_ct, int64_t n_splits,
     int64_t steps_per_split, int64_t ksteps, float2 *__restrict__ part)
 {
+    constexpr int kThreads = 64 * WAVES;
+    constexpr int kCols = 32 * WAVES;
     constexpr int kStage = RB * 4 * 64;
     constexpr int kYZ = kYZRows * kYZStride;
-    constexpr int kPieces = (kYZ + kGemmThreads - 1) / kGemmThreads;
+    constexpr int kPieces = (kYZ + kThreads - 1) / kThreads;
     // Every ring slot is its own LDS variable and the step loop is unrolled
     // by the ring depth, so each LDS read names a different variable than
     // the DMA writes in flight and needs no wait on them.
@@ -238,20 +264,20 @@ __global__ __launch_bounds__(kGemmThreads, 2) void k_cart_gemm_bf16_lds(
     const int64_t k0 = split * steps_per_split;
     const int64_t k1 = (k0 + steps_per_split < ksteps) ? k0 + steps_per_split : ksteps;
 
-    const int64_t c0 = ct * 128;
-    const int64_t clast = (c0 + 127 < NC) ? c0 + 127 : NC - 1;
+    const int64_t c0 = ct * kCols;
+    const int64_t clast = (c0 + kCols - 1 < NC) ? c0 + kCols - 1 : NC - 1;
     const int64_t jf = c0 / n_kzV;
     const int nJ = (int)(clast / n_kzV - jf) + 1;
-    const int nK = (int)(n_kzV < 128 ? n_kzV : 128);
+    const int nK = (int)(n_kzV < kCols ? n_kzV : kCols);
     const int T = kYZStride * (nJ + nK);  // LDS slots per step; nJ + nK <= kYZRows
 
-    // staging: LDS slot s = tid + 256 p holds piece s%9 of row s/9 (the pad
-    // slot 8 repeats piece 7); rows are 144 B apart, so lanes reading the
-    // same piece of consecutive rows hit distinct banks
+    // staging: LDS slot s = tid + kThreads p holds piece s%9 of row s/9 (the
+    // pad slot 8 repeats piece 7); rows are 144 B apart, so lanes reading
+    // the same piece of consecutive rows hit distinct banks
     const float2 *stg[kPieces];
 #pragma unroll
     for (int p = 0; p < kPieces; ++p) {
-        const int s = tid + kGemmThreads * p;
+        const int s = tid + kThreads * p;
         const int row = s / kYZStride;
         const int prt = (s % kYZStride) < 8 ? s % kYZStride : 7;
         const float2 *base = Ey;
@@ -277,11 +303,11 @@ __global__ __launch_bounds__(kGemmThreads, 2) void k_cart_gemm_bf16_lds(
     auto stage_yz = [&](int64_t ks, uint4 *dst) {
 #pragma unroll
         for (int p = 0; p < kPieces; ++p) {
-            if (wave * 64 + kGemmThreads * p < T) {
-                if (tid + kGemmThreads * p < T)
+            if (wave * 64 + kThreads * p < T) {
+                if (tid + kThreads * p < T)
                     __builtin_amdgcn_global_load_lds(
                         (const __attribute__((address_space(1))) void *)(stg[p] + ks * 16),
-                        (__attribute__((address_space(3))) void *)(dst + wave * 64 + kGemmThreads * p), 16, 0, 0);
+                        (__attribute__((address_space(3))) void *)(dst + wave * 64 + kThreads * p), 16, 0, 0);
             }
         }
     };
@@ -301,7 +327,7 @@ __global__ __launch_bounds__(kGemmThreads, 2) void k_cart_gemm_bf16_lds(
 
     BFrag cur;
     if (k0 < k1) {
-        stage_E_to_lds<RB>(E_rt + k0 * nrb * 4 * 64, sE0, wave, lane);
+        stage_E_waves<RB, WAVES>(E_rt + k0 * nrb * 4 * 64, sE0, wave, lane);
         stage_yz(k0, sY0);
         stage_yz(k0 + 1 < k1 ? k0 + 1 : k0, sY1);
         __builtin_amdgcn_s_waitcnt(0);
@@ -325,7 +351,7 @@ __global__ __launch_bounds__(kGemmThreads, 2) void k_cart_gemm_bf16_lds(
         using N2 = std::integral_constant<int, (b + 2) % 3>;
         const int64_t kn1 = (ks + 1 < k1) ? ks + 1 : ks;
         const int64_t kn2 = (ks + 2 < k1) ? ks + 2 : kn1;
-        stage_E_to_lds<RB>(E_rt + kn1 * nrb * 4 * 64, ebuf(N1{}), wave, lane);
+        stage_E_waves<RB, WAVES>(E_rt + kn1 * nrb * 4 * 64, ebuf(N1{}), wave, lane);
         stage_yz(kn2, ybuf(N2{}));
         __builtin_amdgcn_sched_barrier(0);
 
@@ -430,29 +456,38 @@ __global__ __launch_bounds__(kGemmThreads, 2) void k_cart_gemm_bf16_lds(
     }
 }
 
-// Largest number of distinct Ey + Ezb rows any workgroup's 128 columns use.
-static int max_yz_rows(int64_t NC, int64_t n_kzV, int64_t n_ct)
+// Largest number of distinct Ey + Ezb rows any workgroup's `cols` columns use.
+static int max_yz_rows(int64_t NC, int64_t n_kzV, int64_t cols)
 {
     int64_t best = 0;
-    const int64_t nK = n_kzV < 128 ? n_kzV : 128;
-    for (int64_t ct = 0; ct < n_ct; ++ct) {
-        const int64_t c0 = ct * 128, cl = std::min<int64_t>(c0 + 127, NC - 1);
+    const int64_t nK = n_kzV < cols ? n_kzV : cols;
+    for (int64_t ct = 0; ct < cdiv(NC, cols); ++ct) {
+        const int64_t c0 = ct * cols, cl = std::min<int64_t>(c0 + cols - 1, NC - 1);
         best = std::max<int64_t>(best, cl / n_kzV - c0 / n_kzV + 1 + nK);
     }
     return (int)best;
+}
+
+// Staged kernel with WAVES waves when the column tiles' rows fit the ring.
+template <int RB, int WAVES>
+static bool launch_bf16_lds(const GemmShape &s, char *ws, int64_t n_kzV, hipStream_t st)
+{
+    constexpr int kCols = 32 * WAVES;
+    if (max_yz_rows(s.NC, n_kzV, kCols) > kYZRows) return false;
+    const int64_t n_ct = cdiv(s.NC, kCols);
+    const unsigned grid = (unsigned)(s.n_rt * n_ct * s.n_splits);
+    k_cart_gemm_bf16_lds<RB, WAVES><<<grid, 64 * WAVES, 0, st>>>(
+        reinterpret_cast<const uint4 *>(ws + s.off_E), reinterpret_cast<const float2 *>(ws + s.off_Ey),
+        reinterpret_cast<const float2 *>(ws + s.off_Ez), s.Np, s.Mp, s.M, s.NC, n_kzV, s.n_rt, n_ct, s.n_splits,
+        s.steps_per_split, s.ksteps, reinterpret_cast<float2 *>(ws + s.off_part));
+    return true;
 }
 
 template <int RB>
 static void launch_bf16(const GemmShape &s, char *ws, int64_t n_kzV, hipStream_t st)
 {
     const unsigned grid = (unsigned)(s.n_rt * s.n_ct * s.n_splits);
-    if (max_yz_rows(s.NC, n_kzV, s.n_ct) <= kYZRows) {
-        k_cart_gemm_bf16_lds<RB><<<grid, kGemmThreads, 0, st>>>(
-            reinterpret_cast<const uint4 *>(ws + s.off_E), reinterpret_cast<const float2 *>(ws + s.off_Ey),
-            reinterpret_cast<const float2 *>(ws + s.off_Ez), s.Np, s.Mp, s.M, s.NC, n_kzV, s.n_rt, s.n_ct,
-            s.n_splits, s.steps_per_split, s.ksteps, reinterpret_cast<float2 *>(ws + s.off_part));
-        return;
-    }
+    if (launch_bf16_lds<RB, lds_waves<RB>()>(s, ws, n_kzV, st)) return;
     k_cart_gemm_bf16<RB><<<grid, kGemmThreads, 0, st>>>(
         reinterpret_cast<const uint4 *>(ws + s.off_E), reinterpret_cast<const float2 *>(ws + s.off_Ey),
         reinterpret_cast<const float2 *>(ws + s.off_Ez), s.Np, s.Mp, s.M, s.NC, n_kzV, s.n_rt, s.n_ct, s.n_splits,
