@@ -143,7 +143,9 @@ static GemmShape gemm_shape(const fm_signal_plan *p, int precision, int64_t N, i
     s.NCp = s.n_ct * 4 * colblk;
     s.ksteps = s.Np / 16;
     const int64_t base = s.n_rt * s.n_ct;
-    const int64_t target = 4 * (int64_t)device_cu_count();
+    // mixed: one round of workgroups (one 8-wave or two 4-wave workgroups
+    // per CU); FP64: two rounds (two 4-wave workgroups per CU, measured faster)
+    const int64_t target = (s.mixed ? 2 : 4) * (int64_t)device_cu_count();
     int64_t splits = cdiv(target, base);
     splits = std::max<int64_t>(1, std::min<int64_t>(splits, cdiv(s.ksteps, 8)));
     s.steps_per_split = cdiv(s.ksteps, splits);
@@ -206,11 +208,11 @@ __device__ inline void cis_neg_cycles(double u, double &c, double &s)
 // FP64 E table, one thread per (k-step, 4-node step, lane): the thread owns
 // node n = 16 kb + 4 st + l/16 and walks the row blocks (row 16 rb + l%16),
 // so a wave writes 512 contiguous bytes per plane.
-__global__ __launch_bounds__(256) void k_cart_E_f64(const double *__restrict__ packed, int64_t N, int64_t Np,
-                                                    int64_t M, int64_t Mp, const double *__restrict__ F,
-                                                    const double *__restrict__ tr, double *__restrict__ E)
+__device__ __forceinline__ void cart_E_f64(int64_t t, const double *__restrict__ packed, int64_t N, int64_t Np,
+                                           int64_t M, int64_t Mp, const double *__restrict__ F,
+                                           const double *__restrict__ tr, double *__restrict__ E)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over (Np / 16) * 4 * 64
+    // t over (Np / 16) * 4 * 64
     if (t >= (Np / 16) * 256) return;
     const int lane = (int)(t % 64);
     const int st = (int)((t / 64) % 4);
@@ -245,11 +247,11 @@ __global__ __launch_bounds__(256) void k_cart_E_f64(const double *__restrict__ p
 // writing each of the 4 planes of a tile as one 16-byte store (a wave writes
 // 1 KB contiguous per plane).  Phase in FP64 with range reduction, then the
 // hardware FP32 sine/cosine and exp2.
-__global__ __launch_bounds__(256) void k_cart_E_bf16(const double *__restrict__ packed, int64_t N, int64_t Np,
-                                                     int64_t M, int64_t Mp, const double *__restrict__ F,
-                                                     const double *__restrict__ tr, uint4 *__restrict__ E)
+__device__ __forceinline__ void cart_E_bf16(int64_t t, const double *__restrict__ packed, int64_t N, int64_t Np,
+                                            int64_t M, int64_t Mp, const double *__restrict__ F,
+                                            const double *__restrict__ tr, uint4 *__restrict__ E)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // over (Np / 16) * 64
+    // t over (Np / 16) * 64
     if (t >= (Np / 16) * 64) return;
     const int lane = (int)(t % 64);
     const int64_t kb = t / 64;
@@ -311,14 +313,13 @@ __global__ __launch_bounds__(256) void k_cart_E_bf16(const double *__restrict__ 
 constexpr int kYZRowsPerBlock = 16;
 
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_cart_yz(const double *__restrict__ packed, int64_t N, int64_t Np, int V,
-                                                 int64_t n_ln, int64_t n_kz, const double *__restrict__ G,
-                                                 const double *__restrict__ H, void *__restrict__ Ey,
-                                                 void *__restrict__ Ez)
+__device__ __forceinline__ void cart_yz(int64_t n, int64_t rblk, const double *__restrict__ packed, int64_t N,
+                                        int64_t Np, int V, int64_t n_ln, int64_t n_kz,
+                                        const double *__restrict__ G, const double *__restrict__ H,
+                                        void *__restrict__ Ey, void *__restrict__ Ez)
 {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= Np) return;
-    const int64_t r0 = (int64_t)blockIdx.y * kYZRowsPerBlock;
+    const int64_t r0 = rblk * kYZRowsPerBlock;
     const int64_t r1 = (r0 + kYZRowsPerBlock < n_ln + n_kz) ? r0 + kYZRowsPerBlock : n_ln + n_kz;
     const bool ok = n < N;
     const double x = ok ? packed[n] : 0.0, y = ok ? packed[N + n] : 0.0, z = ok ? packed[2 * N + n] : 0.0;
@@ -348,6 +349,30 @@ __global__ __launch_bounds__(256) void k_cart_yz(const double *__restrict__ pack
             }
         }
     }
+}
+
+// Both operand tables in one launch: blocks [0, nb_E) build the E table,
+// the rest the Ey/Ezb rows (nb_n node blocks x row blocks), so the two
+// memory-bound jobs overlap instead of running back to back.
+template <bool MIXED>
+__global__ __launch_bounds__(256) void k_cart_tables(const double *__restrict__ packed, int64_t N, int64_t Np,
+                                                     int64_t M, int64_t Mp, const double *__restrict__ F,
+                                                     const double *__restrict__ tr, void *__restrict__ E,
+                                                     int64_t nb_E, int V, int64_t n_ln, int64_t n_kz,
+                                                     const double *__restrict__ G, const double *__restrict__ H,
+                                                     void *__restrict__ Ey, void *__restrict__ Ez, int64_t nb_n)
+{
+    const int64_t b = blockIdx.x;
+    if (b < nb_E) {
+        const int64_t t = b * 256 + threadIdx.x;
+        if (MIXED)
+            cart_E_bf16(t, packed, N, Np, M, Mp, F, tr, reinterpret_cast<uint4 *>(E));
+        else
+            cart_E_f64(t, packed, N, Np, M, Mp, F, tr, reinterpret_cast<double *>(E));
+        return;
+    }
+    const int64_t c = b - nb_E;
+    cart_yz<MIXED>((c % nb_n) * 256 + threadIdx.x, c / nb_n, packed, N, Np, V, n_ln, n_kz, G, H, Ey, Ez);
 }
 
 template <typename P>
@@ -381,26 +406,22 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
     char *ws = reinterpret_cast<char *>(wsv);
     const int64_t n_kzV = c.n_kz * V;
     {
-        if (s.mixed)
-            k_cart_E_bf16<<<(unsigned)cdiv(s.Np / 16 * 64, 256), 256, 0, st>>>(
-                packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t, reinterpret_cast<uint4 *>(ws + s.off_E));
-        else
-            k_cart_E_f64<<<(unsigned)cdiv(s.Np / 16 * 256, 256), 256, 0, st>>>(
-                packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t, reinterpret_cast<double *>(ws + s.off_E));
-        FM_LAUNCH_CHECK("k_cart_E");
-        if (cdiv(c.n_ln + c.n_kz, kYZRowsPerBlock) > 65535) {
-            set_error("Cartesian plan with %lld lines + kz rows exceeds the operand-table grid",
-                      (long long)(c.n_ln + c.n_kz));
+        const int64_t nb_E = s.mixed ? cdiv(s.Np / 16 * 64, 256) : cdiv(s.Np / 16 * 256, 256);
+        const int64_t nb_n = cdiv(s.Np, 256);
+        const int64_t nb = nb_E + nb_n * cdiv(c.n_ln + c.n_kz, kYZRowsPerBlock);
+        if (nb > (int64_t)0x7fffffff) {
+            set_error("Cartesian operand tables exceed the launch grid (%lld blocks)", (long long)nb);
             return FM_ERR_UNSUPPORTED;
         }
-        const dim3 gyz((unsigned)cdiv(s.Np, 256), (unsigned)cdiv(c.n_ln + c.n_kz, kYZRowsPerBlock));
         if (s.mixed)
-            k_cart_yz<true><<<gyz, 256, 0, st>>>(packed, N, s.Np, V, c.n_ln, c.n_kz, c.d_G, c.d_H, ws + s.off_Ey,
-                                                 ws + s.off_Ez);
+            k_cart_tables<true><<<(unsigned)nb, 256, 0, st>>>(packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t,
+                                                             ws + s.off_E, nb_E, V, c.n_ln, c.n_kz, c.d_G, c.d_H,
+                                                             ws + s.off_Ey, ws + s.off_Ez, nb_n);
         else
-            k_cart_yz<false><<<gyz, 256, 0, st>>>(packed, N, s.Np, V, c.n_ln, c.n_kz, c.d_G, c.d_H, ws + s.off_Ey,
-                                                  ws + s.off_Ez);
-        FM_LAUNCH_CHECK("k_cart_yz");
+            k_cart_tables<false><<<(unsigned)nb, 256, 0, st>>>(packed, N, s.Np, s.M, s.Mp, c.d_F, c.d_t,
+                                                              ws + s.off_E, nb_E, V, c.n_ln, c.n_kz, c.d_G, c.d_H,
+                                                              ws + s.off_Ey, ws + s.off_Ez, nb_n);
+        FM_LAUNCH_CHECK("k_cart_tables");
     }
     if (s.mixed) {
         plan_timing_start(p, st);
