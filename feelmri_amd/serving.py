@@ -7,9 +7,11 @@ trajectory in FP64 or mixed precision.  The loop
 * admits requests in arrival order while their HBM footprint fits the
   budget (288 GB by default) and a stream slot is free; a request larger
   than the whole budget is rejected;
-* keeps phantoms (mesh, tissue, lumped weights) and trajectory plans resident
-  across requests, and each admitted request's Bloch state and packed nodes
-  resident until it completes;
+* keeps phantoms (mesh, tissue, lumped weights; 9 size classes) and
+  trajectory plans resident across requests, and each admitted request's
+  packed nodes and output resident until it completes;
+* runs each request's spoiled-GRE steady state as the replay of a HIP graph
+  captured once per phantom class and stream slot (``SteadyStateGraph``);
 * cuts each request's signal into chunks of readout rows and interleaves the
   chunks of all active requests over their own HIP streams, so a request
   that arrives while others run starts at the next chunk boundary and its
