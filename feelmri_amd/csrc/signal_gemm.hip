@@ -147,6 +147,10 @@ static GemmShape gemm_shape(const fm_signal_plan *p, int precision, int64_t N, i
     // per CU); FP64: two rounds (two 4-wave workgroups per CU, measured faster)
     const int64_t target = (s.mixed ? 2 : 4) * (int64_t)device_cu_count();
     int64_t splits = cdiv(target, base);
+    // mixed: FP32 accumulators sum at most 8192 steps (131 072 nodes) per
+    // split before the FP64 split reduction, bounding the accumulation error
+    // for large meshes
+    if (s.mixed) splits = std::max<int64_t>(splits, cdiv(s.ksteps, 8192));
     splits = std::max<int64_t>(1, std::min<int64_t>(splits, cdiv(s.ksteps, 8)));
     s.steps_per_split = cdiv(s.ksteps, splits);
     s.n_splits = cdiv(s.ksteps, s.steps_per_split);

@@ -95,3 +95,21 @@ def test_non_cartesian_falls_back_to_direct(gpu):
     kloc, t = trajectories.cartesian((0.22, 0.22, 0.22), (8, 8, 2))
     t = t + np.arange(8)[None, :, None] * 0.1
     assert device.SignalPlan(kloc, t, method="auto").method == "direct"
+
+
+def test_large_mesh_mixed_vs_f64(gpu):
+    """3M nodes on a small Cartesian grid: the mixed GEMM keeps its FP32
+    accumulation to bounded node splits (summed in FP64), so its error
+    against the FP64 GEMM stays at the small-mesh level."""
+    from feelmri_amd import device
+
+    kloc, t = trajectories.cartesian((0.22, 0.22, 0.22), (32, 8, 4), t_first=1.0, t_readout=3.0)
+    r0, phi, T2, mxy, w = _nodes(3_000_000, 1, seed=77)
+    plan = device.SignalPlan(kloc, t, method="cartesian", device=gpu)
+    nodes = device.SignalNodes.from_arrays(r0, phi, T2, mxy, weights=w, device=gpu)
+    a = plan.execute(nodes, precision="mixed").cpu().numpy()
+    b = plan.execute(nodes, precision="f64").cpu().numpy()
+    err = rel_l2(a, b)
+    print("3M nodes mixed vs f64", err)
+    assert err <= 1e-4, err
+
