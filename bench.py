@@ -163,6 +163,61 @@ def time_shim_per_line(wl, w_h, mxy_h, form, reps: int = 3):
     return (time.perf_counter() - t0) / reps * 1e3, lines
 
 
+def time_scan(wl, dev, plan, form, reps: int = 3) -> dict:
+    """End-to-end scan on the device: lumped weights, the Bloch solve of the
+    workload's sequence to its stored magnetisation, the packed signal
+    nodes and the k-space.  Mesh colouring, event tables and the plan are
+    set up once beforehand (host preprocessing)."""
+    import torch
+
+    from feelmri_amd import device
+    from feelmri_amd.Bloch import BlochSolver
+
+    m, tis = wl.mesh, wl.tissue
+    col = device.P1Colouring(m.elems, m.n_nodes, device=dev)
+    nodes_d = torch.from_numpy(m.nodes).to(dev)
+    M0 = torch.from_numpy(tis.M0).to(dev)
+    phi = torch.from_numpy(tis.delta_omega).to(dev)
+    T2s = torch.from_numpy(tis.T2star).to(dev)
+    # the mixed form integrates the Bloch equations in FP32, the FP64 form in FP64
+    solver = BlochSolver(wl.blocks, m.nodes, tis.T1, tis.T2, delta_B=tis.delta_B_mT,
+                         precision="f32" if form == "mixed" else "f64", dev=dev)
+    mxy0, mz0 = solver.mxy.clone(), solver.mz.clone()
+    out = torch.empty((plan.n_samples, wl.n_enc), dtype=torch.complex128, device=dev)
+    ws = torch.empty(max(plan.workspace_bytes(form, m.n_nodes, wl.n_enc), 1), dtype=torch.uint8, device=dev)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+
+    def once(record):
+        solver.mxy.copy_(mxy0)
+        solver.mz.copy_(mz0)
+        if record:
+            ev[0].record()
+        w = device.lumped_mass(col, nodes_d)
+        if record:
+            ev[1].record()
+        store, _ = solver.solve_device()
+        if record:
+            ev[2].record()
+        mxy = (store[:, -1].to(torch.complex128) * M0).reshape(-1, 1).repeat(1, wl.n_enc)
+        nodes = device.SignalNodes.from_arrays(nodes_d, phi, T2s, mxy, weights=w, device=dev)
+        plan.execute(nodes, precision=form, out=out, workspace=ws)
+        if record:
+            ev[3].record()
+
+    once(False)
+    torch.cuda.synchronize()
+    parts = [0.0, 0.0, 0.0]
+    for _ in range(reps):
+        once(True)
+        torch.cuda.synchronize()
+        for i in range(3):
+            parts[i] += ev[i].elapsed_time(ev[i + 1]) / reps
+    total = sum(parts)
+    return {"ms": total, "node_ksamples_per_s": wl.node_ksamples / (total * 1e-3),
+            "parts_ms": {"lumped_mass": parts[0], "bloch": parts[1], "pack_and_signal": parts[2]},
+            "bloch_blocks": len(wl.blocks), "bloch_precision": "f32" if form == "mixed" else "f64"}
+
+
 def load_traffic(kernel: str, config: str, form: str):
     """HBM bytes per launch of ``kernel`` from the committed counter summary
     (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel of
@@ -523,6 +578,11 @@ def main() -> int:
 
             shape = (*wl.t.shape, wl.n_enc)
             err_img = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape))
+        elif wl.name.startswith("c2"):
+            from feelmri_amd.recon import adjoint_image_rel_l2
+
+            # 2-D radial / spiral: direct adjoint DFT with ramp density compensation, 128^2 over the FOV
+            err_img = adjoint_image_rel_l2(wl.kloc[0], wl.kloc[1], out_h[:, 0], out64_h[:, 0], 0.24, n=128)
         log(f"f64: {ms64:.3f} ms/step; {args.form} vs f64 relL2 {err_f64:.2e} (image {err_img}); "
             f"speed-up {ms64 / ms_ev:.2f}x")
     w_h = w.cpu().numpy()
@@ -537,6 +597,9 @@ def main() -> int:
             f"per k-space, {ms_lines / n_lines:.3f} ms per call")
         forms[args.form]["per_line_shim_ms_per_kspace"] = ms_lines
         forms[args.form]["per_line_shim_calls"] = n_lines
+    scan = time_scan(wl, dev, plan, args.form)
+    log(f"end-to-end scan (weights + Bloch + pack + signal): {scan['ms']:.3f} ms, "
+        f"{scan['node_ksamples_per_s']:.3e} node*k/s, parts {scan['parts_ms']}")
     err_oracle = oracle_subset_error(wl, w_h, mxy_h, out_h)
     log(f"relL2 vs FP64 oracle on sampled k-points: {err_oracle:.2e}")
 
@@ -611,6 +674,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "serving": srv,
         "rows": rows,
+        "scan": scan,
     }
     print(json.dumps(line), flush=True)
     return 0
