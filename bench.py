@@ -213,7 +213,17 @@ def time_scan(wl, dev, plan, form, reps: int = 3) -> dict:
         for i in range(3):
             parts[i] += ev[i].elapsed_time(ev[i + 1]) / reps
     total = sum(parts)
+    bloch_err = None
+    if form == "mixed":
+        # FP32 steady state against the FP64 integration of the same sequence
+        ref = BlochSolver(wl.blocks, m.nodes, tis.T1, tis.T2, delta_B=tis.delta_B_mT, precision="f64", dev=dev)
+        solver.mxy.copy_(mxy0)
+        solver.mz.copy_(mz0)
+        a = solver.solve_device()[0][:, -1].to(torch.complex128)
+        b = ref.solve_device()[0][:, -1].to(torch.complex128)
+        bloch_err = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
     return {"ms": total, "node_ksamples_per_s": wl.node_ksamples / (total * 1e-3),
+            "bloch_relL2_vs_f64": bloch_err,
             "parts_ms": {"lumped_mass": parts[0], "bloch": parts[1], "pack_and_signal": parts[2]},
             "bloch_blocks": len(wl.blocks), "bloch_precision": "f32" if form == "mixed" else "f64"}
 
