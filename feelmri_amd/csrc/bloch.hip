@@ -29,14 +29,23 @@ __device__ inline void dev_sincos(T x, T *s, T *c);
 template <>
 __device__ inline void dev_sincos<double>(double x, double *s, double *c) { sincos(x, s, c); }
 // FP32 form: the hardware sine/cosine, which take revolutions, after
-// reducing the argument to [-1/2, 1/2] revolution.
+// reducing the argument to [-1/2, 1/2] revolution.  Their error is absolute
+// (a few 1e-7), which is large relative to the small per-event rotations of
+// an RF pulse (e.g. 0.013 rad per 10 us step of a 15 deg, 0.2 ms pulse) and
+// accumulates over the pulse; below 0.25 rad the Taylor series to x^7 / x^6
+// (truncation < 1e-10) is used instead.
 template <>
 __device__ inline void dev_sincos<float>(float x, float *s, float *c)
 {
     float r = x * 0.159154943091895336f;
     r -= __builtin_rintf(r);
-    *s = __builtin_amdgcn_sinf(r);
-    *c = __builtin_amdgcn_cosf(r);
+    const float hs = __builtin_amdgcn_sinf(r), hc = __builtin_amdgcn_cosf(r);
+    const float a = r * 6.28318530717958648f, a2 = a * a;
+    const float ps = a * (1.0f - a2 * (1.0f / 6.0f) * (1.0f - a2 * (1.0f / 20.0f) * (1.0f - a2 * (1.0f / 42.0f))));
+    const float pc = 1.0f - a2 * 0.5f * (1.0f - a2 * (1.0f / 12.0f) * (1.0f - a2 * (1.0f / 30.0f)));
+    const bool small = __builtin_fabsf(a) < 0.25f;
+    *s = small ? ps : hs;
+    *c = small ? pc : hc;
 }
 
 template <typename T>
@@ -45,6 +54,40 @@ template <>
 __device__ inline double dev_exp<double>(double x) { return exp(x); }
 template <>
 __device__ inline float dev_exp<float>(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+
+// T1 recovery Mz' = Mz e1 + (1 - e1) M0.  FP64 form: exactly the reference's
+// expression (BlochSimulator.cpp:271).  FP32 form: Mz - om1 (Mz - M0) with
+// om1 = 1 - e1 = -expm1(x) evaluated in FP64 and rounded once.  The spoiled
+// steady state (1 - E1) / (1 - E1 cos a) magnifies a relative error in E1 by
+// E1 / (1 - E1) (~200 for TR 5 ms, T1 1 s), so e1 rounded to FP32 (or 1 - e1
+// formed in FP32) would bias Mz by ~1e-5; om1 carries the recovery with full
+// FP32 relative precision.  Evaluated only when dt changes.
+template <typename T>
+__device__ inline T one_minus_exp(T x, T e);
+template <>
+__device__ inline double one_minus_exp<double>(double, double e) { return 1.0 - e; }
+template <>
+__device__ inline float one_minus_exp<float>(float x, float) { return (float)(-expm1((double)x)); }
+
+template <typename T>
+__device__ inline T relax_mz(T mz, T e1, T om1, T M0);
+template <>
+__device__ inline double relax_mz<double>(double mz, double e1, double om1, double M0) { return mz * e1 + om1 * M0; }
+template <>
+__device__ inline float relax_mz<float>(float mz, float, float om1, float M0) { return mz - om1 * (mz - M0); }
+
+// Mz coefficient |alpha|^2 - |beta|^2 of the rotation.  FP64 form: the
+// reference's expression (BlochSimulator.cpp:262, :133).  FP32 form: 1 - 2
+// |beta|^2, equal to it for a unitary (alpha, beta).  In FP32, c^2 + s^2
+// misses 1 by up to an ulp, the same way on every event of a readout (same
+// node, same gradient), so a2 - b2 scales Mz systematically during free
+// precession; 1 - 2 b2 is exactly 1 there (beta = 0).
+template <typename T>
+__device__ inline T mz_coef(T a2, T b2);
+template <>
+__device__ inline double mz_coef<double>(double a2, double b2) { return a2 - b2; }
+template <>
+__device__ inline float mz_coef<float>(float, float b2) { return 1.0f - 2.0f * b2; }
 
 template <typename T>
 __device__ inline T dev_sqrt(T x);
@@ -138,7 +181,7 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
             mz_out[p] = mz;
         }
     }
-    T e1 = 0, e2 = 0, dt_prev = (T)-1;
+    T e1 = 0, e2 = 0, om1 = 0, dt_prev = (T)-1;
     const int64_t n_steps = n_time > 0 ? n_time - 1 : 0;
     const int RC = R + 3;
 
@@ -178,6 +221,7 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                 if (uniform_ne(ev.dt, dt_prev)) {
                     e1 = dev_exp<T>(-ev.dt * invT1);
                     e2 = dev_exp<T>(-ev.dt * invT2);
+                    om1 = one_minus_exp<T>(-ev.dt * invT1, e1);
                     dt_prev = ev.dt;
                 }
                 T px = x, py = y, pz = z;
@@ -215,10 +259,10 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                 const T nmr = (T)2 * cab_r * mz + (ca2_r * mxr - ca2_i * mxi) - (bb_r * cmr - bb_i * cmi);
                 const T nmi = (T)2 * cab_i * mz + (ca2_r * mxi + ca2_i * mxr) - (bb_r * cmi + bb_i * cmr);
                 const T ab_r = ar * br - ai * bi, ab_i = ar * bi + ai * br;
-                const T nmz = (a2 - b2) * mz - (T)2 * (ab_r * cmr - ab_i * cmi);
+                const T nmz = mz_coef<T>(a2, b2) * mz - (T)2 * (ab_r * cmr - ab_i * cmi);
                 mxr = nmr * e2;
                 mxi = nmi * e2;
-                mz = nmz * e1 + ((T)1 - e1) * M0;
+                mz = relax_mz<T>(nmz, e1, om1, M0);
                 if (HISTORY) {
                     const int64_t o = p + (base + e + 1) * N;
                     C2 v;
