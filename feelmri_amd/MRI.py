@@ -153,7 +153,11 @@ def Signal(MPI_rank, M, kloc, t, r0, phi_dB0, T2, Mxy, pod_trajectory=None, *, p
 
     moving = pod_trajectory is not None
     generic = moving and not Motion.is_supported(pod_trajectory)
-    plan = _plan_for(kloc, t_arr, "direct" if moving else method, dev)
+    # Supported motion classes keep the requested method: on a Cartesian plan
+    # the mixed form runs the per-readout GEMM and the FP64 form the direct
+    # kernels.  Generic callables are evaluated in passes over the distinct
+    # times, which the direct kernels serve.
+    plan = _plan_for(kloc, t_arr, "direct" if generic else method, dev)
     with torch.cuda.device(dev):
         mmxy, diag = weighted_mxy(M, Mxy_arr, dev)
         nodes = device.SignalNodes.from_arrays(r0, phi_dB0, T2, mmxy, weights=diag,

@@ -47,4 +47,18 @@ __device__ inline void gemm_tile_ids(int64_t n_rt, int64_t n_ct, int64_t n_split
 void launch_cart_gemm_bf16(const GemmShape &s, char *ws, int64_t n_kzV, hipStream_t st);
 void launch_cart_gemm_f64(const GemmShape &s, char *ws, int64_t n_kzV, hipStream_t st);
 
+// Moving nodes on a Cartesian stack (cart_moving_bf16.hip): per readout index
+// a (lines x nodes) by (nodes x kz*V) product with in-kernel operands.
+struct MovingShape {
+    int W, CB;                        // row blocks (waves) and column blocks per workgroup
+    int64_t M, L, K, V, KV;           // readouts, lines, kz, encodings, kz*V
+    int64_t n_rt, n_ct;               // row tiles, column tiles
+    int64_t n_splits, nodes_per_split;
+    int ez_rows;                      // kz rows of one column tile (LDS buffer rows)
+    size_t part_bytes;                // (n_splits, M, L, KV) float2 partial tiles
+};
+MovingShape moving_shape(const CartesianTables &c, int64_t N, int V);
+void launch_cart_moving_bf16(const MovingShape &s, const CartesianTables &c, const double *packed, int64_t N,
+                             const float *modes, int R, const double *mcoef, float2 *part, hipStream_t st);
+
 }  // namespace fm

@@ -907,10 +907,15 @@ extern "C" int fm_signal_workspace_size(const fm_signal_plan *p, int precision, 
 {
     FM_CHECK_ARG(p != nullptr && bytes != nullptr, "NULL argument");
     FM_CHECK_ARG(n_nodes >= 0 && n_enc >= 1, "bad sizes");
-    if (p->method == FM_METHOD_CARTESIAN)
-        *bytes = cartesian_workspace(p, precision, n_nodes, n_enc);
-    else
+    // A Cartesian plan runs static nodes on its GEMM and moving nodes on the
+    // moving GEMM (mixed) or the direct kernels (FP64): enough for each.
+    if (p->method == FM_METHOD_CARTESIAN) {
+        size_t b = std::max(cartesian_workspace(p, precision, n_nodes, n_enc), direct_workspace(p, n_nodes, n_enc));
+        if (precision != FM_PREC_F64) b = std::max(b, cartesian_moving_workspace(p, n_nodes, n_enc));
+        *bytes = b;
+    } else {
         *bytes = direct_workspace(p, n_nodes, n_enc);
+    }
     return FM_OK;
 }
 
@@ -954,10 +959,11 @@ extern "C" int fm_signal_execute(const fm_signal_plan *p, int precision, int64_t
     const bool motion = modes != nullptr && n_modes > 0;
     if (p->method == FM_METHOD_CARTESIAN && !motion)
         return cartesian_execute(p, precision, n_nodes, n_enc, packed, workspace, workspace_bytes, out, st);
-    if (p->method == FM_METHOD_CARTESIAN && motion) {
-        set_error("Cartesian GEMM path takes static nodes; build the plan with FM_METHOD_DIRECT for moving nodes");
-        return FM_ERR_UNSUPPORTED;
-    }
+    // moving nodes on a Cartesian plan: the mixed form takes the per-readout
+    // GEMM (up to kMovMaxV encodings), the FP64 form the direct kernels
+    if (p->method == FM_METHOD_CARTESIAN && motion && precision == FM_PREC_MIXED && n_enc <= kMovMaxV)
+        return cartesian_moving_execute(p, n_nodes, n_enc, packed, modes, n_modes, motion_coef, workspace,
+                                        workspace_bytes, out, st);
     return direct_execute(p, precision, n_nodes, n_enc, packed, modes, n_modes, motion_coef, workspace,
                           workspace_bytes, out, st);
 }

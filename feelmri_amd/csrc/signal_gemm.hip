@@ -55,6 +55,8 @@ bool cartesian_analyse(fm_signal_plan *p, const double *kx, const double *ky, co
     c.H.assign((size_t)K * 3, 0.0);
     c.t.resize((size_t)R);
     for (int64_t i = 0; i < R; ++i) c.t[(size_t)i] = at(t, i, 0, 0);
+    c.ro_group.resize((size_t)R);
+    for (int64_t i = 0; i < R; ++i) c.ro_group[(size_t)i] = p->sample_group[(size_t)(i * L * K)];
     const double *ax[3] = {kx, ky, kz};
     for (int a = 0; a < 3; ++a) {
         const double *k3 = ax[a];
@@ -107,6 +109,8 @@ int cartesian_upload(fm_signal_plan *p)
     FM_HIP(hipMemcpy(c.d_G, c.G.data(), sizeof(double) * c.G.size(), hipMemcpyHostToDevice));
     FM_HIP(hipMemcpy(c.d_H, c.H.data(), sizeof(double) * c.H.size(), hipMemcpyHostToDevice));
     FM_HIP(hipMemcpy(c.d_t, c.t.data(), sizeof(double) * c.t.size(), hipMemcpyHostToDevice));
+    FM_HIP(hipMalloc(&c.d_ro_group, sizeof(int32_t) * c.ro_group.size()));
+    FM_HIP(hipMemcpy(c.d_ro_group, c.ro_group.data(), sizeof(int32_t) * c.ro_group.size(), hipMemcpyHostToDevice));
     return FM_OK;
 }
 
@@ -117,7 +121,9 @@ void cartesian_free(fm_signal_plan *p)
     (void)hipFree(c.d_G);
     (void)hipFree(c.d_H);
     (void)hipFree(c.d_t);
+    (void)hipFree(c.d_ro_group);
     c.d_F = c.d_G = c.d_H = c.d_t = nullptr;
+    c.d_ro_group = nullptr;
 }
 
 
@@ -444,6 +450,47 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
         k_cart_reduce<double2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(
             reinterpret_cast<const double2 *>(ws + s.off_part), s.n_splits, MNC, reinterpret_cast<double2 *>(out));
     }
+    FM_LAUNCH_CHECK("k_cart_reduce");
+    return FM_OK;
+}
+
+size_t cartesian_moving_workspace(const fm_signal_plan *p, int64_t N, int V)
+{
+    if (p->n_samples == 0 || N == 0 || V > kMovMaxV) return 0;
+    return moving_shape(p->cart, N, V).part_bytes;
+}
+
+int cartesian_moving_execute(const fm_signal_plan *p, int64_t N, int V, const double *packed, const float *modes,
+                             int R, const double *mcoef, void *ws, size_t ws_bytes, double *out, hipStream_t st)
+{
+    const CartesianTables &c = p->cart;
+    const int64_t S = p->n_samples;
+    if (S == 0) return FM_OK;
+    if (N == 0) {
+        return zero_f64_async(out, 2 * S * V, st);
+    }
+    if (V > kMovMaxV) {
+        set_error("moving Cartesian GEMM takes at most %d encodings per call (got %d)", kMovMaxV, V);
+        return FM_ERR_UNSUPPORTED;
+    }
+    const MovingShape s = moving_shape(c, N, V);
+    if (s.part_bytes > ws_bytes) {
+        set_error("signal workspace too small: need %zu bytes, got %zu", s.part_bytes, ws_bytes);
+        return FM_ERR_WORKSPACE;
+    }
+    const int64_t grid = s.n_splits * s.M * s.n_rt * s.n_ct;
+    if (grid > (int64_t)0x7fffffff) {
+        set_error("moving Cartesian GEMM exceeds the launch grid (%lld workgroups)", (long long)grid);
+        return FM_ERR_UNSUPPORTED;
+    }
+    float2 *part = reinterpret_cast<float2 *>(ws);
+    plan_timing_start(p, st);
+    launch_cart_moving_bf16(s, c, packed, N, modes, R, mcoef, part, st);
+    plan_timing_stop(p, st);
+    FM_LAUNCH_CHECK("k_cart_moving_bf16");
+    const int64_t MNC = s.M * s.L * s.KV;
+    k_cart_reduce<float2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(part, s.n_splits, MNC,
+                                                                     reinterpret_cast<double2 *>(out));
     FM_LAUNCH_CHECK("k_cart_reduce");
     return FM_OK;
 }

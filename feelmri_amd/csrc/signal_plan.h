@@ -14,6 +14,8 @@ constexpr int kDirectTile = kDirectThreads * kDirectSPL;
 constexpr int kMaxVB = 4;
 // Upper bound on POD modes fused into the loaders.
 constexpr int kMaxModes = 32;
+// Encodings per launch of the moving-node Cartesian GEMM (node records in LDS).
+constexpr int kMovMaxV = 4;
 
 // Readout runs (mixed-form direct kernel for static nodes): kRunLen
 // consecutive readout samples of one line whose k and t are linear in the
@@ -43,7 +45,9 @@ struct CartesianTables {
     // already multiplied by (double)(float)pi / pi.
     std::vector<double> F, G, H;  // (n_ro,3), (n_ln,3), (n_kz,3)
     std::vector<double> t;        // (n_ro)
+    std::vector<int32_t> ro_group;  // (n_ro) t-group of each readout index
     double *d_F = nullptr, *d_G = nullptr, *d_H = nullptr, *d_t = nullptr;
+    int32_t *d_ro_group = nullptr;
 };
 
 }  // namespace fm
@@ -96,4 +100,9 @@ size_t cartesian_workspace(const fm_signal_plan *p, int precision, int64_t n_nod
 int cartesian_execute(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc,
                       const double *packed, void *ws, size_t ws_bytes, double *out,
                       hipStream_t stream);
+// Moving nodes, mixed form (n_enc <= kMovMaxV).
+size_t cartesian_moving_workspace(const fm_signal_plan *p, int64_t n_nodes, int n_enc);
+int cartesian_moving_execute(const fm_signal_plan *p, int64_t n_nodes, int n_enc, const double *packed,
+                             const float *modes, int n_modes, const double *motion_coef, void *ws,
+                             size_t ws_bytes, double *out, hipStream_t stream);
 }  // namespace fm

@@ -1,0 +1,113 @@
+"""Moving nodes on Cartesian stacks: the per-readout GEMM (mixed form) against
+the FP64 oracle and the direct kernels.
+
+With POD motion the positions change with t (MRI.cpp:192-217); on a Cartesian
+stack t depends on the readout index only, so each readout's samples share
+one set of positions and the sum over nodes is a (lines x nodes) by
+(nodes x kz*V) product that the mixed form runs on the matrix cores with
+in-kernel operands.  The FP64 form of the same plan runs the direct kernels.
+
+Shapes cover every workgroup configuration of the kernel (1-4 row blocks per
+workgroup, 1-4 column blocks, several row and column tiles, partial blocks),
+node counts around the 64 W-node record chunk and the 16-node step, one to
+four encodings, translation-only motion, and readouts that share a t.
+Tolerances: mixed <= 1e-4 (bf16 hi+lo products, FP32 accumulation), FP64
+<= 1e-10.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from feelmri_amd import Motion, trajectories
+
+from test_gpu_signal import _nodes, _pod, descriptor_callable, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL_F64 = 1e-10
+TOL_MIXED = 1e-4
+
+
+def _plan_method(kloc, t):
+    from feelmri_amd import device
+
+    return device.analyse_trajectory(kloc, t, "auto")["method"]
+
+
+def _run(kloc, t, n, v, motion, seed=3, oracle_motion=None, gpu=None):
+    from feelmri_amd import MRI
+    import oracle
+
+    r0, phi, T2, mxy, w = _nodes(n, v, seed)
+    M = sp.diags(w)
+    ref = oracle.Signal(0, M, kloc, t, r0, phi, T2, mxy, oracle_motion or descriptor_callable(motion))
+    gemm = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, motion, precision="mixed")
+    direct = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, motion, precision="mixed", method="direct")
+    f64 = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, motion, precision="f64")
+    assert gemm.shape == ref.shape == (*np.asarray(t).shape, v)
+    errs = {"gemm": rel_l2(gemm, ref), "direct": rel_l2(direct, ref), "f64": rel_l2(f64, ref)}
+    print(f"n={n} v={v} shape={np.asarray(t).shape}:", {k: f"{e:.2e}" for k, e in errs.items()})
+    assert errs["gemm"] <= TOL_MIXED, errs
+    assert errs["direct"] <= TOL_MIXED, errs
+    assert errs["f64"] <= TOL_F64, errs
+    return errs
+
+
+@pytest.mark.parametrize("shape,v", [
+    ((12, 6, 2), 1),      # 1 row block, 1 column block
+    ((10, 40, 3), 4),     # 2 row blocks, KV = 12
+    ((6, 96, 24), 4),     # the 4-D flow tile: 3 x 3 blocks
+    ((5, 130, 5), 3),     # 5 row blocks: two row tiles of 4 waves
+    ((4, 20, 70), 2),     # KV = 140: two column tiles of 4 blocks
+    ((3, 33, 129), 1),    # V = 1: one kz row per column
+])
+def test_shapes(gpu, shape, v):
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), shape, t_first=0.4)
+    assert _plan_method(kloc, t) == "cartesian"
+    n = 1500
+    m = _pod(n)
+    m.update_timeshift(2.0)
+    _run(kloc, t, n, v, m)
+
+
+@pytest.mark.parametrize("n", [1, 17, 191, 192, 193, 4097])
+def test_ragged_node_counts(gpu, n):
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (8, 96, 4), t_first=0.4)
+    _run(kloc, t, n, 2, _pod(n, Motion.PODVelocity, seed=n))
+
+
+def test_translation_only(gpu):
+    """A motion without modes (uniform translation): the R = 0 path."""
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (10, 32, 4), t_first=0.4)
+    resp = Motion.RespiratoryMotion(times=np.linspace(0, 100, 25), data=np.sin(np.linspace(0, 6, 25)) * 0.004,
+                                    is_periodic=True, direction=np.array([0, 1, 0]))
+    _run(kloc, t, 2000, 1, resp)
+
+
+def test_readouts_sharing_t(gpu):
+    """Readouts with equal t share a t-group (one motion coefficient row)."""
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (8, 24, 3), t_first=0.4)
+    t = np.asarray(t).copy()
+    t[1::2] = t[0::2]
+    assert _plan_method(kloc, t) == "cartesian"
+    n = 1200
+    _run(kloc, t, n, 3, _pod(n, seed=5))
+
+
+def test_large_mesh_against_f64(gpu):
+    """300 000 nodes (three node splits) on the 4-D flow tile, mixed GEMM
+    against the FP64 form of the same plan (the direct kernels, which the
+    tests above pin to the oracle)."""
+    from feelmri_amd import MRI
+
+    n = 300_000
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (4, 96, 24), t_first=0.4)
+    r0, phi, T2, mxy, w = _nodes(n, 4, 9)
+    m = _pod(n, seed=9)
+    m.update_timeshift(1.0)
+    M = sp.diags(w)
+    got = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, m, precision="mixed")
+    ref = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, m, precision="f64")
+    err = rel_l2(got, ref)
+    print(f"300k nodes: mixed GEMM vs FP64 relL2 {err:.2e}")
+    assert err <= TOL_MIXED
