@@ -88,8 +88,13 @@ __device__ inline void cis_neg_f32(double u, float &c, float &s)
     s = -__builtin_amdgcn_sinf(rf);
 }
 
+// kz rows of the exp(-2 pi i H_k . x) buffer: 16 nodes + 2 entries of pad
+constexpr int kEzStride = 18;
+
+// Up to 3 column blocks the kernel fits 168 VGPRs (3 waves per SIMD);
+// 4 column blocks keep 8 accumulator tiles per wave and take 2.
 template <int W, int CB>
-__global__ __launch_bounds__(64 * W, 2) void k_cart_moving_bf16(
+__global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
     const double *__restrict__ packed, int64_t N, int V, const float *__restrict__ modes, int R,
     const double *__restrict__ mcoef, const int32_t *__restrict__ ro_group, const double *__restrict__ F,
     const double *__restrict__ G, const double *__restrict__ H, const double *__restrict__ tr, int64_t M,
@@ -102,11 +107,14 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_bf16(
     constexpr int kCols = 32 * CB;         // columns of the workgroup tile
     constexpr int kBF = (CB * 64 + kThreads - 1) / kThreads;  // B fragments per thread
     __shared__ double sx[kChunk], sy[kChunk], sz[kChunk];
-    __shared__ float2 sc[kMovMaxV][kChunk];
+    // rows padded by 2 entries (16 B): the B build reads rows of different
+    // encodings / kz in one instruction, and unpadded strides (a multiple of
+    // 128 B) put them on the same LDS banks
+    __shared__ float2 sc[kMovMaxV][kChunk + 2];
     __shared__ uint4 sB[2][CB * 4 * 64];
     __shared__ double sH[3 * (kCols + 1)];
     __shared__ double scoef[kMaxModes + 3];
-    extern __shared__ float2 sEz[];  // 2 buffers of ez_rows * 16 (kz row, node)
+    extern __shared__ float2 sEz[];  // 2 buffers of ez_rows rows of kEzStride (kz row, node)
 
     // workgroup -> (split, readout index, row tile, column tile); the ids of
     // one split are contiguous and dealt onto one XCD
@@ -163,7 +171,7 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_bf16(
 #pragma unroll
         for (int q = 0; q < 16; ++q) acc_re[cb][q] = acc_im[cb][q] = 0.0f;
 
-    const int ez_buf = ez_rows * 16;
+    const int ez_buf = ez_rows * kEzStride;
     int pz = 0, pb = 0;
     const int half = lane / 32;
 
@@ -175,7 +183,7 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_bf16(
             const double u = fma(sH[3 * kk], sx[m], fma(sH[3 * kk + 1], sy[m], sH[3 * kk + 2] * sz[m]));
             float c, sn;
             cis_neg_f32(u, c, sn);
-            dst[e] = make_float2(c, sn);
+            dst[kk * kEzStride + e % 16] = make_float2(c, sn);
         }
     };
 
@@ -237,7 +245,7 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_bf16(
 #pragma unroll
                         for (int q = 0; q < 8; ++q) {
                             const int m = 8 * half + q;
-                            const float2 a = ez[bkk[r] * 16 + m];
+                            const float2 a = ez[bkk[r] * kEzStride + m];
                             const float2 b = sc[bv[r]][s * 16 + m];
                             br_[q] = bok[r] ? a.x * b.x - a.y * b.y : 0.0f;
                             bi_[q] = bok[r] ? a.x * b.y + a.y * b.x : 0.0f;
@@ -340,7 +348,7 @@ static void launch_moving_wc(const MovingShape &s, const CartesianTables &c, con
                              const float *modes, int R, const double *mcoef, float2 *part, hipStream_t st)
 {
     const int64_t grid = s.n_splits * s.M * s.n_rt * s.n_ct;
-    const size_t dyn = 2 * (size_t)s.ez_rows * 16 * sizeof(float2);
+    const size_t dyn = 2 * (size_t)s.ez_rows * kEzStride * sizeof(float2);
     k_cart_moving_bf16<W, CB><<<(unsigned)grid, 64 * W, dyn, st>>>(
         packed, N, (int)s.V, modes, R, mcoef, c.d_ro_group, c.d_F, c.d_G, c.d_H, c.d_t, s.M, s.L, s.K, s.n_rt,
         s.n_ct, s.n_splits, s.nodes_per_split, s.ez_rows, part);
