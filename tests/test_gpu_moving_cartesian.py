@@ -111,3 +111,14 @@ def test_large_mesh_against_f64(gpu):
     err = rel_l2(got, ref)
     print(f"300k nodes: mixed GEMM vs FP64 relL2 {err:.2e}")
     assert err <= TOL_MIXED
+
+
+def test_lines_out_of_order(gpu):
+    """Phase-encode lines in shuffled order (still an additive Cartesian
+    split, with the line coordinates in arbitrary order)."""
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (6, 40, 3), t_first=0.4)
+    perm = np.random.default_rng(4).permutation(40)
+    kloc = tuple(np.ascontiguousarray(np.asarray(k)[:, perm, :]) for k in kloc)
+    assert _plan_method(kloc, t) == "cartesian"
+    n = 1500
+    _run(kloc, t, n, 2, _pod(n, seed=8))
