@@ -303,6 +303,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
     const int64_t rid = (int64_t)blockIdx.x * kDirectThreads + tid;
     const bool valid = rid < n_runs;
     const RunDesc rd = runs[valid ? rid : 0];
+    const float dkx = (float)rd.dkx, dky = (float)rd.dky, dkz = (float)rd.dkz, dtf = (float)rd.dt;
 
     const double *px = packed;
     const double *py = packed + N;
@@ -363,9 +364,9 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
             float er = d0 * __builtin_amdgcn_cosf(r);
             float ei = -d0 * __builtin_amdgcn_sinf(r);
             // step factor
-            float du = fmaf(rd.dkx, nd.xf, fmaf(rd.dky, nd.yf, fmaf(rd.dkz, nd.zf, -nd.dwcf * rd.dt)));
+            float du = fmaf(dkx, nd.xf, fmaf(dky, nd.yf, fmaf(dkz, nd.zf, -nd.dwcf * dtf)));
             du -= __builtin_rintf(du);
-            const float ds = __builtin_amdgcn_exp2f(-rd.dt * nd.r2f * 1.44269504f);
+            const float ds = __builtin_amdgcn_exp2f(-dtf * nd.r2f * 1.44269504f);
             const float qr = ds * __builtin_amdgcn_cosf(du);
             const float qi = -ds * __builtin_amdgcn_sinf(du);
 #pragma unroll
@@ -388,6 +389,106 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
                 tot_re[m][v] += (double)acc_re[m][v];
                 tot_im[m][v] += (double)acc_im[m][v];
             }
+    }
+
+    if (!valid) return;
+#pragma unroll
+    for (int m = 0; m < kRunLen; ++m) {
+        if (m >= rd.len) continue;
+        const int64_t sid = (int64_t)rd.s0 + m * stride;
+        const int64_t row = write_final ? sid : split * S + sid;
+#pragma unroll
+        for (int v = 0; v < VB; ++v) dst[row * V + v0 + v] = make_double2(tot_re[m][v], tot_im[m][v]);
+    }
+}
+
+// FP64 form, static nodes, readout runs: the same recurrence in FP64
+// (FP64 phase and sincospi for the anchor and the step, FP64 products and
+// accumulators).  Taken only when every run is linear to 1e-14 of the
+// largest |k'| and |t|; 15 FP64 complex products then leave the result
+// ~1e-15 from the per-sample sum of MRI.cpp:343-371.
+template <int VB>
+struct alignas(16) RunNode64 {
+    double x, y, z, dwc, r2, pad;
+    double c[2 * VB];
+};
+
+template <int VB>
+__global__ __launch_bounds__(kDirectThreads) void k_signal_runs_f64(
+    const double *__restrict__ packed, int64_t N, int V, int v0, const RunDesc *__restrict__ runs,
+    int64_t n_runs, int64_t stride, int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S,
+    int write_final)
+{
+    __shared__ RunNode64<VB> snode[kDirectThreads];
+    const int64_t split = blockIdx.y;
+    const int64_t n_begin = split * nodes_per_split;
+    const int64_t n_end = (n_begin + nodes_per_split < N) ? n_begin + nodes_per_split : N;
+    const int tid = threadIdx.x;
+    const int64_t rid = (int64_t)blockIdx.x * kDirectThreads + tid;
+    const bool valid = rid < n_runs;
+    const RunDesc rd = runs[valid ? rid : 0];
+
+    double tot_re[kRunLen][VB], tot_im[kRunLen][VB];
+#pragma unroll
+    for (int m = 0; m < kRunLen; ++m)
+#pragma unroll
+        for (int v = 0; v < VB; ++v) tot_re[m][v] = tot_im[m][v] = 0.0;
+
+    for (int64_t base = n_begin; base < n_end; base += kDirectThreads) {
+        __syncthreads();
+        {
+            const int64_t n = base + tid;
+            RunNode64<VB> nd;
+            nd.pad = 0.0;
+            if (n < n_end) {
+                nd.x = packed[n];
+                nd.y = packed[N + n];
+                nd.z = packed[2 * N + n];
+                nd.dwc = packed[3 * N + n] * (1.0 / kTwoPi);
+                nd.r2 = packed[4 * N + n];
+#pragma unroll
+                for (int v = 0; v < VB; ++v) {
+                    nd.c[2 * v] = packed[(int64_t)(5 + 2 * (v0 + v)) * N + n];
+                    nd.c[2 * v + 1] = packed[(int64_t)(6 + 2 * (v0 + v)) * N + n];
+                }
+            } else {
+                nd.x = nd.y = nd.z = nd.dwc = nd.r2 = 0.0;
+#pragma unroll
+                for (int v = 0; v < 2 * VB; ++v) nd.c[v] = 0.0;
+            }
+            snode[tid] = nd;
+        }
+        __syncthreads();
+        const int cnt = (n_end - base < kDirectThreads) ? (int)(n_end - base) : kDirectThreads;
+        for (int i = 0; i < cnt; ++i) {
+            const RunNode64<VB> &nd = snode[i];
+            // anchor: the first sample's term, as the per-sample sum forms it
+            const double u = fma(rd.k0x, nd.x, fma(rd.k0y, nd.y, fma(rd.k0z, nd.z, -nd.dwc * rd.t0)));
+            const double r = u - __builtin_rint(u);
+            double sn, cs;
+            sincospi(2.0 * r, &sn, &cs);
+            const double d0 = exp(-rd.t0 * nd.r2);
+            double er = d0 * cs, ei = -d0 * sn;
+            // step factor
+            double du = fma(rd.dkx, nd.x, fma(rd.dky, nd.y, fma(rd.dkz, nd.z, -nd.dwc * rd.dt)));
+            du -= __builtin_rint(du);
+            double qs, qc;
+            sincospi(2.0 * du, &qs, &qc);
+            const double ds = exp(-rd.dt * nd.r2);
+            const double qr = ds * qc, qi = -ds * qs;
+#pragma unroll
+            for (int m = 0; m < kRunLen; ++m) {
+#pragma unroll
+                for (int v = 0; v < VB; ++v) {
+                    const double cr = nd.c[2 * v], ci = nd.c[2 * v + 1];
+                    tot_re[m][v] = fma(er, cr, fma(-ei, ci, tot_re[m][v]));
+                    tot_im[m][v] = fma(er, ci, fma(ei, cr, tot_im[m][v]));
+                }
+                const double nr = er * qr - ei * qi;
+                ei = fma(er, qi, ei * qr);
+                er = nr;
+            }
+        }
     }
 
     if (!valid) return;
@@ -548,8 +649,9 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
     }
     const bool motion = modes != nullptr && R > 0;
     // positions depend on t, so moving nodes always take the t-grouped tiles;
-    // static nodes in the mixed form take the readout runs when the plan has them
-    const bool use_runs = !motion && precision != FM_PREC_F64 && p->n_runs > 0;
+    // static nodes take the readout runs when the plan has them (the FP64
+    // form only when they are linear to 1e-14)
+    const bool use_runs = !motion && p->n_runs > 0 && (precision != FM_PREC_F64 || p->runs_exact);
     const bool grouped = motion || p->prefer_grouped;
     const int tiling = use_runs ? 2 : (grouped ? 1 : 0);
     const DirectLaunch L = direct_launch_shape(p, N, tiling);
@@ -566,9 +668,14 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
         // one encoding per launch: 16 samples x 1 encoding of accumulators
         // keeps the kernel at ~130 VGPRs (3 waves per SIMD)
         for (int v0 = 0; v0 < V; ++v0) {
-            k_signal_runs<1><<<grid, kDirectThreads, 0, st>>>(packed, N, V, v0, p->d_runs, p->n_runs,
-                                                              p->run_stride, L.nodes_per_split, dst, S,
-                                                              write_final);
+            if (precision == FM_PREC_F64)
+                k_signal_runs_f64<1><<<grid, kDirectThreads, 0, st>>>(packed, N, V, v0, p->d_runs, p->n_runs,
+                                                                      p->run_stride, L.nodes_per_split, dst, S,
+                                                                      write_final);
+            else
+                k_signal_runs<1><<<grid, kDirectThreads, 0, st>>>(packed, N, V, v0, p->d_runs, p->n_runs,
+                                                                  p->run_stride, L.nodes_per_split, dst, S,
+                                                                  write_final);
             FM_LAUNCH_CHECK("k_signal_runs");
         }
     }
@@ -691,6 +798,7 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
         }
         const double tolk = 1e-12 * (kmax > 0.0 ? kmax : 1.0), tolt = 1e-12 * (tmax > 0.0 ? tmax : 1.0);
         bool linear = true;
+        double devk = 0.0, devt = 0.0;  // largest departure from the linear model
         runs.reserve((size_t)(stride * cdiv(n_meas, kRunLen)));
         for (int64_t line = 0; line < stride && linear; ++line) {
             for (int64_t i0 = 0; i0 < n_meas && linear; i0 += kRunLen) {
@@ -704,18 +812,22 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
                 }
                 for (int64_t m = 2; m < len && linear; ++m) {
                     const double4 c = ks[(size_t)(s0 + m * stride)];
-                    linear = std::fabs(c.x - a.x - m * d.x) <= tolk && std::fabs(c.y - a.y - m * d.y) <= tolk &&
-                             std::fabs(c.z - a.z - m * d.z) <= tolk && std::fabs(c.w - a.w - m * d.w) <= tolt;
+                    const double ek = std::max({std::fabs(c.x - a.x - m * d.x), std::fabs(c.y - a.y - m * d.y),
+                                                std::fabs(c.z - a.z - m * d.z)});
+                    const double et = std::fabs(c.w - a.w - m * d.w);
+                    devk = std::max(devk, ek);
+                    devt = std::max(devt, et);
+                    linear = ek <= tolk && et <= tolt;
                 }
                 RunDesc r{};
                 r.k0x = a.x;
                 r.k0y = a.y;
                 r.k0z = a.z;
                 r.t0 = a.w;
-                r.dkx = (float)d.x;
-                r.dky = (float)d.y;
-                r.dkz = (float)d.z;
-                r.dt = (float)d.w;
+                r.dkx = d.x;
+                r.dky = d.y;
+                r.dkz = d.z;
+                r.dt = d.w;
                 r.s0 = (int32_t)s0;
                 r.len = (int32_t)len;
                 runs.push_back(r);
@@ -724,6 +836,7 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
         if (linear) {
             p->n_runs = (int64_t)runs.size();
             p->run_stride = stride;
+            p->runs_exact = devk <= 1e-2 * tolk && devt <= 1e-2 * tolt;
         } else {
             runs.clear();
         }
@@ -808,7 +921,7 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
 extern "C" int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n_kz, const double *kx,
                                       const double *ky, const double *kz, const double *t, int method,
                                       int *method_out, int64_t *n_tgroups, int64_t *n_runs,
-                                      int64_t *n_tiles_grouped)
+                                      int64_t *n_tiles_grouped, int *runs_exact)
 {
     FM_CHECK_ARG(n_meas >= 0 && n_lines >= 0 && n_kz >= 0, "negative k-space dimension");
     FM_CHECK_ARG(method == FM_METHOD_AUTO || method == FM_METHOD_DIRECT || method == FM_METHOD_CARTESIAN,
@@ -828,6 +941,7 @@ extern "C" int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n
     if (n_tgroups) *n_tgroups = p.n_tgroups;
     if (n_runs) *n_runs = p.n_runs;
     if (n_tiles_grouped) *n_tiles_grouped = p.n_tiles_grouped;
+    if (runs_exact) *runs_exact = p.runs_exact ? 1 : 0;
     return FM_OK;
 }
 
@@ -894,11 +1008,12 @@ extern "C" int fm_signal_plan_info(const fm_signal_plan *p, int64_t *n_samples, 
     return FM_OK;
 }
 
-extern "C" int fm_signal_plan_runs(const fm_signal_plan *p, int64_t *n_runs, int *run_len)
+extern "C" int fm_signal_plan_runs(const fm_signal_plan *p, int64_t *n_runs, int *run_len, int *runs_exact)
 {
     FM_CHECK_ARG(p != nullptr, "plan is NULL");
     if (n_runs) *n_runs = p->n_runs;
     if (run_len) *run_len = kRunLen;
+    if (runs_exact) *runs_exact = p->runs_exact ? 1 : 0;
     return FM_OK;
 }
 

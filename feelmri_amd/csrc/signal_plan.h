@@ -25,7 +25,7 @@ constexpr int kRunLen = 16;
 
 struct RunDesc {
     double k0x, k0y, k0z, t0;  // first sample (k' in cycles/m, t in ms)
-    float dkx, dky, dkz, dt;   // step per sample
+    double dkx, dky, dkz, dt;  // step per sample
     int32_t s0;                // first sample index; sample m is s0 + m * run_stride
     int32_t len;               // samples in the run (<= kRunLen)
     int32_t pad[2];
@@ -71,6 +71,9 @@ struct fm_signal_plan {
     double *d_tg_t = nullptr;
     // readout runs (empty when some run is not linear in k and t)
     int64_t n_runs = 0, run_stride = 0;
+    // every run linear to 1e-14 of the largest |k'| and |t| (not only the
+    // 1e-12 of run detection): the FP64 form may then take the runs too
+    bool runs_exact = false;
     fm::RunDesc *d_runs = nullptr;
     fm::CartesianTables cart;
     // optional timing of the dominant kernel (fm_signal_plan_set_timing)

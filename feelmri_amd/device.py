@@ -62,12 +62,13 @@ def analyse_trajectory(kloc, t, method: str = "auto") -> dict:
     if tt.ndim != 3 or any(k.shape != tt.shape for k in (kx, ky, kz)):
         raise ValueError("kloc arrays and t must share one 3-D shape (n_meas, n_lines, n_kz)")
     meth, ng, nr, nt = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    ex = ctypes.c_int()
     _lib.call("fm_signal_plan_analyse", *tt.shape, kx.ctypes.data_as(ctypes.c_void_p),
               ky.ctypes.data_as(ctypes.c_void_p), kz.ctypes.data_as(ctypes.c_void_p),
               tt.ctypes.data_as(ctypes.c_void_p), METHODS[method], ctypes.byref(meth), ctypes.byref(ng),
-              ctypes.byref(nr), ctypes.byref(nt))
+              ctypes.byref(nr), ctypes.byref(nt), ctypes.byref(ex))
     return {"method": METHOD_NAMES[meth.value], "n_tgroups": ng.value, "n_runs": nr.value,
-            "n_tiles_grouped": nt.value}
+            "n_tiles_grouped": nt.value, "runs_exact": bool(ex.value)}
 
 
 class SignalPlan:
@@ -103,11 +104,12 @@ class SignalPlan:
                   tg.ctypes.data_as(ctypes.c_void_p))
         self.tgroup_t = tg
         self._t = tt
-        nr, rl = ctypes.c_int64(), ctypes.c_int()
-        _lib.call("fm_signal_plan_runs", self._h, ctypes.byref(nr), ctypes.byref(rl))
+        nr, rl, ex = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        _lib.call("fm_signal_plan_runs", self._h, ctypes.byref(nr), ctypes.byref(rl), ctypes.byref(ex))
         # readout runs (k and t linear along every readout line): the mixed
-        # form of static nodes takes the run kernel when n_runs > 0
-        self.n_runs, self.run_len = nr.value, rl.value
+        # form of static nodes takes the run kernel when n_runs > 0, the FP64
+        # form its FP64 run kernel when the runs are also exact to 1e-14
+        self.n_runs, self.run_len, self.runs_exact = nr.value, rl.value, bool(ex.value)
 
     def sample_groups(self) -> np.ndarray:
         """Index into ``tgroup_t`` of every sample, in sample order."""

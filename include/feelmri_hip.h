@@ -77,12 +77,13 @@ int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int64_t n_lines
 int fm_signal_plan_destroy(fm_signal_plan *plan);
 /* Host-only analysis of a trajectory, as fm_signal_plan_create performs it,
  * without a device: the method it would choose, the number of distinct t,
- * of 16-sample readout runs and of t-grouped tiles.  No counterpart in the
- * reference (plan introspection; usable without a GPU). */
+ * of 16-sample readout runs and of t-grouped tiles, and whether the runs are
+ * linear to 1e-14 (runs_exact: the FP64 form takes them too).  No
+ * counterpart in the reference (plan introspection; usable without a GPU). */
 int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n_kz, const double *kx,
                            const double *ky, const double *kz, const double *t, int method,
                            int *method_out, int64_t *n_tgroups, int64_t *n_runs,
-                           int64_t *n_tiles_grouped);
+                           int64_t *n_tiles_grouped, int *runs_exact);
 /* n_tgroups: number of distinct t; method: the method chosen
  * (FM_METHOD_DIRECT or FM_METHOD_CARTESIAN).  tgroup_t (host, may be
  * NULL) receives the n_tgroups distinct times in the order the motion
@@ -90,9 +91,11 @@ int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n_kz, const 
 int fm_signal_plan_info(const fm_signal_plan *plan, int64_t *n_samples, int64_t *n_tgroups,
                         int *method, double *tgroup_t);
 /* Readout runs of the plan: 0 unless k and t are linear in the readout
- * index along every line (then the mixed form of static nodes takes the
- * run kernel).  No counterpart in the reference (plan introspection). */
-int fm_signal_plan_runs(const fm_signal_plan *plan, int64_t *n_runs, int *run_len);
+ * index along every line to 1e-12 (then the mixed form of static nodes takes
+ * the run kernel); runs_exact: linear to 1e-14 (then the FP64 form takes its
+ * FP64 run kernel too).  No counterpart in the reference (plan
+ * introspection). */
+int fm_signal_plan_runs(const fm_signal_plan *plan, int64_t *n_runs, int *run_len, int *runs_exact);
 /* Workspace (device bytes) fm_signal_execute needs for this plan. */
 int fm_signal_workspace_size(const fm_signal_plan *plan, int precision, int64_t n_nodes,
                              int n_enc, size_t *bytes);

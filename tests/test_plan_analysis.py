@@ -66,3 +66,22 @@ def test_groups_merge_signed_zero_and_bad_method():
     assert analyse_trajectory(kloc, t, method="direct")["n_tgroups"] == 1
     with pytest.raises(KeyError):
         analyse_trajectory(kloc, t, method="nufft")
+
+
+def test_runs_exact_for_fp64():
+    """The FP64 form takes the readout runs only when they are linear to
+    1e-14 of the largest |k'| and |t|; the mixed form accepts 1e-12."""
+    kloc, t = trajectories.radial(0.24, 100, 7, t_first=0.3)
+    a = analyse_trajectory(kloc, t)
+    assert a["n_runs"] > 0 and a["runs_exact"]
+    kc, tc = trajectories.cartesian((0.24, 0.24, 0.24), (40, 6, 2), t_first=1.0)
+    assert analyse_trajectory(kc, tc)["runs_exact"]
+    # a departure of ~1e-13 of the largest |k| keeps the runs but not for FP64
+    kx = np.asarray(kloc[0]).copy()
+    kmax = max(np.abs(np.asarray(k)).max() for k in kloc)
+    kx[5, 2, 0] += 1e-13 * kmax
+    b = analyse_trajectory((kx, kloc[1], kloc[2]), t)
+    assert b["n_runs"] == a["n_runs"] and not b["runs_exact"]
+    # ~1e-11 breaks the runs altogether
+    kx[5, 2, 0] += 1e-11 * kmax
+    assert analyse_trajectory((kx, kloc[1], kloc[2]), t)["n_runs"] == 0
