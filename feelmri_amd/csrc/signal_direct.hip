@@ -43,13 +43,14 @@ struct CoefT<false> {
 };
 
 // A node staged in LDS.  Shared-t tiles carry one coefficient set per slot
-// (the slots of a tile may belong to different t-groups), and, for moving
-// nodes, one position per slot.
-template <int VB, bool MIXED, bool SHARED_T, bool MOTION>
+// (the slots of a tile may belong to different t-groups) or, in packed slots
+// of GP t-groups, one per (slot, group); for moving nodes one position per
+// slot.
+template <int VB, bool MIXED, bool SHARED_T, bool MOTION, int GP = 1>
 struct alignas(16) StagedNode {
     using CT = typename CoefT<MIXED>::T;
     static constexpr int kPos = MOTION ? kDirectSPL : 1;
-    static constexpr int kCoef = SHARED_T ? kDirectSPL : 1;
+    static constexpr int kCoef = SHARED_T ? kDirectSPL * GP : 1;
     double x[kPos], y[kPos], z[kPos];
     double dwc;                  // phi/(2 pi), cycles/ms   (per-sample-t kernel only)
     double r2;                   // 1/T2, 1/ms              (per-sample-t kernel only)
@@ -58,10 +59,14 @@ struct alignas(16) StagedNode {
 
 // Tiles.  Shared-t ("grouped") tiles are kDirectSPL slot records
 // tiles[kDirectSPL * b + j]: slot j holds up to 256 samples of one t-group
-// (lane l takes sample perm[offset + l]).  Per-sample-t ("consecutive")
-// tiles are one record of up to 512 consecutive samples (lane l takes
-// offset + l + 256 j).
-template <int VB, bool MIXED, bool SHARED_T, bool MOTION>
+// (lane l takes sample perm[offset + l]).  Packed slots (GP > 1, static
+// nodes) hold GP consecutive t-groups of 256 / GP samples each, starting at
+// group `group`: lane l belongs to group + l / (256 / GP), and the
+// t-dependent factor exp(-t/T2) exp(i phi t) is staged per (node, slot,
+// group) instead of being formed per (node, sample).  Per-sample-t
+// ("consecutive") tiles are one record of up to 512 consecutive samples
+// (lane l takes offset + l + 256 j).
+template <int VB, bool MIXED, bool SHARED_T, bool MOTION, int GP = 1>
 __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     const double *__restrict__ packed, int64_t N, int V, int v0,
     const double4 *__restrict__ ks, const int32_t *__restrict__ perm,
@@ -70,8 +75,10 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S, int write_final)
 {
     using CT = typename CoefT<MIXED>::T;
-    using Node = StagedNode<VB, MIXED, SHARED_T, MOTION>;
+    using Node = StagedNode<VB, MIXED, SHARED_T, MOTION, GP>;
+    static_assert(GP == 1 || (SHARED_T && !MOTION), "packed t-groups: shared-t tiles of static nodes");
     constexpr int kSlots = SHARED_T ? kDirectSPL : 1;
+    constexpr int kGSize = kDirectThreads / GP;  // samples per group of a packed slot
     __shared__ Node snode[kDirectThreads];
     __shared__ double scoef[MOTION ? kDirectSPL : 1][kMaxModes + 3];
 
@@ -90,14 +97,18 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     SignalTile tl[kSlots];
 #pragma unroll
     for (int j = 0; j < kSlots; ++j) tl[j] = tiles[(int64_t)blockIdx.x * kSlots + j];
-    double tg[kDirectSPL];
-    bool same_group = true;  // both slots share one t-group (stage once)
+    // slot (and group) times: uniform over the workgroup, kept in LDS
+    __shared__ double tg[kDirectSPL * GP];
+    bool same_group = GP == 1;  // both slots share one t-group (stage once)
     if (SHARED_T) {
-#pragma unroll
-        for (int j = 0; j < kDirectSPL; ++j) tg[j] = tl[j].count > 0 ? tg_t[tl[j].group] : 0.0;
+        if (tid < kDirectSPL * GP) {
+            const int j = tid / GP, q = tid % GP;
+            tg[tid] = tl[j].count > q * kGSize ? tg_t[tl[j].group + q] : 0.0;
+        }
 #pragma unroll
         for (int j = 1; j < kDirectSPL; ++j) same_group = same_group && (tl[j].group == tl[0].group || tl[j].count == 0);
     }
+    const int gq = GP > 1 ? threadIdx.x / kGSize : 0;  // this lane's group in a packed slot
     if (MOTION) {
 #pragma unroll
         for (int j = 0; j < kDirectSPL; ++j)
@@ -135,7 +146,9 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
         __syncthreads();  // previous chunk fully consumed (and scoef visible)
         {
             const int64_t n = base + tid;
-            Node nd;
+            // staged in place: packed slots carry 2 GP coefficient sets per
+            // node, too many to assemble in registers first
+            Node &nd = snode[tid];
             if (n < n_end) {
                 const double x0 = px[n], y0 = py[n], z0 = pz[n];
 #pragma unroll
@@ -166,7 +179,6 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                 const double phi = pphi[n], r2 = pr2[n];
                 nd.dwc = phi * (1.0 / kTwoPi);
                 nd.r2 = r2;
-#pragma unroll
                 for (int j = 0; j < Node::kCoef; ++j) {
                     if (SHARED_T) {
                         if (j > 0 && same_group) {
@@ -203,7 +215,6 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
 #pragma unroll
                     for (int v = 0; v < 2 * VB; ++v) nd.c[j][v] = (CT)0;
             }
-            snode[tid] = nd;
         }
         __syncthreads();
         const int cnt = (n_end - base < kDirectThreads) ? (int)(n_end - base) : kDirectThreads;
@@ -246,7 +257,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                     sn *= d;
                 }
                 // e = cs - i sn ; acc += e * c
-                const int jc = SHARED_T ? j : 0;
+                const int jc = SHARED_T ? j * GP + gq : 0;
 #pragma unroll
                 for (int v = 0; v < VB; ++v) {
                     const CT cr = nd.c[jc][2 * v], ci = nd.c[jc][2 * v + 1];
@@ -563,9 +574,11 @@ struct DirectLaunch {
     int64_t nodes_per_split;
 };
 
-// Tilings of the direct path: 0 consecutive, 1 t-grouped, 2 readout runs.
+// Tilings of the direct path: 0 consecutive, 1 t-grouped, 2 readout runs,
+// 3 packed t-groups.
 static int64_t direct_tiles(const fm_signal_plan *p, int tiling)
 {
+    if (tiling == 3) return p->n_tiles_packed;
     if (tiling == 2) return cdiv(p->n_runs, kDirectThreads);
     return tiling == 1 ? p->n_tiles_grouped : p->n_tiles_consec;
 }
@@ -624,13 +637,30 @@ static void launch_direct(const fm_signal_plan *p, bool grouped, bool motion, di
 #undef FM_LD
 }
 
+template <bool MIXED>
+static void launch_packed(const fm_signal_plan *p, dim3 grid, const double *packed, int64_t N, int V, int v0,
+                          int64_t nps, double2 *dst, int write_final, hipStream_t st)
+{
+#define FM_LP(GPV)                                                                                          \
+    k_signal_direct<1, MIXED, true, false, GPV><<<grid, kDirectThreads, 0, st>>>(                           \
+        packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles_packed, p->d_tg_t, nullptr, 0, nullptr, nps, dst, \
+        p->n_samples, write_final)
+    switch (p->group_pack) {
+    case 2: FM_LP(2); break;
+    case 4: FM_LP(4); break;
+    default: FM_LP(8); break;
+    }
+#undef FM_LP
+}
+
 static size_t direct_workspace(const fm_signal_plan *p, int64_t N, int V)
 {
     if (N <= 0 || p->n_samples == 0) return 0;
     // the largest of the tilings' needs (motion forces the grouped one)
     size_t need = 0;
-    for (int g = 0; g < 3; ++g) {
+    for (int g = 0; g < 4; ++g) {
         if (g == 2 && p->n_runs == 0) continue;
+        if (g == 3 && p->group_pack <= 1) continue;
         const DirectLaunch L = direct_launch_shape(p, N, g);
         if (L.n_splits > 1)
             need = std::max(need, (size_t)L.n_splits * (size_t)p->n_samples * (size_t)V * sizeof(double2));
@@ -652,8 +682,10 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
     // static nodes take the readout runs when the plan has them (the FP64
     // form only when they are linear to 1e-14)
     const bool use_runs = !motion && p->n_runs > 0 && (precision != FM_PREC_F64 || p->runs_exact);
+    // static nodes with small equal-size t-groups: packed slots
+    const bool use_packed = !motion && !use_runs && p->group_pack > 1;
     const bool grouped = motion || p->prefer_grouped;
-    const int tiling = use_runs ? 2 : (grouped ? 1 : 0);
+    const int tiling = use_runs ? 2 : (use_packed ? 3 : (grouped ? 1 : 0));
     const DirectLaunch L = direct_launch_shape(p, N, tiling);
     const size_t need = L.n_splits > 1 ? (size_t)L.n_splits * (size_t)S * (size_t)V * sizeof(double2) : 0;
     if (need > ws_bytes) {
@@ -679,7 +711,18 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
             FM_LAUNCH_CHECK("k_signal_runs");
         }
     }
-    for (int v0 = 0; v0 < V && !use_runs; v0 += kMaxVB) {
+    if (use_packed) {
+        // one encoding per launch: the per-(slot, group) coefficients of a
+        // staged node take 2 * group_pack complex values per encoding
+        for (int v0 = 0; v0 < V; ++v0) {
+            if (precision == FM_PREC_F64)
+                launch_packed<false>(p, grid, packed, N, V, v0, L.nodes_per_split, dst, write_final, st);
+            else
+                launch_packed<true>(p, grid, packed, N, V, v0, L.nodes_per_split, dst, write_final, st);
+            FM_LAUNCH_CHECK("k_signal_direct (packed t-groups)");
+        }
+    }
+    for (int v0 = 0; v0 < V && !use_runs && !use_packed; v0 += kMaxVB) {
         const int vb = (V - v0 < kMaxVB) ? V - v0 : kMaxVB;
         if (precision == FM_PREC_F64)
             launch_direct<false>(p, grouped, motion, grid, packed, N, V, v0, vb, modes, R, mcoef, L.nodes_per_split, dst, write_final, st);
@@ -708,7 +751,7 @@ namespace fm {
 // Host tables of a plan before upload.
 struct PlanHost {
     std::vector<int32_t> perm;
-    std::vector<SignalTile> tiles_g, tiles_c;
+    std::vector<SignalTile> tiles_g, tiles_c, tiles_p;
     std::vector<double4> ks;
     std::vector<RunDesc> runs;
 };
@@ -772,6 +815,22 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
             }
         }
         while (tiles_g.size() % kDirectSPL) tiles_g.push_back({0, 0, 0, 0});
+        // packed slots: every t-group has the same size g in {32, 64, 128}
+        // (e.g. the interleaves of a spiral at one readout index), so a slot
+        // takes 256 / g whole groups
+        const int64_t g0 = p->n_tgroups > 0 ? count[1] - count[0] : 0;
+        bool uniform = g0 == 32 || g0 == 64 || g0 == 128;
+        for (int64_t g = 0; g < p->n_tgroups && uniform; ++g) uniform = count[(size_t)g + 1] - count[(size_t)g] == g0;
+        if (uniform) {
+            const int64_t gp = kDirectThreads / g0;
+            for (int64_t g = 0; g < p->n_tgroups; g += gp) {
+                const int64_t ng = std::min<int64_t>(gp, p->n_tgroups - g);
+                h.tiles_p.push_back({(int32_t)g, (int32_t)count[(size_t)g], (int32_t)(ng * g0), 0});
+            }
+            while (h.tiles_p.size() % kDirectSPL) h.tiles_p.push_back({0, 0, 0, 0});
+            p->group_pack = (int)gp;
+            p->n_tiles_packed = (int64_t)h.tiles_p.size() / kDirectSPL;
+        }
         for (int64_t o = 0; o < S; o += kDirectTile)
             tiles_c.push_back({-1, (int32_t)o, (int32_t)std::min<int64_t>(kDirectTile, S - o), 0});
     }
@@ -901,6 +960,10 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         if ((e = hipMemcpy(p->d_perm, perm.data(), sizeof(int32_t) * (size_t)S, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if ((e = hipMemcpy(p->d_tiles_grouped, tiles_g.data(), sizeof(SignalTile) * tiles_g.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if ((e = hipMemcpy(p->d_tiles_consec, tiles_c.data(), sizeof(SignalTile) * tiles_c.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        if (!h.tiles_p.empty()) {
+            if ((e = hipMalloc(&p->d_tiles_packed, sizeof(SignalTile) * h.tiles_p.size())) != hipSuccess) return fail(e, "hipMalloc");
+            if ((e = hipMemcpy(p->d_tiles_packed, h.tiles_p.data(), sizeof(SignalTile) * h.tiles_p.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        }
         if ((e = hipMemcpy(p->d_tg_t, p->tg_t.data(), sizeof(double) * (size_t)p->n_tgroups, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if (p->n_runs > 0) {
             if ((e = hipMalloc(&p->d_runs, sizeof(RunDesc) * runs.size())) != hipSuccess) return fail(e, "hipMalloc");
@@ -921,7 +984,7 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
 extern "C" int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n_kz, const double *kx,
                                       const double *ky, const double *kz, const double *t, int method,
                                       int *method_out, int64_t *n_tgroups, int64_t *n_runs,
-                                      int64_t *n_tiles_grouped, int *runs_exact)
+                                      int64_t *n_tiles_grouped, int *runs_exact, int *group_pack)
 {
     FM_CHECK_ARG(n_meas >= 0 && n_lines >= 0 && n_kz >= 0, "negative k-space dimension");
     FM_CHECK_ARG(method == FM_METHOD_AUTO || method == FM_METHOD_DIRECT || method == FM_METHOD_CARTESIAN,
@@ -942,6 +1005,7 @@ extern "C" int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n
     if (n_runs) *n_runs = p.n_runs;
     if (n_tiles_grouped) *n_tiles_grouped = p.n_tiles_grouped;
     if (runs_exact) *runs_exact = p.runs_exact ? 1 : 0;
+    if (group_pack) *group_pack = p.group_pack;
     return FM_OK;
 }
 
@@ -991,6 +1055,7 @@ extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
     (void)hipFree(p->d_tiles_grouped);
     (void)hipFree(p->d_runs);
     (void)hipFree(p->d_tiles_consec);
+    (void)hipFree(p->d_tiles_packed);
     (void)hipFree(p->d_tg_t);
     cartesian_free(p);
     delete p;

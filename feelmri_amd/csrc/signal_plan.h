@@ -63,11 +63,15 @@ struct fm_signal_plan {
     // Two tilings of the samples: t-grouped tiles (each tile one distinct t;
     // always used for moving nodes) and consecutive tiles (any t per sample).
     int64_t n_tiles_grouped = 0, n_tiles_consec = 0;
+    // packed slots of group_pack equal-size t-groups (1: none), static nodes
+    int group_pack = 1;
+    int64_t n_tiles_packed = 0;
     // device tables
     double4 *d_ks = nullptr;      // per sample: kx', ky', kz' (cycles/m, pi_f-scaled), t
     int32_t *d_perm = nullptr;    // samples ordered by t-group (grouped tiles index it)
     fm::SignalTile *d_tiles_grouped = nullptr;
     fm::SignalTile *d_tiles_consec = nullptr;  // offsets are sample indices
+    fm::SignalTile *d_tiles_packed = nullptr;
     double *d_tg_t = nullptr;
     // readout runs (empty when some run is not linear in k and t)
     int64_t n_runs = 0, run_stride = 0;

@@ -135,6 +135,17 @@ def test_spiral_small_groups_general_kernel(gpu):
     _check(kloc, t, n=4000, seed=4)
 
 
+@pytest.mark.parametrize("n_il,v", [(32, 1), (64, 3), (128, 2)])
+def test_spiral_packed_t_groups(gpu, n_il, v):
+    """Interleaves sharing t per readout index: 256 / n_il equal-size groups
+    per packed slot; 90 readout indices leave a partial last slot."""
+    from feelmri_amd.device import analyse_trajectory
+
+    kloc, t = trajectories.spiral(0.24, 64, 90, n_il, t_first=0.2)
+    assert analyse_trajectory(kloc, t)["group_pack"] == 256 // n_il
+    _check(kloc, t, n=3001, v=v, seed=n_il)
+
+
 def test_random_times_per_sample(gpu):
     rng = np.random.default_rng(5)
     kloc = tuple(rng.uniform(-200, 200, (30, 7, 2)) for _ in range(3))

@@ -85,3 +85,20 @@ def test_runs_exact_for_fp64():
     # ~1e-11 breaks the runs altogether
     kx[5, 2, 0] += 1e-11 * kmax
     assert analyse_trajectory((kx, kloc[1], kloc[2]), t)["n_runs"] == 0
+
+
+def test_packed_t_groups():
+    """Equal-size t-groups of 32, 64 or 128 samples pack 8, 4 or 2 to a slot
+    (spiral interleaves share t per readout index); other sizes do not."""
+    for n_il, gp in ((32, 8), (64, 4), (128, 2)):
+        kloc, t = trajectories.spiral(0.24, 64, 40, n_il, t_first=0.2)
+        a = analyse_trajectory(kloc, t)
+        assert a["n_runs"] == 0 and a["n_tgroups"] == 40 and a["group_pack"] == gp, (n_il, a)
+    for n_il in (16, 48, 256):
+        kloc, t = trajectories.spiral(0.24, 64, 40, n_il, t_first=0.2)
+        assert analyse_trajectory(kloc, t)["group_pack"] == 1
+    # one group of a different size breaks the packing
+    kloc, t = trajectories.spiral(0.24, 64, 40, 32, t_first=0.2)
+    t = np.asarray(t).copy()
+    t[3, 0, 0] = 99.0
+    assert analyse_trajectory(kloc, t)["group_pack"] == 1
