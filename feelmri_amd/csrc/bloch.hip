@@ -33,19 +33,33 @@ __device__ inline void dev_sincos<double>(double x, double *s, double *c) { sinc
 // (a few 1e-7), which is large relative to the small per-event rotations of
 // an RF pulse (e.g. 0.013 rad per 10 us step of a 15 deg, 0.2 ms pulse) and
 // accumulates over the pulse; below 0.25 rad the Taylor series to x^7 / x^6
-// (truncation < 1e-10) is used instead.
+// (truncation < 1e-10) is used instead.  The choice is per lane; the wave
+// evaluates only the form(s) its lanes need (series only, hardware only, or
+// both and a select).
 template <>
 __device__ inline void dev_sincos<float>(float x, float *s, float *c)
 {
     float r = x * 0.159154943091895336f;
     r -= __builtin_rintf(r);
-    const float hs = __builtin_amdgcn_sinf(r), hc = __builtin_amdgcn_cosf(r);
-    const float a = r * 6.28318530717958648f, a2 = a * a;
-    const float ps = a * (1.0f - a2 * (1.0f / 6.0f) * (1.0f - a2 * (1.0f / 20.0f) * (1.0f - a2 * (1.0f / 42.0f))));
-    const float pc = 1.0f - a2 * 0.5f * (1.0f - a2 * (1.0f / 12.0f) * (1.0f - a2 * (1.0f / 30.0f)));
+    const float a = r * 6.28318530717958648f;
     const bool small = __builtin_fabsf(a) < 0.25f;
-    *s = small ? ps : hs;
-    *c = small ? pc : hc;
+    auto series = [&](float &ps, float &pc) {
+        const float a2 = a * a;
+        ps = a * (1.0f - a2 * (1.0f / 6.0f) * (1.0f - a2 * (1.0f / 20.0f) * (1.0f - a2 * (1.0f / 42.0f))));
+        pc = 1.0f - a2 * 0.5f * (1.0f - a2 * (1.0f / 12.0f) * (1.0f - a2 * (1.0f / 30.0f)));
+    };
+    if (__all(small)) {
+        series(*s, *c);
+    } else if (!__any(small)) {
+        *s = __builtin_amdgcn_sinf(r);
+        *c = __builtin_amdgcn_cosf(r);
+    } else {
+        float ps, pc;
+        series(ps, pc);
+        const float hs = __builtin_amdgcn_sinf(r), hc = __builtin_amdgcn_cosf(r);
+        *s = small ? ps : hs;
+        *c = small ? pc : hc;
+    }
 }
 
 template <typename T>
