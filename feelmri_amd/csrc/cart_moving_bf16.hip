@@ -26,7 +26,8 @@
 //   * exp(-2 pi i H_k . x) of the step's nodes, one per kz of the column tile,
 //     goes to LDS one step ahead; the B fragments (column 32 cb + l%32) are
 //     formed from it and the records, split into bf16 hi + lo planes in LDS
-//     (double-buffered, one barrier per step);
+//     (one buffer: a barrier after the build and one after the MFMA phase's
+//     reads; the smaller footprint lets 4 workgroups share a CU);
 //   * 12 v_mfma_f32_32x32x16_bf16 per (row block, column block) and step:
 //     Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br, each real product as hh + hl +
 //     lh, with -Ai formed once per step.
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
     // encodings / kz in one instruction, and unpadded strides (a multiple of
     // 128 B) put them on the same LDS banks
     __shared__ float2 sc[kMovMaxV][kChunk + 2];
-    __shared__ uint4 sB[2][CB * 4 * 64];
+    __shared__ uint4 sB[CB * 4 * 64];
     __shared__ double sH[3 * (kCols + 1)];
     __shared__ double scoef[kMaxModes + 3];
     extern __shared__ float2 sEz[];  // 2 buffers of ez_rows rows of kEzStride (kz row, node)
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
         for (int q = 0; q < 16; ++q) acc_re[cb][q] = acc_im[cb][q] = 0.0f;
 
     const int ez_buf = ez_rows * kEzStride;
-    int pz = 0, pb = 0;
+    int pz = 0;
     const int half = lane / 32;
 
     // exp(-2 pi i H_k . x) of step s's 16 nodes into buffer `buf`
@@ -252,7 +253,7 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
                         }
                         bf16x8 rh, rl, ih, il;
                         split8(br_, bi_, rh, rl, ih, il);
-                        uint4 *dst = &sB[pb][(f / 64) * 4 * 64 + lane];
+                        uint4 *dst = &sB[(f / 64) * 4 * 64 + lane];
                         dst[0 * 64] = __builtin_bit_cast(uint4, rh);
                         dst[1 * 64] = __builtin_bit_cast(uint4, rl);
                         dst[2 * 64] = __builtin_bit_cast(uint4, ih);
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
             __syncthreads();
 #pragma unroll
             for (int cb = 0; cb < CB; ++cb) {
-                const uint4 *src = &sB[pb][cb * 4 * 64 + lane];
+                const uint4 *src = &sB[cb * 4 * 64 + lane];
                 const bf16x8 brh = __builtin_bit_cast(bf16x8, src[0 * 64]);
                 const bf16x8 brl = __builtin_bit_cast(bf16x8, src[1 * 64]);
                 const bf16x8 bih = __builtin_bit_cast(bf16x8, src[2 * 64]);
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
                 acc_im[cb] = im;
             }
             pz ^= 1;
-            pb ^= 1;
+            __syncthreads();  // B planes read before the next step rewrites them
         }
     }
 
