@@ -348,7 +348,8 @@ def run_flow(args, dev):
 
     def time_it(form, steps, warmup):
         outs = [torch.empty((plan.n_samples, 4), dtype=torch.complex128, device=dev) for _ in phases]
-        ws = torch.empty(max(plan.workspace_bytes(form, m.n_nodes, 4), 1), dtype=torch.uint8, device=dev)
+        ws = torch.empty(max(plan.workspace_bytes(form, m.n_nodes, 4, moving=True), 1), dtype=torch.uint8,
+                         device=dev)
 
         def step():
             for (nodes, coef, _), o in zip(phases, outs):

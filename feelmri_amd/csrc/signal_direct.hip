@@ -1083,19 +1083,19 @@ extern "C" int fm_signal_plan_runs(const fm_signal_plan *p, int64_t *n_runs, int
 }
 
 extern "C" int fm_signal_workspace_size(const fm_signal_plan *p, int precision, int64_t n_nodes,
-                                        int n_enc, size_t *bytes)
+                                        int n_enc, int moving, size_t *bytes)
 {
     FM_CHECK_ARG(p != nullptr && bytes != nullptr, "NULL argument");
     FM_CHECK_ARG(n_nodes >= 0 && n_enc >= 1, "bad sizes");
-    // A Cartesian plan runs static nodes on its GEMM and moving nodes on the
-    // moving GEMM (mixed) or the direct kernels (FP64): enough for each.
-    if (p->method == FM_METHOD_CARTESIAN) {
-        size_t b = std::max(cartesian_workspace(p, precision, n_nodes, n_enc), direct_workspace(p, n_nodes, n_enc));
-        if (precision != FM_PREC_F64) b = std::max(b, cartesian_moving_workspace(p, n_nodes, n_enc));
-        *bytes = b;
-    } else {
+    // the path fm_signal_execute takes: a Cartesian plan runs static nodes
+    // on its GEMM and moving nodes on the moving GEMM (mixed, up to
+    // kMovMaxV encodings) or the direct kernels
+    if (p->method == FM_METHOD_CARTESIAN && !moving)
+        *bytes = cartesian_workspace(p, precision, n_nodes, n_enc);
+    else if (p->method == FM_METHOD_CARTESIAN && precision == FM_PREC_MIXED && n_enc <= kMovMaxV)
+        *bytes = cartesian_moving_workspace(p, n_nodes, n_enc);
+    else
         *bytes = direct_workspace(p, n_nodes, n_enc);
-    }
     return FM_OK;
 }
 

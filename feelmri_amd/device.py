@@ -137,10 +137,11 @@ class SignalPlan:
         _lib.call("fm_signal_plan_kernel_ms", self._h, ctypes.byref(ms), ctypes.byref(n))
         return ms.value, n.value
 
-    def workspace_bytes(self, precision, n_nodes: int, n_enc: int) -> int:
+    def workspace_bytes(self, precision, n_nodes: int, n_enc: int, moving: bool = False) -> int:
+        """Workspace bytes ``execute`` needs, for static or moving nodes."""
         out = ctypes.c_size_t()
         _lib.call("fm_signal_workspace_size", self._h, _prec(precision), int(n_nodes), int(n_enc),
-                  ctypes.byref(out))
+                  int(bool(moving)), ctypes.byref(out))
         return int(out.value)
 
     def execute(self, nodes: "SignalNodes", precision="f64", motion=None, out=None,
@@ -156,9 +157,6 @@ class SignalPlan:
             out = torch.empty((S, V), dtype=torch.complex128, device=self.device)
         elif out.dtype != torch.complex128 or tuple(out.shape) != (S, V) or not out.is_contiguous():
             raise ValueError("out must be a contiguous complex128 tensor of shape (n_samples, n_enc)")
-        need = self.workspace_bytes(prec, N, V)
-        if workspace is None or workspace.numel() < need:
-            workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=self.device)
         modes = coef = None
         R = 0
         if motion is not None:
@@ -168,6 +166,9 @@ class SignalPlan:
                 raise ValueError("motion modes must be float32 of shape (3R, n_nodes)")
             if coef.dtype != torch.float64 or tuple(coef.shape) != (self.n_tgroups, R + 3):
                 raise ValueError("motion coefficients must be float64 of shape (n_tgroups, R+3)")
+        need = self.workspace_bytes(prec, N, V, moving=R > 0)
+        if workspace is None or workspace.numel() < need:
+            workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=self.device)
         with torch.cuda.device(self.device):
             _lib.call("fm_signal_execute", self._h, prec, N, V, ptr(nodes.packed), ptr(modes), R,
                       ptr(coef), ptr(workspace), workspace.numel(), ptr(out), stream_handle(stream))

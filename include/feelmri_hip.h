@@ -98,9 +98,10 @@ int fm_signal_plan_info(const fm_signal_plan *plan, int64_t *n_samples, int64_t 
  * FP64 run kernel too).  No counterpart in the reference (plan
  * introspection). */
 int fm_signal_plan_runs(const fm_signal_plan *plan, int64_t *n_runs, int *run_len, int *runs_exact);
-/* Workspace (device bytes) fm_signal_execute needs for this plan. */
+/* Workspace (device bytes) fm_signal_execute needs for this plan, with
+ * static nodes (moving = 0) or moving nodes (moving = 1: modes given). */
 int fm_signal_workspace_size(const fm_signal_plan *plan, int precision, int64_t n_nodes,
-                             int n_enc, size_t *bytes);
+                             int n_enc, int moving, size_t *bytes);
 
 /* Pack reference-layout node inputs into the SoA table fm_signal_execute
  * reads: rows x, y, z, phi_dB0 (rad/ms), 1/T2, then Re/Im of (w .* Mxy)[:, v]

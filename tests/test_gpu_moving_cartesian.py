@@ -122,3 +122,31 @@ def test_lines_out_of_order(gpu):
     assert _plan_method(kloc, t) == "cartesian"
     n = 1500
     _run(kloc, t, n, 2, _pod(n, seed=8))
+
+
+def test_workspace_per_path(gpu):
+    """The workspace query follows the path execute takes: static nodes on a
+    Cartesian plan do not pay for the moving GEMM's partial tiles, and each
+    path runs with exactly the bytes reported."""
+    import torch
+
+    from feelmri_amd import Motion, device
+
+    n = 40_000
+    kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (6, 64, 8), t_first=0.4)
+    plan = device.SignalPlan(kloc, t, method="auto")
+    assert plan.method == "cartesian"
+    r0, phi, T2, mxy, w = _nodes(n, 2, 21)
+    nodes = device.SignalNodes.from_arrays(r0, phi, T2, mxy, weights=w)
+    m = _pod(n, seed=21)
+    modes = device.modes_soa(torch.from_numpy(Motion.motion_modes(m)).cuda())
+    coef = torch.from_numpy(Motion.motion_coefficients(m, plan.tgroup_t)).cuda()
+    ws_static = plan.workspace_bytes("mixed", n, 2)
+    ws_moving = plan.workspace_bytes("mixed", n, 2, moving=True)
+    assert ws_static != ws_moving
+    for prec in ("mixed", "f64"):
+        for motion in (None, (modes, coef)):
+            need = plan.workspace_bytes(prec, n, 2, moving=motion is not None)
+            ws = torch.empty(max(need, 1), dtype=torch.uint8, device="cuda")
+            out = plan.execute(nodes, precision=prec, motion=motion, workspace=ws)
+            assert torch.isfinite(torch.view_as_real(out)).all()
