@@ -537,6 +537,7 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests in the serving stream reported beside the throughput (0: none)")
     ap.add_argument("--rate", type=float, default=20.0, help="serving stream arrival rate (requests/s)")
+    ap.add_argument("--serve-slots", type=int, default=8, help="concurrent requests (stream slots) in serving")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device-to-device copy of this many MiB first (a known-size kernel for "
                          "scaling hardware traffic counters)")
@@ -564,7 +565,7 @@ def main() -> int:
         log(f"copy calibration: {args.copy_calibration_mib} MiB read + written")
         del src, dst
     if args.config == "c4":
-        srv = serving_run(dev, max(1, args.serve_requests), args.rate)
+        srv = serving_run(dev, max(1, args.serve_requests), args.rate, args.serve_slots)
         log(f"serving: p50 {srv['p50_latency_s']:.3f} s, p99 {srv['p99_latency_s']:.3f} s, "
             f"{srv['node_ksamples_per_s']:.3e} node*k/s")
         print(json.dumps({
@@ -672,7 +673,7 @@ def main() -> int:
                                   for k, v in rows.items()))
     srv = None
     if args.serve_requests > 0:
-        srv = serving_run(dev, args.serve_requests, args.rate)
+        srv = serving_run(dev, args.serve_requests, args.rate, args.serve_slots)
         log(f"serving: p50 {srv['p50_latency_s']:.3f} s, p99 {srv['p99_latency_s']:.3f} s, "
             f"{srv['node_ksamples_per_s']:.3e} node*k/s")
 
