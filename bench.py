@@ -305,6 +305,22 @@ def serving_run(dev, n_requests: int, rate: float, max_active: int = 8):
     return out
 
 
+def _dtype(method: str, form: str) -> str:
+    """The type the dominant kernel computes in: bf16 MFMA inputs on the
+    Cartesian GEMMs, FP32 on the direct kernels, FP64 in the FP64 form."""
+    if form == "f64":
+        return "f64"
+    return "bf16" if method == "cartesian" else "f32"
+
+
+def _arithmetic(method: str, form: str) -> str:
+    if form == "f64":
+        return "FP64 throughout (FP64 MFMA on Cartesian plans)"
+    if method == "cartesian":
+        return "bf16 hi+lo operands, 3 products per real product, FP32 accumulation, FP64 split reduction"
+    return "FP64 phase and range reduction, FP32 sine/cosine and accumulation, FP64 chunk sums"
+
+
 def run_flow(args, dev):
     """c3: 4-D flow.  Set-up on the GPU: lumped weights, the whole
     phase-contrast acquisition through the Bloch integrator with nodes moving
@@ -414,10 +430,10 @@ def run_flow(args, dev):
         "metric": "node*k-sample/s at relL2<=1e-4 vs FP64 oracle", "value": value, "unit": "node*k-sample/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": ("bf16x3->f32 (f64 phase)" if plan.method == "cartesian" and args.form == "mixed"
-                  else {"mixed": "f32 (f64 phase)", "f64": "f64"}[args.form]), "data": "synthetic",
+        "dtype": _dtype(plan.method, args.form), "data": "synthetic",
         "config": {"workload": wl.info["desc"], "config": "c3", "nodes": m.n_nodes, "k_samples": wl.n_samples,
-                   "encodings": 4, "phases": n_ph, "method": plan.method, "form": args.form},
+                   "encodings": 4, "phases": n_ph, "method": plan.method, "form": args.form,
+                   "arithmetic": _arithmetic(plan.method, args.form)},
         "relL2_vs_f64_gpu": err64, "relL2_vs_oracle_sampled": err_or, "forms": forms,
         "roofline": roof, "cpu_baseline": None, "serving": None}), flush=True)
     return 0
@@ -688,11 +704,11 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"mixed": "bf16x3->f32 (cartesian) / f32 sincos + f64 phase (direct)", "f64": "f64"}[args.form]
-        if plan.method == "cartesian" else ({"mixed": "f32 (f64 phase)", "f64": "f64"}[args.form]),
+        "dtype": _dtype(plan.method, args.form),
         "data": "synthetic",
         "config": {"workload": wl.info["desc"], "config": wl.name, "nodes": wl.n_nodes,
-                   "k_samples": wl.n_samples, "encodings": wl.n_enc, "method": plan.method, "form": args.form},
+                   "k_samples": wl.n_samples, "encodings": wl.n_enc, "method": plan.method, "form": args.form,
+                   "arithmetic": _arithmetic(plan.method, args.form)},
         "relL2_vs_f64_gpu": err_f64,
         "image_relL2_vs_f64_gpu": err_img,
         "relL2_vs_oracle_sampled": err_oracle,
