@@ -420,6 +420,13 @@ def run_flow(args, dev):
                 "executed": 3.0 * achieved,
                 "per_unit": "8 flops per node*k-sample*encoding (complex MAC); operands generated in the "
                             "kernel (FP64 phase, FP32 sin/cos per node and line / kz, bf16 hi+lo split)"}
+    elif plan.method == "cartesian":
+        achieved = 8.0 * m.n_nodes * wl.n_samples * 4 / k_s / 1e12
+        roof = {"bound": "mfma", "kernel": "k_cart_moving_f64", "achieved": achieved,
+                "peak": PEAKS["fp64_mfma_tflops"], "unit": "TFLOP/s",
+                "frac": achieved / PEAKS["fp64_mfma_tflops"], "traffic": None, "kernel_ms": k_ms,
+                "per_unit": "8 flops per node*k-sample*encoding (complex MAC) on FP64 MFMA; operands "
+                            "generated in the kernel (FP64 sincospi per node and line / kz)"}
     else:
         peak = PEAKS["fp32_valu_tflops"] if args.form == "mixed" else PEAKS["fp64_valu_tflops"]
         achieved = (8.0 + 8.0 * 4) * m.n_nodes * wl.n_samples / k_s / 1e12

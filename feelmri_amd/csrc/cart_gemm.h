@@ -60,5 +60,9 @@ struct MovingShape {
 MovingShape moving_shape(const CartesianTables &c, int64_t N, int V);
 void launch_cart_moving_bf16(const MovingShape &s, const CartesianTables &c, const double *packed, int64_t N,
                              const float *modes, int R, const double *mcoef, float2 *part, hipStream_t st);
+// FP64 form (cart_moving_f64.hip): row and column blocks of 16, double2 partials.
+MovingShape moving_shape_f64(const CartesianTables &c, int64_t N, int V);
+void launch_cart_moving_f64(const MovingShape &s, const CartesianTables &c, const double *packed, int64_t N,
+                            const float *modes, int R, const double *mcoef, double2 *part, hipStream_t st);
 
 }  // namespace fm

@@ -1088,12 +1088,12 @@ extern "C" int fm_signal_workspace_size(const fm_signal_plan *p, int precision, 
     FM_CHECK_ARG(p != nullptr && bytes != nullptr, "NULL argument");
     FM_CHECK_ARG(n_nodes >= 0 && n_enc >= 1, "bad sizes");
     // the path fm_signal_execute takes: a Cartesian plan runs static nodes
-    // on its GEMM and moving nodes on the moving GEMM (mixed, up to
-    // kMovMaxV encodings) or the direct kernels
+    // on its GEMM and moving nodes on the per-readout moving GEMM (either
+    // form, up to kMovMaxV encodings) or the direct kernels
     if (p->method == FM_METHOD_CARTESIAN && !moving)
         *bytes = cartesian_workspace(p, precision, n_nodes, n_enc);
-    else if (p->method == FM_METHOD_CARTESIAN && precision == FM_PREC_MIXED && n_enc <= kMovMaxV)
-        *bytes = cartesian_moving_workspace(p, n_nodes, n_enc);
+    else if (p->method == FM_METHOD_CARTESIAN && n_enc <= kMovMaxV)
+        *bytes = cartesian_moving_workspace(p, precision, n_nodes, n_enc);
     else
         *bytes = direct_workspace(p, n_nodes, n_enc);
     return FM_OK;
@@ -1139,11 +1139,11 @@ extern "C" int fm_signal_execute(const fm_signal_plan *p, int precision, int64_t
     const bool motion = modes != nullptr && n_modes > 0;
     if (p->method == FM_METHOD_CARTESIAN && !motion)
         return cartesian_execute(p, precision, n_nodes, n_enc, packed, workspace, workspace_bytes, out, st);
-    // moving nodes on a Cartesian plan: the mixed form takes the per-readout
-    // GEMM (up to kMovMaxV encodings), the FP64 form the direct kernels
-    if (p->method == FM_METHOD_CARTESIAN && motion && precision == FM_PREC_MIXED && n_enc <= kMovMaxV)
-        return cartesian_moving_execute(p, n_nodes, n_enc, packed, modes, n_modes, motion_coef, workspace,
-                                        workspace_bytes, out, st);
+    // moving nodes on a Cartesian plan: the per-readout GEMM in either form
+    // (up to kMovMaxV encodings), otherwise the direct kernels
+    if (p->method == FM_METHOD_CARTESIAN && motion && n_enc <= kMovMaxV)
+        return cartesian_moving_execute(p, precision, n_nodes, n_enc, packed, modes, n_modes, motion_coef,
+                                        workspace, workspace_bytes, out, st);
     return direct_execute(p, precision, n_nodes, n_enc, packed, modes, n_modes, motion_coef, workspace,
                           workspace_bytes, out, st);
 }

@@ -454,14 +454,15 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t N, int V, 
     return FM_OK;
 }
 
-size_t cartesian_moving_workspace(const fm_signal_plan *p, int64_t N, int V)
+size_t cartesian_moving_workspace(const fm_signal_plan *p, int precision, int64_t N, int V)
 {
     if (p->n_samples == 0 || N == 0 || V > kMovMaxV) return 0;
-    return moving_shape(p->cart, N, V).part_bytes;
+    return (precision == FM_PREC_F64 ? moving_shape_f64(p->cart, N, V) : moving_shape(p->cart, N, V)).part_bytes;
 }
 
-int cartesian_moving_execute(const fm_signal_plan *p, int64_t N, int V, const double *packed, const float *modes,
-                             int R, const double *mcoef, void *ws, size_t ws_bytes, double *out, hipStream_t st)
+int cartesian_moving_execute(const fm_signal_plan *p, int precision, int64_t N, int V, const double *packed,
+                             const float *modes, int R, const double *mcoef, void *ws, size_t ws_bytes, double *out,
+                             hipStream_t st)
 {
     const CartesianTables &c = p->cart;
     const int64_t S = p->n_samples;
@@ -473,7 +474,8 @@ int cartesian_moving_execute(const fm_signal_plan *p, int64_t N, int V, const do
         set_error("moving Cartesian GEMM takes at most %d encodings per call (got %d)", kMovMaxV, V);
         return FM_ERR_UNSUPPORTED;
     }
-    const MovingShape s = moving_shape(c, N, V);
+    const bool f64 = precision == FM_PREC_F64;
+    const MovingShape s = f64 ? moving_shape_f64(c, N, V) : moving_shape(c, N, V);
     if (s.part_bytes > ws_bytes) {
         set_error("signal workspace too small: need %zu bytes, got %zu", s.part_bytes, ws_bytes);
         return FM_ERR_WORKSPACE;
@@ -483,14 +485,23 @@ int cartesian_moving_execute(const fm_signal_plan *p, int64_t N, int V, const do
         set_error("moving Cartesian GEMM exceeds the launch grid (%lld workgroups)", (long long)grid);
         return FM_ERR_UNSUPPORTED;
     }
-    float2 *part = reinterpret_cast<float2 *>(ws);
-    plan_timing_start(p, st);
-    launch_cart_moving_bf16(s, c, packed, N, modes, R, mcoef, part, st);
-    plan_timing_stop(p, st);
-    FM_LAUNCH_CHECK("k_cart_moving_bf16");
     const int64_t MNC = s.M * s.L * s.KV;
-    k_cart_reduce<float2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(part, s.n_splits, MNC,
-                                                                     reinterpret_cast<double2 *>(out));
+    plan_timing_start(p, st);
+    if (f64) {
+        double2 *part = reinterpret_cast<double2 *>(ws);
+        launch_cart_moving_f64(s, c, packed, N, modes, R, mcoef, part, st);
+        plan_timing_stop(p, st);
+        FM_LAUNCH_CHECK("k_cart_moving_f64");
+        k_cart_reduce<double2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(part, s.n_splits, MNC,
+                                                                          reinterpret_cast<double2 *>(out));
+    } else {
+        float2 *part = reinterpret_cast<float2 *>(ws);
+        launch_cart_moving_bf16(s, c, packed, N, modes, R, mcoef, part, st);
+        plan_timing_stop(p, st);
+        FM_LAUNCH_CHECK("k_cart_moving_bf16");
+        k_cart_reduce<float2><<<(unsigned)cdiv(MNC, 256), 256, 0, st>>>(part, s.n_splits, MNC,
+                                                                         reinterpret_cast<double2 *>(out));
+    }
     FM_LAUNCH_CHECK("k_cart_reduce");
     return FM_OK;
 }

@@ -107,9 +107,9 @@ size_t cartesian_workspace(const fm_signal_plan *p, int precision, int64_t n_nod
 int cartesian_execute(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc,
                       const double *packed, void *ws, size_t ws_bytes, double *out,
                       hipStream_t stream);
-// Moving nodes, mixed form (n_enc <= kMovMaxV).
-size_t cartesian_moving_workspace(const fm_signal_plan *p, int64_t n_nodes, int n_enc);
-int cartesian_moving_execute(const fm_signal_plan *p, int64_t n_nodes, int n_enc, const double *packed,
-                             const float *modes, int n_modes, const double *motion_coef, void *ws,
-                             size_t ws_bytes, double *out, hipStream_t stream);
+// Moving nodes, per-readout GEMM in either form (n_enc <= kMovMaxV).
+size_t cartesian_moving_workspace(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc);
+int cartesian_moving_execute(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc,
+                             const double *packed, const float *modes, int n_modes, const double *motion_coef,
+                             void *ws, size_t ws_bytes, double *out, hipStream_t stream);
 }  // namespace fm

@@ -115,9 +115,9 @@ int fm_signal_nodes_pack(int64_t n_nodes, int n_enc, int dtype, const void *r0,
                          const double *weights, double *packed, void *stream);
 
 /* Evaluate the signal.  out: (n_samples, n_enc) complex128, sample order
- * (i, j, k) of the plan's arrays.  On a Cartesian plan, moving nodes in the
- * mixed form with n_enc <= 4 take the per-readout GEMM, and in the FP64 form
- * the direct kernels.  Motion (MRI.cpp:192-204): modes (NULL
+ * (i, j, k) of the plan's arrays.  On a Cartesian plan, moving nodes with
+ * n_enc <= 4 take the per-readout GEMM (bf16 or FP64 MFMA by precision),
+ * otherwise the direct kernels.  Motion (MRI.cpp:192-204): modes (NULL
  * for static nodes) is (3 n_modes) rows of n_nodes float32, row c*n_modes+r
  * holding component c of mode r; motion_coef (device) is (n_tgroups,
  * n_modes + 3) doubles: per distinct t the mode weights followed by a

@@ -1,11 +1,11 @@
-"""Moving nodes on Cartesian stacks: the per-readout GEMM (mixed form) against
-the FP64 oracle and the direct kernels.
+"""Moving nodes on Cartesian stacks: the per-readout GEMMs (mixed form on bf16
+MFMA, FP64 form on FP64 MFMA) against the FP64 oracle and the direct kernels.
 
 With POD motion the positions change with t (MRI.cpp:192-217); on a Cartesian
 stack t depends on the readout index only, so each readout's samples share
 one set of positions and the sum over nodes is a (lines x nodes) by
-(nodes x kz*V) product that the mixed form runs on the matrix cores with
-in-kernel operands.  The FP64 form of the same plan runs the direct kernels.
+(nodes x kz*V) product that both forms run on the matrix cores with
+in-kernel operands.
 
 Shapes cover every workgroup configuration of the kernel (1-4 row blocks per
 workgroup, 1-4 column blocks, several row and column tiles, partial blocks),
@@ -44,12 +44,15 @@ def _run(kloc, t, n, v, motion, seed=3, oracle_motion=None, gpu=None):
     gemm = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, motion, precision="mixed")
     direct = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, motion, precision="mixed", method="direct")
     f64 = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, motion, precision="f64")
+    f64d = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, motion, precision="f64", method="direct")
     assert gemm.shape == ref.shape == (*np.asarray(t).shape, v)
-    errs = {"gemm": rel_l2(gemm, ref), "direct": rel_l2(direct, ref), "f64": rel_l2(f64, ref)}
+    errs = {"gemm": rel_l2(gemm, ref), "direct": rel_l2(direct, ref), "f64_gemm": rel_l2(f64, ref),
+            "f64_direct": rel_l2(f64d, ref)}
     print(f"n={n} v={v} shape={np.asarray(t).shape}:", {k: f"{e:.2e}" for k, e in errs.items()})
     assert errs["gemm"] <= TOL_MIXED, errs
     assert errs["direct"] <= TOL_MIXED, errs
-    assert errs["f64"] <= TOL_F64, errs
+    assert errs["f64_gemm"] <= TOL_F64, errs
+    assert errs["f64_direct"] <= TOL_F64, errs
     return errs
 
 
@@ -95,9 +98,9 @@ def test_readouts_sharing_t(gpu):
 
 
 def test_large_mesh_against_f64(gpu):
-    """300 000 nodes (three node splits) on the 4-D flow tile, mixed GEMM
-    against the FP64 form of the same plan (the direct kernels, which the
-    tests above pin to the oracle)."""
+    """300 000 nodes (many node splits) on the 4-D flow tile: mixed GEMM and
+    FP64 GEMM against the FP64 direct kernels, which the tests above pin to
+    the oracle."""
     from feelmri_amd import MRI
 
     n = 300_000
@@ -107,10 +110,12 @@ def test_large_mesh_against_f64(gpu):
     m.update_timeshift(1.0)
     M = sp.diags(w)
     got = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, m, precision="mixed")
-    ref = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, m, precision="f64")
-    err = rel_l2(got, ref)
-    print(f"300k nodes: mixed GEMM vs FP64 relL2 {err:.2e}")
+    g64 = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, m, precision="f64")
+    ref = MRI.Signal(0, M, kloc, t, r0, phi, T2, mxy, m, precision="f64", method="direct")
+    err, err64 = rel_l2(got, ref), rel_l2(g64, ref)
+    print(f"300k nodes: mixed GEMM vs FP64 relL2 {err:.2e}, FP64 GEMM vs FP64 direct {err64:.2e}")
     assert err <= TOL_MIXED
+    assert err64 <= TOL_F64
 
 
 def test_lines_out_of_order(gpu):
