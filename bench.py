@@ -615,7 +615,7 @@ def main() -> int:
     out_h = out.cpu().numpy()
 
     forms = {args.form: {"ms_per_step": ms_ev, "node_ksamples_per_s": value, "kernel_ms": ms_kernel}}
-    err_f64 = err_img = None
+    err_f64 = err_img = err_img_tukey = None
     if not args.no_f64 and args.form != "f64":
         ref_steps = 1 if plan.method == "direct" else max(1, args.steps)
         out64, ms64, _, ms64_kernel = time_form(plan, nodes, "f64", ref_steps, 1)
@@ -628,6 +628,8 @@ def main() -> int:
 
             shape = (*wl.t.shape, wl.n_enc)
             err_img = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape))
+            # CartesianRecon's Tukey-windowed image (Recon.py:13-90, before its resize)
+            err_img_tukey = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape), "Tukey")
         elif wl.name.startswith("c2"):
             from feelmri_amd.recon import adjoint_image_rel_l2
 
@@ -718,6 +720,7 @@ def main() -> int:
                    "arithmetic": _arithmetic(plan.method, args.form)},
         "relL2_vs_f64_gpu": err_f64,
         "image_relL2_vs_f64_gpu": err_img,
+        "image_relL2_tukey_vs_f64_gpu": err_img_tukey,
         "relL2_vs_oracle_sampled": err_oracle,
         "forms": forms,
         "roofline": roof,
