@@ -67,3 +67,25 @@ def test_colouring_is_a_valid_partition():
     for c in range(nc.value):
         ids = el[perm[offs[c]:offs[c + 1]]].ravel()
         assert len(ids) == len(set(ids.tolist())), "two elements of one colour share a node"
+
+
+def _declared_arity():
+    """name -> number of parameters, from the prototypes in include/*.h."""
+    out = {}
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"\b(fm_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text, flags=re.S):
+            params = m.group(2).strip()
+            out[m.group(1)] = 0 if params in ("", "void") else params.count(",") + 1
+    return out
+
+
+def test_ctypes_table_matches_the_prototypes():
+    """Every ctypes signature has as many arguments as its C prototype, so a
+    changed declaration cannot pass the CPU suite with a stale binding."""
+    from feelmri_amd import _lib
+
+    arity = _declared_arity()
+    assert set(arity) == set(_lib.exported_symbols())
+    bad = {n: (arity[n], len(a)) for n, a in _lib._SIGNATURES.items() if arity[n] != len(a)}
+    assert not bad, f"(prototype, ctypes) argument counts differ: {bad}"
