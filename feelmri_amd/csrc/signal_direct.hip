@@ -1048,6 +1048,15 @@ extern "C" int fm_signal_plan_kernel_ms(fm_signal_plan *p, double *total_ms, int
 extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
 {
     if (!p) return FM_OK;
+    // executes still in flight on any stream read the plan's tables: wait
+    // for the device before they are freed
+    {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (cur != p->device) (void)hipSetDevice(p->device);
+        (void)hipDeviceSynchronize();
+        if (cur != p->device) (void)hipSetDevice(cur);
+    }
     for (hipEvent_t e : p->ev_start) (void)hipEventDestroy(e);
     for (hipEvent_t e : p->ev_stop) (void)hipEventDestroy(e);
     (void)hipFree(p->d_ks);

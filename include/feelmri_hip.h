@@ -74,6 +74,8 @@ typedef struct fm_signal_plan fm_signal_plan;
 int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int64_t n_lines, int64_t n_kz,
                           const double *kx, const double *ky, const double *kz,
                           const double *t, int method);
+/* Waits for the plan's device (executes in flight read its tables), then
+ * frees it. */
 int fm_signal_plan_destroy(fm_signal_plan *plan);
 /* Host-only analysis of a trajectory, as fm_signal_plan_create performs it,
  * without a device: the method it would choose, the number of distinct t,
