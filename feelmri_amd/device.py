@@ -362,9 +362,12 @@ def element_matrices(elems: torch.Tensor, nodes: torch.Tensor, stream=None) -> t
     E = int(elems.shape[0])
     vals = torch.empty((E, 4, 4), dtype=torch.float64, device=nodes.device)
     dt_code = _lib.DTYPE_F64 if nodes.dtype == torch.float64 else _lib.DTYPE_F32
+    # converted inputs are bound to names until the launch: a temporary freed
+    # while the argument list is built could be handed to the next one
+    elems_c, nodes_c = elems.contiguous(), nodes.contiguous()
     with torch.cuda.device(nodes.device):
-        _lib.call("fm_mass_p1_element_matrices", E, ptr(elems.contiguous()), dt_code,
-                  ptr(nodes.contiguous()), ptr(vals), stream_handle(stream))
+        _lib.call("fm_mass_p1_element_matrices", E, ptr(elems_c), dt_code, ptr(nodes_c), ptr(vals),
+                  stream_handle(stream))
     return vals
 
 
@@ -375,9 +378,10 @@ def pod_tensordot(modes: torch.Tensor, weights: torch.Tensor, stream=None) -> to
     P, C, M = (int(s) for s in modes.shape)
     out = torch.empty((P, C), dtype=modes.dtype, device=modes.device)
     dt_code = _lib.DTYPE_F64 if modes.dtype == torch.float64 else _lib.DTYPE_F32
+    modes_c, weights_c = modes.contiguous(), weights.contiguous()
     with torch.cuda.device(modes.device):
-        _lib.call("fm_pod_tensordot", dt_code, P, C, M, ptr(modes.contiguous()),
-                  ptr(weights.contiguous()), ptr(out), stream_handle(stream))
+        _lib.call("fm_pod_tensordot", dt_code, P, C, M, ptr(modes_c), ptr(weights_c), ptr(out),
+                  stream_handle(stream))
     return out
 
 
@@ -387,9 +391,9 @@ def modes_soa(modes_pcr: torch.Tensor, stream=None) -> torch.Tensor:
     if C != 3:
         raise ValueError("modes must have 3 components")
     out = torch.empty((3 * R, P), dtype=torch.float32, device=modes_pcr.device)
+    src = modes_pcr.to(torch.float32).contiguous()
     with torch.cuda.device(modes_pcr.device):
-        _lib.call("fm_modes_pack", P, R, ptr(modes_pcr.to(torch.float32).contiguous()), ptr(out),
-                  stream_handle(stream))
+        _lib.call("fm_modes_pack", P, R, ptr(src), ptr(out), stream_handle(stream))
     return out
 
 
@@ -399,7 +403,8 @@ def pod_positions(xyz_soa: torch.Tensor, modes: torch.Tensor | None, coef: torch
     N = int(xyz_soa.shape[1])
     R = 0 if modes is None else int(modes.shape[0]) // 3
     out = torch.empty((N, 3), dtype=torch.float64, device=xyz_soa.device)
+    xyz_c, coef_c = xyz_soa.contiguous(), coef.contiguous()
     with torch.cuda.device(xyz_soa.device):
-        _lib.call("fm_pod_positions", N, ptr(xyz_soa.contiguous()), ptr(modes), R,
-                  ptr(coef.contiguous()), ptr(out), stream_handle(stream))
+        _lib.call("fm_pod_positions", N, ptr(xyz_c), ptr(modes), R, ptr(coef_c), ptr(out),
+                  stream_handle(stream))
     return out
