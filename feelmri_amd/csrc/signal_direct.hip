@@ -290,8 +290,8 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
 //   e0 = exp(-t0/T2) exp(-2 pi i (k0.x - phi t0 / 2 pi))      (FP64 phase,
 //        range-reduced, FP32 hardware sine/cosine)
 //   q  = exp(-dt/T2) exp(-2 pi i (dk.x - phi dt / 2 pi))       (FP32)
-// and accumulates e0 q^m c for m = 0 .. kRunLen-1, advancing e by one
-// complex multiply per sample.  FP32 partial sums over a 256-node chunk are
+// and accumulates e0 q^m c for m = 0 .. kRunLen-1, advancing e (one
+// encoding: y = e c) by one complex multiply per sample.  FP32 partial sums over a 256-node chunk are
 // added into FP64, as in k_signal_direct.
 template <int VB>
 struct alignas(16) RunNode {
@@ -380,17 +380,32 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
             const float ds = __builtin_amdgcn_exp2f(-dtf * nd.r2f * 1.44269504f);
             const float qr = ds * __builtin_amdgcn_cosf(du);
             const float qi = -ds * __builtin_amdgcn_sinf(du);
+            if constexpr (VB == 1) {
+                // y = e c advances by q: one complex multiply and one complex
+                // add per sample
+                float yr = fmaf(er, nd.c[0], -ei * nd.c[1]);
+                float yi = fmaf(er, nd.c[1], ei * nd.c[0]);
 #pragma unroll
-            for (int m = 0; m < kRunLen; ++m) {
-#pragma unroll
-                for (int v = 0; v < VB; ++v) {
-                    const float cr = nd.c[2 * v], ci = nd.c[2 * v + 1];
-                    acc_re[m][v] = fmaf(er, cr, fmaf(-ei, ci, acc_re[m][v]));
-                    acc_im[m][v] = fmaf(er, ci, fmaf(ei, cr, acc_im[m][v]));
+                for (int m = 0; m < kRunLen; ++m) {
+                    acc_re[m][0] += yr;
+                    acc_im[m][0] += yi;
+                    const float nr = fmaf(yr, qr, -yi * qi);
+                    yi = fmaf(yr, qi, yi * qr);
+                    yr = nr;
                 }
-                const float nr = er * qr - ei * qi;
-                ei = fmaf(er, qi, ei * qr);
-                er = nr;
+            } else {
+#pragma unroll
+                for (int m = 0; m < kRunLen; ++m) {
+#pragma unroll
+                    for (int v = 0; v < VB; ++v) {
+                        const float cr = nd.c[2 * v], ci = nd.c[2 * v + 1];
+                        acc_re[m][v] = fmaf(er, cr, fmaf(-ei, ci, acc_re[m][v]));
+                        acc_im[m][v] = fmaf(er, ci, fmaf(ei, cr, acc_im[m][v]));
+                    }
+                    const float nr = er * qr - ei * qi;
+                    ei = fmaf(er, qi, ei * qr);
+                    er = nr;
+                }
             }
         }
 #pragma unroll
@@ -487,17 +502,31 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs_f64(
             sincospi(2.0 * du, &qs, &qc);
             const double ds = exp(-rd.dt * nd.r2);
             const double qr = ds * qc, qi = -ds * qs;
+            if constexpr (VB == 1) {
+                // y = e c advances by q, as in k_signal_runs
+                double yr = fma(er, nd.c[0], -ei * nd.c[1]);
+                double yi = fma(er, nd.c[1], ei * nd.c[0]);
 #pragma unroll
-            for (int m = 0; m < kRunLen; ++m) {
-#pragma unroll
-                for (int v = 0; v < VB; ++v) {
-                    const double cr = nd.c[2 * v], ci = nd.c[2 * v + 1];
-                    tot_re[m][v] = fma(er, cr, fma(-ei, ci, tot_re[m][v]));
-                    tot_im[m][v] = fma(er, ci, fma(ei, cr, tot_im[m][v]));
+                for (int m = 0; m < kRunLen; ++m) {
+                    tot_re[m][0] += yr;
+                    tot_im[m][0] += yi;
+                    const double nr = fma(yr, qr, -yi * qi);
+                    yi = fma(yr, qi, yi * qr);
+                    yr = nr;
                 }
-                const double nr = er * qr - ei * qi;
-                ei = fma(er, qi, ei * qr);
-                er = nr;
+            } else {
+#pragma unroll
+                for (int m = 0; m < kRunLen; ++m) {
+#pragma unroll
+                    for (int v = 0; v < VB; ++v) {
+                        const double cr = nd.c[2 * v], ci = nd.c[2 * v + 1];
+                        tot_re[m][v] = fma(er, cr, fma(-ei, ci, tot_re[m][v]));
+                        tot_im[m][v] = fma(er, ci, fma(ei, cr, tot_im[m][v]));
+                    }
+                    const double nr = er * qr - ei * qi;
+                    ei = fma(er, qi, ei * qr);
+                    er = nr;
+                }
             }
         }
     }
