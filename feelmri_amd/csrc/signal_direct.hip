@@ -51,10 +51,12 @@ struct alignas(16) StagedNode {
     using CT = typename CoefT<MIXED>::T;
     static constexpr int kPos = MOTION ? kDirectSPL : 1;
     static constexpr int kCoef = SHARED_T ? kDirectSPL * GP : 1;
+    static constexpr bool kPair = MIXED && GP > 1;  // second slot's phase by FP32 offset
     double x[kPos], y[kPos], z[kPos];
     double dwc;                  // phi/(2 pi), cycles/ms   (per-sample-t kernel only)
     double r2;                   // 1/T2, 1/ms              (per-sample-t kernel only)
     CT c[kCoef][2 * VB];
+    float xf[kPair ? 4 : 1];     // packed mixed slots: x, y, z in FP32 and max |x|, |y|, |z|
 };
 
 // Tiles.  Shared-t ("grouped") tiles are kDirectSPL slot records
@@ -66,17 +68,26 @@ struct alignas(16) StagedNode {
 // group) instead of being formed per (node, sample).  Per-sample-t
 // ("consecutive") tiles are one record of up to 512 consecutive samples
 // (lane l takes offset + l + 256 j).
-template <int VB, bool MIXED, bool SHARED_T, bool MOTION, int GP = 1>
+//
+// In packed slots of the mixed form the two samples of a lane lie on one
+// line, GP readout indices apart.  When |k'1 - k'0|_1 max(|x|,|y|,|z|) <= 4
+// cycles for every node of a chunk (pair_dk is the plan's bound on the
+// first factor), the second phase is formed as the first one's FP64-reduced
+// phase plus (k'1 - k'0).x in FP32: an FP32 error of ~1e-6 cycles on top of
+// the FP64 range reduction.  PLANAR drops the kz.z term (every kz = 0).
+template <int VB, bool MIXED, bool SHARED_T, bool MOTION, int GP = 1, bool PLANAR = false>
 __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     const double *__restrict__ packed, int64_t N, int V, int v0,
     const double4 *__restrict__ ks, const int32_t *__restrict__ perm,
     const SignalTile *__restrict__ tiles, const double *__restrict__ tg_t,
     const float *__restrict__ modes, int R, const double *__restrict__ mcoef,
-    int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S, int write_final)
+    int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S, int write_final, double pair_dk)
 {
     using CT = typename CoefT<MIXED>::T;
     using Node = StagedNode<VB, MIXED, SHARED_T, MOTION, GP>;
     static_assert(GP == 1 || (SHARED_T && !MOTION), "packed t-groups: shared-t tiles of static nodes");
+    static_assert(!PLANAR || GP > 1, "planar phases: packed slots only");
+    constexpr bool kPair = Node::kPair && kDirectSPL == 2;
     constexpr int kSlots = SHARED_T ? kDirectSPL : 1;
     constexpr int kGSize = kDirectThreads / GP;  // samples per group of a packed slot
     __shared__ Node snode[kDirectThreads];
@@ -135,6 +146,8 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
         kzs[j] = k.z;
         ts[j] = k.w;
     }
+    const float dkx = (float)(kxs[kSlots - 1] - kxs[0]), dky = (float)(kys[kSlots - 1] - kys[0]),
+                dkz = (float)(kzs[kSlots - 1] - kzs[0]);
 
     double tot_re[kDirectSPL][VB], tot_im[kDirectSPL][VB];
 #pragma unroll
@@ -144,6 +157,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
 
     for (int64_t base = n_begin; base < n_end; base += kDirectThreads) {
         __syncthreads();  // previous chunk fully consumed (and scoef visible)
+        int far_node = 0;  // this thread's node too far out for the FP32 offset
         {
             const int64_t n = base + tid;
             // staged in place: packed slots carry 2 GP coefficient sets per
@@ -179,6 +193,13 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                 const double phi = pphi[n], r2 = pr2[n];
                 nd.dwc = phi * (1.0 / kTwoPi);
                 nd.r2 = r2;
+                if constexpr (kPair) {
+                    nd.xf[0] = (float)x0;
+                    nd.xf[1] = (float)y0;
+                    nd.xf[2] = PLANAR ? 0.0f : (float)z0;
+                    nd.xf[3] = (float)fmax(fabs(x0), fmax(fabs(y0), PLANAR ? 0.0 : fabs(z0)));
+                    far_node = fmax(fabs(x0), fmax(fabs(y0), PLANAR ? 0.0 : fabs(z0))) * pair_dk > 4.0;
+                }
                 for (int j = 0; j < Node::kCoef; ++j) {
                     if (SHARED_T) {
                         if (j > 0 && same_group) {
@@ -210,13 +231,16 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
 #pragma unroll
                 for (int j = 0; j < Node::kPos; ++j) nd.x[j] = nd.y[j] = nd.z[j] = 0.0;
                 nd.dwc = nd.r2 = 0.0;
+                if constexpr (kPair) nd.xf[0] = nd.xf[1] = nd.xf[2] = nd.xf[3] = 0.0f;
 #pragma unroll
                 for (int j = 0; j < Node::kCoef; ++j)
 #pragma unroll
                     for (int v = 0; v < 2 * VB; ++v) nd.c[j][v] = (CT)0;
             }
         }
-        __syncthreads();
+        bool pair_path = false;
+        if constexpr (kPair) pair_path = !__syncthreads_or(far_node);
+        else __syncthreads();
         const int cnt = (n_end - base < kDirectThreads) ? (int)(n_end - base) : kDirectThreads;
 
         CT acc_re[kDirectSPL][VB], acc_im[kDirectSPL][VB];
@@ -225,13 +249,41 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
 #pragma unroll
             for (int v = 0; v < VB; ++v) acc_re[j][v] = acc_im[j][v] = (CT)0;
 
+        if (pair_path) {
+            // first slot: FP64 phase, range-reduced; second: plus dk.x in FP32
+            if constexpr (kPair) {
+#pragma unroll 2
+            for (int m = 0; m < cnt; ++m) {
+                const Node &nd = snode[m];
+                const double u = PLANAR ? fma(kxs[0], nd.x[0], kys[0] * nd.y[0])
+                                        : fma(kxs[0], nd.x[0], fma(kys[0], nd.y[0], kzs[0] * nd.z[0]));
+                float r[2];
+                r[0] = (float)(u - __builtin_rint(u));
+                r[1] = r[0] + (PLANAR ? fmaf(dkx, nd.xf[0], dky * nd.xf[1])
+                                      : fmaf(dkx, nd.xf[0], fmaf(dky, nd.xf[1], dkz * nd.xf[2])));
+                r[1] -= __builtin_rintf(r[1]);
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const CT sn = __builtin_amdgcn_sinf(r[j]), cs = __builtin_amdgcn_cosf(r[j]);
+                    const int jc = j * GP + gq;
+#pragma unroll
+                    for (int v = 0; v < VB; ++v) {
+                        const CT cr = nd.c[jc][2 * v], ci = nd.c[jc][2 * v + 1];
+                        acc_re[j][v] = fma(cs, cr, fma(sn, ci, acc_re[j][v]));
+                        acc_im[j][v] = fma(cs, ci, fma(-sn, cr, acc_im[j][v]));
+                    }
+                }
+            }
+            }
+        } else {
 #pragma unroll 2
         for (int m = 0; m < cnt; ++m) {
             const Node &nd = snode[m];
 #pragma unroll
             for (int j = 0; j < kDirectSPL; ++j) {
                 const int jp = MOTION ? j : 0;
-                double u = fma(kxs[j], nd.x[jp], fma(kys[j], nd.y[jp], kzs[j] * nd.z[jp]));
+                double u = PLANAR ? fma(kxs[j], nd.x[jp], kys[j] * nd.y[jp])
+                                  : fma(kxs[j], nd.x[jp], fma(kys[j], nd.y[jp], kzs[j] * nd.z[jp]));
                 if (!SHARED_T) u = fma(-nd.dwc, ts[j], u);  // phase - phi t, in cycles
                 const double rr = u - __builtin_rint(u);
                 CT cs, sn;
@@ -265,6 +317,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                     acc_im[j][v] = fma(cs, ci, fma(-sn, cr, acc_im[j][v]));
                 }
             }
+        }
         }
 #pragma unroll
         for (int j = 0; j < kDirectSPL; ++j)
@@ -638,15 +691,15 @@ static void launch_direct_vb(const fm_signal_plan *p, bool grouped, bool motion,
         if (motion)
             k_signal_direct<VB, MIXED, true, true><<<grid, kDirectThreads, 0, st>>>(
                 packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles_grouped, p->d_tg_t, modes, R, mcoef,
-                nps, dst, p->n_samples, write_final);
+                nps, dst, p->n_samples, write_final, 0.0);
         else
             k_signal_direct<VB, MIXED, true, false><<<grid, kDirectThreads, 0, st>>>(
                 packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles_grouped, p->d_tg_t, modes, R, mcoef,
-                nps, dst, p->n_samples, write_final);
+                nps, dst, p->n_samples, write_final, 0.0);
     } else {
         k_signal_direct<VB, MIXED, false, false><<<grid, kDirectThreads, 0, st>>>(
             packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles_consec, p->d_tg_t, modes, R, mcoef, nps,
-            dst, p->n_samples, write_final);
+            dst, p->n_samples, write_final, 0.0);
     }
 }
 
@@ -670,14 +723,22 @@ template <bool MIXED>
 static void launch_packed(const fm_signal_plan *p, dim3 grid, const double *packed, int64_t N, int V, int v0,
                           int64_t nps, double2 *dst, int write_final, hipStream_t st)
 {
-#define FM_LP(GPV)                                                                                          \
-    k_signal_direct<1, MIXED, true, false, GPV><<<grid, kDirectThreads, 0, st>>>(                           \
+#define FM_LP(GPV, PL)                                                                                      \
+    k_signal_direct<1, MIXED, true, false, GPV, PL><<<grid, kDirectThreads, 0, st>>>(                      \
         packed, N, V, v0, p->d_ks, p->d_perm, p->d_tiles_packed, p->d_tg_t, nullptr, 0, nullptr, nps, dst, \
-        p->n_samples, write_final)
-    switch (p->group_pack) {
-    case 2: FM_LP(2); break;
-    case 4: FM_LP(4); break;
-    default: FM_LP(8); break;
+        p->n_samples, write_final, p->packed_dk)
+    if (p->planar) {
+        switch (p->group_pack) {
+        case 2: FM_LP(2, true); break;
+        case 4: FM_LP(4, true); break;
+        default: FM_LP(8, true); break;
+        }
+    } else {
+        switch (p->group_pack) {
+        case 2: FM_LP(2, false); break;
+        case 4: FM_LP(4, false); break;
+        default: FM_LP(8, false); break;
+        }
     }
 #undef FM_LP
 }
@@ -872,6 +933,22 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
     ks.resize((size_t)S);
     for (int64_t s = 0; s < S; ++s)
         ks[(size_t)s] = make_double4(kx[s] * kPiFOverPi, ky[s] * kPiFOverPi, kz[s] * kPiFOverPi, t[s]);
+    p->planar = 1;
+    for (int64_t s = 0; s < S && p->planar; ++s) p->planar = ks[(size_t)s].z == 0.0;
+    // the two samples of a lane in a packed tile (same line, group_pack
+    // readout indices apart): the kernel may form the second one's phase as
+    // the first one's reduced phase plus dk.x in FP32 when |dk|_1 |x|_inf
+    // stays small; record the largest |dk|_1
+    p->packed_dk = 0.0;
+    for (size_t b = 0; b + 1 < h.tiles_p.size(); b += kDirectSPL) {
+        const SignalTile &s0 = h.tiles_p[b], &s1 = h.tiles_p[b + 1];
+        const int32_t c = std::min(s0.count, s1.count);
+        for (int32_t l = 0; l < c; ++l) {
+            const double4 a = ks[(size_t)perm[(size_t)(s0.offset + l)]];
+            const double4 q = ks[(size_t)perm[(size_t)(s1.offset + l)]];
+            p->packed_dk = std::max(p->packed_dk, std::fabs(q.x - a.x) + std::fabs(q.y - a.y) + std::fabs(q.z - a.z));
+        }
+    }
 
     // Readout runs: kRunLen consecutive readout samples of a line (stride
     // lines * kz in the sample index) whose k' and t are linear in the

@@ -66,6 +66,10 @@ struct fm_signal_plan {
     // packed slots of group_pack equal-size t-groups (1: none), static nodes
     int group_pack = 1;
     int64_t n_tiles_packed = 0;
+    // largest |k'(slot 1) - k'(slot 0)|_1 over the lanes of the packed tiles
+    // (cycles per unit length), and whether every sample has kz = 0
+    double packed_dk = 0.0;
+    int planar = 0;
     // device tables
     double4 *d_ks = nullptr;      // per sample: kx', ky', kz' (cycles/m, pi_f-scaled), t
     int32_t *d_perm = nullptr;    // samples ordered by t-group (grouped tiles index it)

@@ -146,6 +146,28 @@ def test_spiral_packed_t_groups(gpu, n_il, v):
     _check(kloc, t, n=3001, v=v, seed=n_il)
 
 
+@pytest.mark.parametrize("far", [False, True])
+def test_spiral_packed_pair_phase(gpu, far):
+    """In packed slots the mixed form takes the second sample of a lane (same
+    interleave, 8 readout indices on) as the first one's FP64-reduced phase
+    plus dk.x in FP32, for 256-node chunks whose nodes all have
+    |dk|_1 max|x| <= 4 cycles.  With `far`, nodes 700..1299 sit 25x further
+    out (up to 1.5 m), so their chunks take the FP64 phase for both samples.
+    Either way the error stays far inside the mixed bar: <= 2e-6."""
+    from feelmri_amd import MRI
+    import oracle
+
+    kloc, t = trajectories.spiral(0.24, 64, 90, 32, t_first=0.2)
+    r0, phi, T2, mxy, w = _nodes(3001, 1, 21)
+    if far:
+        r0[700:1300] *= 25.0
+    ref = oracle.Signal(0, sp.diags(w), kloc, t, r0, phi, T2, mxy, None)
+    got = MRI.Signal(0, sp.diags(w), kloc, t, r0, phi, T2, mxy, None, precision="mixed")
+    err = rel_l2(got, ref)
+    print("relL2 mixed", f"{err:.2e}")
+    assert err <= 2e-6
+
+
 def test_random_times_per_sample(gpu):
     rng = np.random.default_rng(5)
     kloc = tuple(rng.uniform(-200, 200, (30, 7, 2)) for _ in range(3))
