@@ -678,11 +678,14 @@ def main() -> int:
     traffic = load_traffic(dom, wl.name, args.form)
     roof = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "traffic": traffic, "kernel_ms": ms_dom,
-            "executed": (3.0 * achieved if (plan.method == "cartesian" and args.form == "mixed") else achieved),
+            "executed": (3.0 * achieved if (plan.method == "cartesian" and args.form == "mixed")
+                         else (8.0 * wl.n_enc / (6.0 + 8.0 * wl.n_enc) * achieved if dom == "k_signal_runs"
+                               else achieved)),
             "per_unit": "8 flops per node*k-sample*encoding (complex MAC); the mixed GEMM executes 3 bf16 "
                         "products (hi*hi + hi*lo + lo*hi) per real product, i.e. 24" if plan.method == "cartesian"
             else ("6 flops (complex step) + 8 per encoding (complex MAC) per node*k-sample, FP32; one FP64-anchored "
-                  "sincos per node and 16-sample readout run" if dom == "k_signal_runs"
+                  "sincos per node and 16-sample readout run; the kernel advances y = e*c per encoding, executing "
+                  "a complex multiply and a complex add (8 flops) per node*k-sample*encoding" if dom == "k_signal_runs"
                   else "8 flops + 8 per encoding and 2 transcendentals per node*k-sample")}
 
     cpu = None
