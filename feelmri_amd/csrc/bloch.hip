@@ -10,6 +10,12 @@
 // integrates its node through the chunk reading the events by broadcast.
 // Step i -> i+1 uses event i+1; e1/e2 are refreshed only when dt changes;
 // |B| is floored at 1e-12; rotation first, then relaxation toward M0.
+//
+// Events without RF (rf = 0, the same for every lane) take the precession
+// the rotation reduces to: beta = 0 and conj(alpha)^2 = exp(-i gamma dt Bz),
+// so Mxy turns by -gamma dt Bz and the rotation leaves Mz alone.  This skips
+// |B|, its reciprocal and the Cayley-Klein products; the result departs from
+// the general step by rounding only (|alpha|^2 is 1 to an ulp).
 #include "fm_common.h"
 
 namespace fm {
@@ -141,6 +147,17 @@ __device__ inline bool uniform_ne(float a, float b)
     return __builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, a)) !=
            __builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, b));
 }
+// a == 0 for a value that is equal across the wave (first lane's bits,
+// either sign of zero)
+__device__ inline bool uniform_zero(double a)
+{
+    const unsigned long long ua = __builtin_bit_cast(unsigned long long, a) << 1;
+    return (__builtin_amdgcn_readfirstlane((unsigned)ua) | __builtin_amdgcn_readfirstlane((unsigned)(ua >> 32))) == 0;
+}
+__device__ inline bool uniform_zero(float a)
+{
+    return (__builtin_amdgcn_readfirstlane(__builtin_bit_cast(unsigned, a)) << 1) == 0;
+}
 
 template <typename T>
 struct Cplx2;
@@ -262,6 +279,24 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                     pz += dz;
                 }
                 const T Bz = px * ev.gx + py * ev.gy + pz * ev.gz + dB;
+                if (uniform_zero(ev.rf2)) {
+                    // free precession: Mxy exp(-i gamma dt Bz); kappa = -gamma dt / 2
+                    T s, c;
+                    dev_sincos<T>((ev.kappa + ev.kappa) * Bz, &s, &c);
+                    const T nmr = c * mxr - s * mxi, nmi = c * mxi + s * mxr;
+                    mxr = nmr * e2;
+                    mxi = nmi * e2;
+                    mz = relax_mz<T>(mz, e1, om1, M0);
+                    if (HISTORY) {
+                        const int64_t o = p + (base + e + 1) * N;
+                        C2 v;
+                        v.x = mxr;
+                        v.y = mxi;
+                        mxy_out[o] = v;
+                        mz_out[o] = mz;
+                    }
+                    continue;
+                }
                 T Bn = dev_sqrt<T>(Bz * Bz + ev.rf2);
                 if (Bn < (T)1e-12) Bn = (T)1e-12;
                 const T inv = dev_rcp<T>(Bn);

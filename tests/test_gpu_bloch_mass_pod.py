@@ -50,6 +50,29 @@ def test_bloch_history(gpu, n, T):
         assert e1 <= max(tol, 1e-6) and e2 <= max(tol, 1e-6)
 
 
+def test_bloch_free_precession_zero_field(gpu):
+    """rf-free events take the precession step Mxy exp(-i gamma dt Bz); with
+    Bz = 0 (node at the origin, no off-resonance, no gradient) the
+    reference floors |B| at 1e-12 (BlochSimulator.cpp:239) and turns
+    by ~1e-12 rad: both leave Mxy unrotated."""
+    from feelmri_amd import BlochSimulator
+    import oracle
+
+    n, T = 300, 60
+    r0, T1, T2, dB, rf, G, dt, mxy0, mz0 = _bloch_inputs(n, T, seed=33)
+    r0[:100] = 0.0
+    dB[:100] = 0.0
+    G[::3] = 0.0
+    rf[10:] = 0.0
+    ref = oracle.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0, history=True)
+    for prec, tol in (("f64", 1e-12), ("f32", 1e-5)):
+        mxy, mz = BlochSimulator.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0,
+                                           precision=prec)
+        e1, e2 = rel_l2(mxy, ref[0]), rel_l2(mz, ref[1])
+        print(prec, f"{e1:.2e} {e2:.2e}")
+        assert e1 <= max(tol, 1e-6) and e2 <= max(tol, 1e-6)
+
+
 def test_bloch_final_state_f64_exact(gpu):
     import torch
 
