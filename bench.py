@@ -166,8 +166,8 @@ def time_shim_per_line(wl, w_h, mxy_h, form, reps: int = 3):
 def time_scan(wl, dev, plan, form, reps: int = 3) -> dict:
     """End-to-end scan on the device: lumped weights, the Bloch solve of the
     workload's sequence to its stored magnetisation, the packed signal
-    nodes and the k-space.  Mesh colouring, event tables and the plan are
-    set up once beforehand (host preprocessing)."""
+    nodes and the k-space.  Mesh colouring, event tables, the plan and the
+    HIP graph of the Bloch block chain are set up once beforehand."""
     import torch
 
     from feelmri_amd import device
@@ -183,6 +183,7 @@ def time_scan(wl, dev, plan, form, reps: int = 3) -> dict:
     solver = BlochSolver(wl.blocks, m.nodes, tis.T1, tis.T2, delta_B=tis.delta_B_mT,
                          precision="f32" if form == "mixed" else "f64", dev=dev)
     mxy0, mz0 = solver.mxy.clone(), solver.mz.clone()
+    graph = solver.capture()   # the block chain as one graph launch
     out = torch.empty((plan.n_samples, wl.n_enc), dtype=torch.complex128, device=dev)
     ws = torch.empty(max(plan.workspace_bytes(form, m.n_nodes, wl.n_enc), 1), dtype=torch.uint8, device=dev)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -195,7 +196,7 @@ def time_scan(wl, dev, plan, form, reps: int = 3) -> dict:
         w = device.lumped_mass(col, nodes_d)
         if record:
             ev[1].record()
-        store, _ = solver.solve_device()
+        store, _ = graph.replay()
         if record:
             ev[2].record()
         mxy = (store[:, -1].to(torch.complex128) * M0).reshape(-1, 1).repeat(1, wl.n_enc)
@@ -225,7 +226,8 @@ def time_scan(wl, dev, plan, form, reps: int = 3) -> dict:
     return {"ms": total, "node_ksamples_per_s": wl.node_ksamples / (total * 1e-3),
             "bloch_relL2_vs_f64": bloch_err,
             "parts_ms": {"lumped_mass": parts[0], "bloch": parts[1], "pack_and_signal": parts[2]},
-            "bloch_blocks": len(wl.blocks), "bloch_precision": "f32" if form == "mixed" else "f64"}
+            "bloch_blocks": len(wl.blocks), "bloch_graph": True,
+            "bloch_precision": "f32" if form == "mixed" else "f64"}
 
 
 def load_traffic(kernel: str, config: str, form: str):

@@ -7,11 +7,13 @@ kept when the block stores magnetisation (:358-359, :380), and the carried
 Mxy is zeroed after every non-empty block -- the reference's artificial
 spoiling -- while Mz is always carried (:363-368).
 
-The state (Mxy complex128, Mz float64) stays in HBM between blocks and
-between ``solve`` calls (the reference keeps it in ``initial_Mxy`` /
-``initial_Mz``); each block is one integrator launch plus one store/spoil
-launch, with no host round trip.  Event tables are uploaded once per block
-and cached.
+The state (Mxy complex128, Mz float64) stays in HBM between ``solve``
+calls (the reference keeps it in ``initial_Mxy`` / ``initial_Mz``).  Static
+nodes run a whole block range as one chained launch: the state stays in
+registers across blocks and the store/spoil steps happen in the kernel.
+Moving nodes take one integrator launch plus one store/spoil launch per
+block, with their motion tables built per block on the host.  Event tables
+are uploaded once and cached.
 """
 from __future__ import annotations
 
@@ -47,6 +49,7 @@ class BlochSolver:
             self.mxy = device.as_device(mxy0.astype(np.complex128), torch.complex128, self.dev)
             self.mz = device.as_device(mz0, torch.float64, self.dev)
         self._events: dict[int, device.EventTable] = {}
+        self._chains: dict[tuple[int, int], device.BlockChain] = {}
 
     def add_block(self, block: Block) -> None:
         self.blocks.append(block)
@@ -59,20 +62,51 @@ class BlochSolver:
             self._events[i] = ev
         return ev
 
-    def solve_device(self, start: int = 0, end: int | None = None, stream=None):
+    def _chain(self, start: int, end: int) -> device.BlockChain:
+        key = (start, end)
+        ch = self._chains.get(key)
+        if ch is None:
+            idx = range(start, end)
+            cols, col = [], 0
+            for i in idx:
+                cols.append(col if self.blocks[i].store else -1)
+                col += int(self.blocks[i].store)
+            ch = device.BlockChain([(self.blocks[i].rf, self.blocks[i].G, self.blocks[i].dt) for i in idx], cols,
+                                   [0 if self.blocks[i].empty else 1 for i in idx], device=self.dev)
+            self._chains[key] = ch
+        return ch
+
+    def solve_device(self, start: int = 0, end: int | None = None, stream=None, chain: bool = True):
         """Run blocks [start, end); returns device (Mxy (N, n_store) complex64,
-        Mz (N, n_store) float32) of the storing blocks."""
+        Mz (N, n_store) float32) of the storing blocks.
+
+        Static nodes run the whole range as one chained launch (the state
+        stays in registers across blocks; equal, bit for bit, to one
+        integrator launch and one store/spoil launch per block, which moving
+        nodes and ``chain=False`` take)."""
         nb = len(self.blocks)
         if start < 0:
             start += nb
         if end is None:
             end = nb
-        idx = list(range(start, end))
-        store = [i for i in idx if self.blocks[i].store]
+        n_store = sum(1 for i in range(start, end) if self.blocks[i].store)
         N = self.n_nodes
         # column-major (N, n_store) stores, as (n_store, N) row-major buffers
-        mxy_store = torch.zeros((max(len(store), 1), N), dtype=torch.complex64, device=self.dev)
-        mz_store = torch.zeros((max(len(store), 1), N), dtype=torch.float32, device=self.dev)
+        mxy_store = torch.zeros((max(n_store, 1), N), dtype=torch.complex64, device=self.dev)
+        mz_store = torch.zeros((max(n_store, 1), N), dtype=torch.float32, device=self.dev)
+        self._run(start, end, mxy_store, mz_store, stream=stream, chain=chain)
+        return mxy_store[:n_store].T, mz_store[:n_store].T
+
+    def _run(self, start: int, end: int, mxy_store, mz_store, stream=None, chain: bool = True):
+        if end <= start:
+            return
+        if chain and self.pod_trajectory is None:
+            with torch.cuda.device(self.dev):
+                device.bloch_chain(self.nodes, self._chain(start, end), self.M0, self.gamma, self.mxy, self.mz,
+                                   mxy_store, mz_store, precision=self.precision, stream=stream)
+            return
+        idx = list(range(start, end))
+        N = self.n_nodes
         col = 0
         with torch.cuda.device(self.dev):
             for i in idx:
@@ -91,8 +125,60 @@ class BlochSolver:
                     col += 1
                 device.bloch_store_spoil(self.mxy, self.mz, mxy_store, mz_store, col=c,
                                          spoil=not blk.empty, stream=stream)
-        return mxy_store[: len(store)].T, mz_store[: len(store)].T
 
     def solve(self, start: int = 0, end: int | None = None):
         mxy, mz = self.solve_device(start, end)
         return mxy.cpu().numpy(), mz.cpu().numpy()
+
+    def capture(self, stream=None) -> "BlochSolveGraph":
+        """Record the whole block chain as one HIP graph (static nodes).
+
+        Each replay integrates every block from the state then held in
+        ``self.mxy`` / ``self.mz`` and leaves the carried state there, as
+        ``solve_device()`` does, with one graph launch.
+        """
+        return BlochSolveGraph(self, stream)
+
+
+class BlochSolveGraph:
+    """A ``BlochSolver``'s block chain captured once and replayed.
+
+    Event tables are uploaded and the kernels loaded (one eager run, after
+    which the carried state is restored) before the capture. The store
+    buffers belong to the graph; every replay rewrites all stored columns.
+    Moving nodes are not covered: their per-block tables come from the host.
+    """
+
+    def __init__(self, solver: BlochSolver, stream=None):
+        if solver.pod_trajectory is not None:
+            raise ValueError("graph capture covers static nodes; moving nodes build per-block motion tables "
+                             "on the host")
+        self.solver = solver
+        N = solver.n_nodes
+        nb = len(solver.blocks)
+        self.n_store = sum(1 for b in solver.blocks if b.store)
+        solver._chain(0, nb)   # uploaded before the capture
+        with torch.cuda.device(solver.dev):
+            self.stream = stream if stream is not None else torch.cuda.Stream(device=solver.dev)
+            self.mxy_store = torch.zeros((max(self.n_store, 1), N), dtype=torch.complex64, device=solver.dev)
+            self.mz_store = torch.zeros((max(self.n_store, 1), N), dtype=torch.float32, device=solver.dev)
+            mxy0, mz0 = solver.mxy.clone(), solver.mz.clone()
+            self.stream.wait_stream(torch.cuda.current_stream(solver.dev))
+            with torch.cuda.stream(self.stream):
+                self._body()
+                solver.mxy.copy_(mxy0)
+                solver.mz.copy_(mz0)
+            self.stream.synchronize()
+            self.graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(self.graph, stream=self.stream):
+                self._body()
+
+    def _body(self):
+        s = self.solver
+        s._run(0, len(s.blocks), self.mxy_store, self.mz_store)
+
+    def replay(self):
+        """Replay on the current stream's device; returns device (Mxy (N, n_store)
+        complex64, Mz (N, n_store) float32), valid once the replay has run."""
+        self.graph.replay()
+        return self.mxy_store[: self.n_store].T, self.mz_store[: self.n_store].T

@@ -50,6 +50,8 @@ _SIGNATURES = {
     "fm_bloch_execute": [_int, _i64, _i64, _p, _dbl, _dbl, _p, _p, _p, _p, _p, _p, _int, _p, _p,
                          _p, _int, _p],
     "fm_bloch_store_spoil": [_i64, _p, _p, _p, _p, _i64, _int, _p],
+    "fm_bloch_chain": [_int, _i64, _p, _dbl, _dbl, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p,
+                       _p],
     "fm_mass_p1_color": [_i64, _i64, _p, _p, _p, _int, ctypes.POINTER(_int)],
     "fm_mass_p1_lumped": [_i64, _i64, _p, _int, _p, _p, _p, _int, _p, _p],
     "fm_mass_p1_element_matrices": [_i64, _p, _int, _p, _p, _p],

@@ -16,6 +16,9 @@
 // so Mxy turns by -gamma dt Bz and the rotation leaves Mz alone.  This skips
 // |B|, its reciprocal and the Cayley-Klein products; the result departs from
 // the general step by rounding only (|alpha|^2 is 1 to an ulp).
+#include <algorithm>
+#include <type_traits>
+
 #include "fm_common.h"
 
 namespace fm {
@@ -23,6 +26,21 @@ namespace fm {
 constexpr int kBlochThreads = 256;
 constexpr int kBlochChunk = 256;
 constexpr int kBlochMaxModes = 16;
+constexpr int kChainMaxBlocks = 128;  // blocks per chained launch
+
+// A chain of blocks in one launch (static nodes, final state): the blocks'
+// event tables are concatenated; offset[b] is block b's first event.  That
+// event is not a step (step i of a block uses its event i + 1): it ends
+// block b - 1 instead, applying code[b - 1] = spoil | (store column + 1) << 1
+// -- the store of the state into column `col` of the (n_store, N) stores,
+// then the artificial spoiling Mxy = 0 (Bloch.py:355-368).  code[n - 1] is
+// applied after the last event.
+struct ChainDesc {
+    int64_t offset[kChainMaxBlocks + 1];
+    int32_t code[kChainMaxBlocks];
+    int32_t n;
+};
+struct NoChain {};
 
 template <typename T>
 struct alignas(16) Event {
@@ -170,18 +188,21 @@ struct Cplx2<float> {
     using type = float2;
 };
 
-template <typename T, bool MOTION, bool HISTORY>
+template <typename T, bool MOTION, bool HISTORY, bool CHAIN = false>
 __global__ __launch_bounds__(kBlochThreads) void k_bloch(
     const double *__restrict__ packed, int64_t N, int64_t n_time, double M0d, double gammad,
     const double *__restrict__ rf, const double *__restrict__ G, const double *__restrict__ dt,
     const double *mxy_in, const double *mz_in,  // may alias mxy_fin / mz_fin
     const float *__restrict__ modes, int R, const double *__restrict__ mcoef,
     typename Cplx2<T>::type *__restrict__ mxy_out, T *__restrict__ mz_out,
-    double2 *mxy_fin, double *mz_fin)
+    double2 *mxy_fin, double *mz_fin, const std::conditional_t<CHAIN, ChainDesc, NoChain> chain = {},
+    float2 *__restrict__ store_xy = nullptr, float *__restrict__ store_z = nullptr)
 {
     using C2 = typename Cplx2<T>::type;
+    static_assert(!CHAIN || (!MOTION && !HISTORY), "chained blocks: static nodes, final state");
     __shared__ Event<T> sev[kBlochChunk];
     __shared__ T scoef[MOTION ? kBlochChunk * (kBlochMaxModes + 3) : 1];
+    __shared__ int32_t smark[CHAIN ? kBlochChunk : 1];  // 0: a step; else 1 + block-end code
 
     const int tid = threadIdx.x;
     const int64_t p = (int64_t)blockIdx.x * kBlochThreads + tid;
@@ -242,6 +263,23 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                 ev.rf_re = ev.rf_im = ev.rf2 = ev.kappa = ev.gx = ev.gy = ev.gz = ev.dt = 0;
             }
             sev[tid] = ev;
+            if constexpr (CHAIN) {
+                // event i + 1 starts block b > 0: it ends block b - 1
+                int mk = 0;
+                if (i < n_steps) {
+                    int lo = 1, hi = chain.n - 1;
+                    while (lo <= hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (chain.offset[mid] == i + 1) {
+                            mk = 1 + chain.code[mid - 1];
+                            break;
+                        }
+                        if (chain.offset[mid] < i + 1) lo = mid + 1;
+                        else hi = mid - 1;
+                    }
+                }
+                smark[tid] = mk;
+            }
             if (MOTION) {
                 for (int q = 0; q < RC; ++q)
                     scoef[tid * RC + q] = (i < n_steps) ? (T)mcoef[i * RC + q] : (T)0;
@@ -251,6 +289,18 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
         const int cnt = (n_steps - base < kBlochChunk) ? (int)(n_steps - base) : kBlochChunk;
         if (active) {
             for (int e = 0; e < cnt; ++e) {
+                if constexpr (CHAIN) {
+                    const int mk = smark[e];
+                    if (mk != 0) {  // the same for every lane: a block ends here
+                        const int col = ((mk - 1) >> 1) - 1;
+                        if (col >= 0) {
+                            store_xy[p + col * N] = make_float2((float)mxr, (float)mxi);
+                            store_z[p + col * N] = (float)mz;
+                        }
+                        if ((mk - 1) & 1) mxr = mxi = 0;
+                        continue;
+                    }
+                }
                 const Event<T> ev = sev[e];
                 // BlochSimulator.cpp:287-294: e1/e2 only when dt changes.  dt
                 // is the same for every lane (LDS broadcast), so test it on
@@ -330,6 +380,17 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
             }
         }
     }
+    if constexpr (CHAIN) {
+        if (active) {  // the last block's end
+            const int code = chain.code[chain.n - 1];
+            const int col = (code >> 1) - 1;
+            if (col >= 0) {
+                store_xy[p + col * N] = make_float2((float)mxr, (float)mxi);
+                store_z[p + col * N] = (float)mz;
+            }
+            if (code & 1) mxr = mxi = 0;
+        }
+    }
     if (!HISTORY && active) {
         mxy_fin[p] = make_double2((double)mxr, (double)mxi);
         mz_fin[p] = (double)mz;
@@ -391,6 +452,18 @@ static int launch_bloch(bool motion, bool history, int64_t N, int64_t n_time, co
 #undef FM_BLOCH_ARGS
     FM_LAUNCH_CHECK("k_bloch");
     return FM_OK;
+}
+
+template <typename T>
+static void launch_bloch_chain(int64_t N, int64_t n_time, const double *packed, double M0, double gamma,
+                               const double *rf, const double *G, const double *dt, const double *mxy_in,
+                               const double *mz_in, double *mxy_out, double *mz_out, const ChainDesc &cd,
+                               float2 *store_xy, float *store_z, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)cdiv(N, kBlochThreads);
+    k_bloch<T, false, false, true><<<blocks, kBlochThreads, 0, st>>>(
+        packed, N, n_time, M0, gamma, rf, G, dt, mxy_in, mz_in, nullptr, 0, nullptr, nullptr, nullptr,
+        reinterpret_cast<double2 *>(mxy_out), mz_out, cd, store_xy, store_z);
 }
 
 }  // namespace fm
@@ -455,5 +528,53 @@ extern "C" int fm_bloch_store_spoil(int64_t n_pos, double *mxy_state, const doub
         n_pos, reinterpret_cast<double2 *>(mxy_state), mz_state, reinterpret_cast<float2 *>(mxy_store),
         reinterpret_cast<float *>(mz_store), store_col, spoil);
     FM_LAUNCH_CHECK("k_store_spoil");
+    return FM_OK;
+}
+
+extern "C" int fm_bloch_chain(int precision, int64_t n_pos, const double *packed, double M0, double gamma,
+                              int64_t n_blocks, const int64_t *block_len, const int32_t *store_col,
+                              const int32_t *spoil, const double *rf, const double *G, const double *dt,
+                              const double *mxy_in, const double *mz_in, void *mxy_store, void *mz_store,
+                              int64_t n_store, double *mxy_out, double *mz_out, void *stream)
+{
+    FM_CHECK_ARG(precision == FM_PREC_F64 || precision == FM_PREC_F32,
+                 "Bloch precision must be FM_PREC_F64 or FM_PREC_F32 (got %d)", precision);
+    FM_CHECK_ARG(n_pos >= 0 && n_blocks >= 0 && n_store >= 0, "negative size");
+    if (n_pos == 0 || n_blocks == 0) return FM_OK;
+    FM_CHECK_ARG(block_len && store_col && spoil, "NULL block description");
+    FM_CHECK_ARG(packed && mxy_in && mz_in && mxy_out && mz_out && rf && G && dt, "NULL array");
+    for (int64_t b = 0; b < n_blocks; ++b) {
+        FM_CHECK_ARG(block_len[b] >= 1, "block %lld has no event", (long long)b);
+        FM_CHECK_ARG(store_col[b] >= -1 && store_col[b] < n_store, "block %lld: store column %d outside [-1, %lld)",
+                     (long long)b, store_col[b], (long long)n_store);
+    }
+    FM_CHECK_ARG(n_store == 0 || (mxy_store && mz_store), "NULL store");
+    hipStream_t st = as_stream(stream);
+    int64_t ev0 = 0;
+    for (int64_t b0 = 0; b0 < n_blocks; b0 += kChainMaxBlocks) {
+        const int nb = (int)std::min<int64_t>(kChainMaxBlocks, n_blocks - b0);
+        ChainDesc cd;
+        cd.n = nb;
+        int64_t o = 0;
+        for (int k = 0; k < nb; ++k) {
+            cd.offset[k] = o;
+            cd.code[k] = (spoil[b0 + k] ? 1 : 0) | ((store_col[b0 + k] + 1) << 1);
+            o += block_len[b0 + k];
+        }
+        cd.offset[nb] = o;
+        // later launches continue from the state the previous one left
+        const double *xin = b0 == 0 ? mxy_in : mxy_out;
+        const double *zin = b0 == 0 ? mz_in : mz_out;
+        if (precision == FM_PREC_F64)
+            launch_bloch_chain<double>(n_pos, o, packed, M0, gamma, rf + 2 * ev0, G + 3 * ev0, dt + ev0, xin, zin,
+                                       mxy_out, mz_out, cd, reinterpret_cast<float2 *>(mxy_store),
+                                       reinterpret_cast<float *>(mz_store), st);
+        else
+            launch_bloch_chain<float>(n_pos, o, packed, M0, gamma, rf + 2 * ev0, G + 3 * ev0, dt + ev0, xin, zin,
+                                      mxy_out, mz_out, cd, reinterpret_cast<float2 *>(mxy_store),
+                                      reinterpret_cast<float *>(mz_store), st);
+        FM_LAUNCH_CHECK("k_bloch (chained blocks)");
+        ev0 += o;
+    }
     return FM_OK;
 }

@@ -308,6 +308,49 @@ def bloch(nodes: BlochNodes, events: EventTable, M0: float, gamma: float, mxy0: 
     return mxy_out, mz_out
 
 
+class BlockChain:
+    """Consecutive blocks for ``bloch_chain``: their event tables concatenated
+    on the device, and per block (host) its event count, store column (-1:
+    none) and spoil flag."""
+
+    def __init__(self, events: list, store_col, spoil, device=None):
+        device = torch.device(device) if device is not None else default_device()
+        rf = np.concatenate([np.asarray(e[0]).reshape(-1) for e in events])
+        G = np.concatenate([np.asarray(e[1]).reshape(-1, 3) for e in events])
+        dt = np.concatenate([np.asarray(e[2]).reshape(-1) for e in events])
+        self.events = EventTable(rf, G, dt, device=device)
+        self.block_len = np.ascontiguousarray([np.asarray(e[2]).size for e in events], dtype=np.int64)
+        self.store_col = np.ascontiguousarray(store_col, dtype=np.int32)
+        self.spoil = np.ascontiguousarray(spoil, dtype=np.int32)
+        self.n_blocks = int(self.block_len.size)
+        if self.store_col.size != self.n_blocks or self.spoil.size != self.n_blocks:
+            raise ValueError("one store column and one spoil flag per block")
+        self.n_store = int(self.store_col.max()) + 1 if self.n_blocks else 0
+
+
+def bloch_chain(nodes: BlochNodes, chain: BlockChain, M0: float, gamma: float, mxy, mz, mxy_store, mz_store,
+                precision="f64", stream=None):
+    """Integrate a block chain of static nodes in one launch (per 128 blocks):
+    the state (``mxy`` complex128, ``mz`` float64, (N,)) is updated in place
+    to the state after the last block; stored columns go to ``mxy_store`` /
+    ``mz_store`` ((n_store, N) complex64 / float32, row-major)."""
+    prec = _prec(precision)
+    N = nodes.n_pos
+    if mxy.dtype != torch.complex128 or mz.dtype != torch.float64 or mxy.numel() != N or mz.numel() != N:
+        raise ValueError("chain state must be complex128 / float64 of n_pos entries")
+    if chain.n_store and (mxy_store is None or mz_store is None or mxy_store.dtype != torch.complex64
+                          or mz_store.dtype != torch.float32 or mxy_store.shape[0] < chain.n_store
+                          or mxy_store.shape[1] != N or tuple(mz_store.shape) != tuple(mxy_store.shape)):
+        raise ValueError("stores must be (n_store, N) complex64 / float32")
+    ev = chain.events
+    with torch.cuda.device(nodes.packed.device):
+        _lib.call("fm_bloch_chain", prec, N, ptr(nodes.packed), float(M0), float(gamma), chain.n_blocks,
+                  chain.block_len.ctypes.data_as(ctypes.c_void_p), chain.store_col.ctypes.data_as(ctypes.c_void_p),
+                  chain.spoil.ctypes.data_as(ctypes.c_void_p), ptr(torch.view_as_real(ev.rf)), ptr(ev.G),
+                  ptr(ev.dt), ptr(torch.view_as_real(mxy)), ptr(mz), ptr(mxy_store), ptr(mz_store), chain.n_store,
+                  ptr(torch.view_as_real(mxy)), ptr(mz), stream_handle(stream))
+
+
 def bloch_store_spoil(mxy_state, mz_state, mxy_store=None, mz_store=None, col: int = -1,
                       spoil: bool = False, stream=None):
     N = int(mz_state.numel())

@@ -263,9 +263,10 @@ class SteadyStateGraph:
     """Spoiled-GRE steady state of one resident phantom, captured as a HIP graph.
 
     The TR loop is recorded once per phantom class and stream slot, then
-    replayed per request as one launch instead of two per block:
+    replayed per request as one graph launch:
     - per TR one imaging block and one delay block;
-    - each block is an integrator launch and a store/spoil launch.
+    - all blocks run as one chained integrator launch, with the state in
+      registers and the store and spoiling in the kernel.
 
     The graph owns its state and store buffers. The initial state
     (Mxy = 0, Mz = M0 = 1) is written by fill kernels inside the graph, so
@@ -279,7 +280,8 @@ class SteadyStateGraph:
         from . import device, sequences
 
         N = ph["mesh"].n_nodes
-        self.events = [device.EventTable(b.rf, b.G, b.dt, device=dev) for b in blocks]
+        self.chain = device.BlockChain([(b.rf, b.G, b.dt) for b in blocks], [0 if b.store else -1 for b in blocks],
+                                       [0 if b.empty else 1 for b in blocks], device=dev)
         self.blocks = blocks
         self.nodes = ph["bloch"]
         self.mxy = torch.empty(N, dtype=torch.complex128, device=dev)
@@ -301,10 +303,9 @@ class SteadyStateGraph:
         self.mz.fill_(1.0)
         self.store_xy.zero_()
         self.store_z.zero_()
-        for b, ev in zip(self.blocks, self.events):
-            device.bloch(self.nodes, ev, 1.0, self.gamma, self.mxy, self.mz, mxy_out=self.mxy, mz_out=self.mz)
-            device.bloch_store_spoil(self.mxy, self.mz, self.store_xy, self.store_z, col=0 if b.store else -1,
-                                     spoil=not b.empty)
+        # the TR loop as one chained integrator launch (state in registers)
+        device.bloch_chain(self.nodes, self.chain, 1.0, self.gamma, self.mxy, self.mz, self.store_xy,
+                           self.store_z)
 
     def replay(self):
         self.graph.replay()

@@ -180,6 +180,26 @@ int fm_bloch_store_spoil(int64_t n_pos, double *mxy_state, const double *mz_stat
                          void *mxy_store, void *mz_store, int64_t store_col, int spoil,
                          void *stream);
 
+/* A whole block chain of BlochSolver.solve (python/feelmri/Bloch.py:
+ * 331-368: per block one solve_mri call from the carried state, the last
+ * column stored when the block stores, Mxy zeroed after a non-empty block,
+ * Mz carried) for static nodes, with the state kept in registers across
+ * blocks.  Equal, bit for bit, to fm_bloch_execute (history == 0) followed by
+ * fm_bloch_store_spoil for every block.
+ * block_len, store_col, spoil: host arrays of n_blocks entries; block b has
+ * block_len[b] >= 1 events, stores into column store_col[b] (-1: none) of
+ * the (n_pos, n_store) column-major complex64/float32 stores and spoils when
+ * spoil[b] != 0.  rf, G, dt: device, the blocks' event tables concatenated
+ * (sum of block_len events, layouts as fm_bloch_execute).  mxy_in/mz_in:
+ * initial state; mxy_out/mz_out: state after the last block (may alias the
+ * inputs), complex128/float64.  One launch per 128 blocks; nothing is
+ * allocated, so the call can be recorded into a HIP graph. */
+int fm_bloch_chain(int precision, int64_t n_pos, const double *packed, double M0, double gamma,
+                   int64_t n_blocks, const int64_t *block_len, const int32_t *store_col,
+                   const int32_t *spoil, const double *rf, const double *G, const double *dt,
+                   const double *mxy_in, const double *mz_in, void *mxy_store, void *mz_store,
+                   int64_t n_store, double *mxy_out, double *mz_out, void *stream);
+
 /* ------------------------------------------------------------------------
  * P1 mass -- replaces Assemble.MassAssemble (cpp/feelmri/Assemble.h:21-39;
  * Assemble.cpp:116-155) followed by lumping (python/feelmri/

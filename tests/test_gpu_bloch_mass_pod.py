@@ -139,6 +139,83 @@ def test_bloch_solver_spoiled_gre_chain(gpu):
     assert rel_l2(got_xy, want_xy) <= 1e-6 and rel_l2(got_z, want_z) <= 1e-6
 
 
+@pytest.mark.parametrize("n_dummy", [3, 70])
+def test_bloch_chain_equals_block_launches(gpu, n_dummy):
+    """Static nodes run a block range as one chained launch (one per 128
+    blocks), with the store and the artificial spoiling in the kernel.  It
+    equals one integrator launch plus one store/spoil launch per block bit
+    for bit (stores and carried state, both precisions) over spoiling and
+    non-spoiling blocks, stored and unstored ones, a one-event block, random
+    gradients, and with 70 dummy TRs (149 blocks) across a launch boundary;
+    a second solve continues from the carried state."""
+    import torch
+
+    from feelmri_amd.Bloch import BlochSolver
+
+    n = 2000
+    rng = np.random.default_rng(13)
+    r0 = rng.uniform(-0.05, 0.05, (n, 3))
+    T1 = rng.uniform(500, 1500, n)
+    T2 = rng.uniform(40, 100, n)
+    dB = rng.normal(0, 1e-4, n)
+    blocks = sequences.spoiled_gre(flip_deg=15, TR=5.0, n_dummy=n_dummy, n_lines=3, rf_dur=0.2)
+    one = sequences.Block(times=np.array([0.0]), rf=np.zeros(1, complex), G=np.zeros((1, 3)), dt=np.array([0.5]),
+                          store=True)
+    grad = sequences.Block(times=np.arange(7) * 0.1, rf=np.zeros(7, complex), G=rng.normal(0, 5, (7, 3)),
+                           dt=np.full(7, 0.1), store=True)
+    blocks = blocks[:4] + [one, grad] + blocks[4:] + [grad]
+    n_store = sum(b.store for b in blocks)
+    for prec in ("f64", "f32"):
+        a = BlochSolver(blocks, r0, T1, T2, delta_B=dB, precision=prec, dev=gpu)
+        b = BlochSolver(blocks, r0, T1, T2, delta_B=dB, precision=prec, dev=gpu)
+        for _ in range(2):
+            xa, za = a.solve_device(chain=True)
+            xb, zb = b.solve_device(chain=False)
+            torch.cuda.synchronize()
+            assert xa.shape == (n, n_store)
+            assert torch.equal(xa, xb) and torch.equal(za, zb)
+            assert torch.equal(a.mxy, b.mxy) and torch.equal(a.mz, b.mz)
+        # a sub-range, as solve(start, end) runs it
+        xa, za = a.solve_device(2, 9)
+        xb, zb = b.solve_device(2, 9, chain=False)
+        torch.cuda.synchronize()
+        assert torch.equal(xa, xb) and torch.equal(za, zb)
+
+
+def test_bloch_solver_graph_replays_match_eager(gpu):
+    """BlochSolver.capture() records the block chain as one HIP graph.  Every
+    replay, with eager work and synchronisation between replays, equals the
+    eager solve_device() bitwise: stored columns and carried state, in both
+    precisions."""
+    import torch
+
+    from feelmri_amd.Bloch import BlochSolver
+
+    n = 3000
+    rng = np.random.default_rng(9)
+    r0 = rng.uniform(-0.05, 0.05, (n, 3))
+    T1 = rng.uniform(500, 1500, n)
+    T2 = rng.uniform(40, 100, n)
+    dB = rng.normal(0, 1e-4, n)
+    blocks = sequences.spoiled_gre(flip_deg=15, TR=5.0, n_dummy=4, n_lines=3, rf_dur=0.2)
+    for prec in ("f64", "f32"):
+        solver = BlochSolver(blocks, r0, T1, T2, delta_B=dB, precision=prec, dev=gpu)
+        mxy0, mz0 = solver.mxy.clone(), solver.mz.clone()
+        want_xy, want_z = (t.clone() for t in solver.solve_device())
+        carried = (solver.mxy.clone(), solver.mz.clone())
+        graph = solver.capture()
+        for _ in range(3):
+            solver.mxy.copy_(mxy0)
+            solver.mz.copy_(mz0)
+            got_xy, got_z = graph.replay()
+            torch.cuda.synchronize()
+            assert got_xy.shape == want_xy.shape == (n, 3)
+            assert torch.equal(got_xy, want_xy) and torch.equal(got_z, want_z)
+            assert torch.equal(solver.mxy, carried[0]) and torch.equal(solver.mz, carried[1])
+            # eager work between replays
+            assert float((torch.arange(4096, device=gpu, dtype=torch.float64) * 2.0).sum()) == 4095 * 4096
+
+
 def test_lumped_mass_c0(gpu):
     from feelmri_amd import Assemble
     import oracle
