@@ -540,16 +540,29 @@ extern "C" int fm_bloch_chain(int precision, int64_t n_pos, const double *packed
     FM_CHECK_ARG(precision == FM_PREC_F64 || precision == FM_PREC_F32,
                  "Bloch precision must be FM_PREC_F64 or FM_PREC_F32 (got %d)", precision);
     FM_CHECK_ARG(n_pos >= 0 && n_blocks >= 0 && n_store >= 0, "negative size");
-    if (n_pos == 0 || n_blocks == 0) return FM_OK;
+    if (n_pos == 0) return FM_OK;
+    FM_CHECK_ARG(mxy_in && mz_in && mxy_out && mz_out, "NULL state");
+    hipStream_t st = as_stream(stream);
+    if (n_blocks == 0) {  // no block: the state passes through
+        hipError_t e = hipSuccess;
+        if (mxy_out != mxy_in)
+            e = hipMemcpyAsync(mxy_out, mxy_in, 2 * sizeof(double) * (size_t)n_pos, hipMemcpyDeviceToDevice, st);
+        if (e == hipSuccess && mz_out != mz_in)
+            e = hipMemcpyAsync(mz_out, mz_in, sizeof(double) * (size_t)n_pos, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) {
+            set_error("fm_bloch_chain: %s", hipGetErrorString(e));
+            return FM_ERR_HIP;
+        }
+        return FM_OK;
+    }
     FM_CHECK_ARG(block_len && store_col && spoil, "NULL block description");
-    FM_CHECK_ARG(packed && mxy_in && mz_in && mxy_out && mz_out && rf && G && dt, "NULL array");
+    FM_CHECK_ARG(packed && rf && G && dt, "NULL array");
     for (int64_t b = 0; b < n_blocks; ++b) {
         FM_CHECK_ARG(block_len[b] >= 1, "block %lld has no event", (long long)b);
         FM_CHECK_ARG(store_col[b] >= -1 && store_col[b] < n_store, "block %lld: store column %d outside [-1, %lld)",
                      (long long)b, store_col[b], (long long)n_store);
     }
     FM_CHECK_ARG(n_store == 0 || (mxy_store && mz_store), "NULL store");
-    hipStream_t st = as_stream(stream);
     int64_t ev0 = 0;
     for (int64_t b0 = 0; b0 < n_blocks; b0 += kChainMaxBlocks) {
         const int nb = (int)std::min<int64_t>(kChainMaxBlocks, n_blocks - b0);

@@ -192,8 +192,9 @@ int fm_bloch_store_spoil(int64_t n_pos, double *mxy_state, const double *mz_stat
  * spoil[b] != 0.  rf, G, dt: device, the blocks' event tables concatenated
  * (sum of block_len events, layouts as fm_bloch_execute).  mxy_in/mz_in:
  * initial state; mxy_out/mz_out: state after the last block (may alias the
- * inputs), complex128/float64.  One launch per 128 blocks; nothing is
- * allocated, so the call can be recorded into a HIP graph. */
+ * inputs), complex128/float64; with n_blocks == 0 they receive the input
+ * state.  One launch per 128 blocks; nothing is allocated, so the call can
+ * be recorded into a HIP graph. */
 int fm_bloch_chain(int precision, int64_t n_pos, const double *packed, double M0, double gamma,
                    int64_t n_blocks, const int64_t *block_len, const int32_t *store_col,
                    const int32_t *spoil, const double *rf, const double *G, const double *dt,

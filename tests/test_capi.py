@@ -50,6 +50,48 @@ def test_argument_errors_do_not_need_a_gpu():
         _lib.call("fm_signal_execute", None, 0, 1, 1, None, None, 99, None, None, 0, None, None)
 
 
+def test_bloch_chain_argument_checks_do_not_need_a_gpu():
+    """fm_bloch_chain rejects bad block descriptions before any launch."""
+    import numpy as np
+
+    from feelmri_amd import _lib
+
+    d = ctypes.c_void_p(1)   # never dereferenced: the checks fail first
+    lens = np.array([3, 0], np.int64)
+    cols = np.array([-1, 0], np.int32)
+    spoil = np.array([1, 0], np.int32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    with pytest.raises(_lib.FeelmriHipError, match="has no event"):
+        _lib.call("fm_bloch_chain", _lib.PREC_F32, 10, d, 1.0, 1.0, 2, p(lens), p(cols), p(spoil), d, d, d, d, d, d, d, 1,
+                  d, d, None)
+    lens[1] = 2
+    cols[1] = 1
+    with pytest.raises(_lib.FeelmriHipError, match="store column"):
+        _lib.call("fm_bloch_chain", _lib.PREC_F32, 10, d, 1.0, 1.0, 2, p(lens), p(cols), p(spoil), d, d, d, d, d, d, d, 1,
+                  d, d, None)
+    with pytest.raises(_lib.FeelmriHipError, match="precision"):
+        _lib.call("fm_bloch_chain", 7, 10, d, 1.0, 1.0, 2, p(lens), p(cols), p(spoil), d, d, d, d, d, d, d, 1,
+                  d, d, None)
+
+
+def test_block_chain_encoding():
+    """BlochSolver's chain description: per block its event count, store
+    column (stored blocks numbered in order, -1 otherwise) and spoil flag
+    (set after every non-empty block, Bloch.py:363-367)."""
+    import numpy as np
+
+    from feelmri_amd import device, sequences
+
+    blocks = sequences.spoiled_gre(flip_deg=15, TR=5.0, n_dummy=2, n_lines=3, rf_dur=0.2)
+    ch = device.BlockChain([(b.rf, b.G, b.dt) for b in blocks],
+                           [-1] * 4 + [0, -1, 1, -1, 2, -1], [0 if b.empty else 1 for b in blocks], device="cpu")
+    assert ch.n_blocks == len(blocks) == 10 and ch.n_store == 3
+    assert ch.block_len.tolist() == [b.dt.size for b in blocks]
+    assert ch.events.n_time == sum(b.dt.size for b in blocks)
+    assert ch.spoil.tolist() == [1, 0] * 5            # imaging blocks spoil, delay blocks are empty
+    np.testing.assert_array_equal(ch.events.dt.numpy(), np.concatenate([b.dt for b in blocks]))
+
+
 def test_colouring_is_a_valid_partition():
     import numpy as np
 

@@ -182,6 +182,29 @@ def test_bloch_chain_equals_block_launches(gpu, n_dummy):
         assert torch.equal(xa, xb) and torch.equal(za, zb)
 
 
+def test_bloch_chain_without_blocks_passes_the_state(gpu):
+    import ctypes
+
+    import torch
+
+    from feelmri_amd import _lib, device
+
+    n = 1000
+    rng = np.random.default_rng(2)
+    nodes = device.BlochNodes.from_arrays(rng.uniform(-0.05, 0.05, (n, 3)), np.full(n, 900.0),
+                                          np.full(n, 60.0), np.zeros(n), device=gpu)
+    mxy = torch.randn(n, dtype=torch.complex128, device=gpu)
+    mz = torch.rand(n, dtype=torch.float64, device=gpu)
+    mxy_o = torch.zeros_like(mxy)
+    mz_o = torch.zeros_like(mz)
+    ptr = device.ptr
+    _lib.call("fm_bloch_chain", _lib.PREC_F64, n, ptr(nodes.packed), 1.0, 1.0, 0, None, None, None, None, None,
+              None, ptr(torch.view_as_real(mxy)), ptr(mz), None, None, 0, ptr(torch.view_as_real(mxy_o)),
+              ptr(mz_o), None)
+    torch.cuda.synchronize()
+    assert torch.equal(mxy_o, mxy) and torch.equal(mz_o, mz)
+
+
 def test_bloch_solver_graph_replays_match_eager(gpu):
     """BlochSolver.capture() records the block chain as one HIP graph.  Every
     replay, with eager work and synchronisation between replays, equals the
