@@ -73,6 +73,38 @@ def test_bloch_free_precession_zero_field(gpu):
         assert e1 <= max(tol, 1e-6) and e2 <= max(tol, 1e-6)
 
 
+def test_bloch_repeated_events(gpu):
+    """Runs of equal events (rf, G and dt repeated, as in a hard pulse or on a
+    gradient plateau, at 10-20 us spacing): the oracle's result, history and
+    final state."""
+    import torch
+
+    from feelmri_amd import BlochSimulator, device
+    import oracle
+
+    n, T = 700, 400
+    r0, T1, T2, dB, rf, G, dt, mxy0, mz0 = _bloch_inputs(n, T, seed=41)
+    rng = np.random.default_rng(41)
+    dt = rng.choice([0.01, 0.02], T)   # event spacing of pulses and plateaus
+    i = 0
+    while i < T:
+        L = int(rng.integers(1, 30))
+        rf[i:i + L], G[i:i + L], dt[i:i + L] = rf[i], G[i], dt[i]
+        i += L
+    ref = oracle.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0, history=True)
+    for prec, tol in (("f64", 1e-10), ("f32", 1e-4)):
+        mxy, mz = BlochSimulator.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0,
+                                           precision=prec)
+        e1, e2 = rel_l2(mxy, ref[0]), rel_l2(mz, ref[1])
+        print(prec, f"{e1:.2e} {e2:.2e}")
+        assert e1 <= max(tol, 1e-6) and e2 <= max(tol, 1e-6)
+    fin = oracle.solve_mri(r0, T1, T2, dB, 1.0, GAMMA, rf, G, dt, None, mxy0, mz0, history=False)
+    nodes = device.BlochNodes.from_arrays(r0, T1, T2, dB, device=gpu)
+    ev = device.EventTable(rf, G, dt, device=gpu)
+    gx, gz = device.bloch(nodes, ev, 1.0, GAMMA, torch.from_numpy(mxy0).to(gpu), torch.from_numpy(mz0).to(gpu))
+    assert rel_l2(gx.cpu().numpy(), fin[0]) <= 1e-10 and rel_l2(gz.cpu().numpy(), fin[1]) <= 1e-10
+
+
 def test_bloch_final_state_f64_exact(gpu):
     import torch
 
