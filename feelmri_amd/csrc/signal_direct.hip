@@ -353,13 +353,20 @@ struct alignas(16) RunNode {
     float c[2 * VB];
 };
 
-template <int VB>
+// SHQ: the tile's runs share at most kRunSteps steps (dk, dt); the staging
+// threads form each node's step factor once per step into LDS, and the lanes
+// read it instead of forming their own.
+constexpr int kSqStride = kDirectThreads + 1;  // LDS row of step factors (+1 entry)
+
+template <int VB, bool SHQ = false>
 __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
     const double *__restrict__ packed, int64_t N, int V, int v0, const RunDesc *__restrict__ runs,
     int64_t n_runs, int64_t stride, int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S,
-    int write_final)
+    int write_final, const double *__restrict__ run_steps = nullptr, const int32_t *__restrict__ run_nsteps = nullptr)
 {
     __shared__ RunNode<VB> snode[kDirectThreads];
+    extern __shared__ float2 sq32[];  // SHQ: (step, node) factors, rows of kSqStride
+    const int nst = SHQ ? run_nsteps[blockIdx.x] : 0;
     const int64_t split = blockIdx.y;
     const int64_t n_begin = split * nodes_per_split;
     const int64_t n_end = (n_begin + nodes_per_split < N) ? n_begin + nodes_per_split : N;
@@ -409,6 +416,16 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
             nd.dwcf = (float)nd.dwc;
             nd.r2f = (float)nd.r2;
             snode[tid] = nd;
+            if constexpr (SHQ) {
+                for (int j = 0; j < nst; ++j) {
+                    const double *st = &run_steps[((int64_t)blockIdx.x * kRunSteps + j) * 4];
+                    const float sdkx = (float)st[0], sdky = (float)st[1], sdkz = (float)st[2], sdt = (float)st[3];
+                    float du = fmaf(sdkx, nd.xf, fmaf(sdky, nd.yf, fmaf(sdkz, nd.zf, -nd.dwcf * sdt)));
+                    du -= __builtin_rintf(du);
+                    const float ds = __builtin_amdgcn_exp2f(-sdt * nd.r2f * 1.44269504f);
+                    sq32[j * kSqStride + tid] = make_float2(ds * __builtin_amdgcn_cosf(du), -ds * __builtin_amdgcn_sinf(du));
+                }
+            }
         }
         __syncthreads();
         const int cnt = (n_end - base < kDirectThreads) ? (int)(n_end - base) : kDirectThreads;
@@ -428,11 +445,18 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
             float er = d0 * __builtin_amdgcn_cosf(r);
             float ei = -d0 * __builtin_amdgcn_sinf(r);
             // step factor
-            float du = fmaf(dkx, nd.xf, fmaf(dky, nd.yf, fmaf(dkz, nd.zf, -nd.dwcf * dtf)));
-            du -= __builtin_rintf(du);
-            const float ds = __builtin_amdgcn_exp2f(-dtf * nd.r2f * 1.44269504f);
-            const float qr = ds * __builtin_amdgcn_cosf(du);
-            const float qi = -ds * __builtin_amdgcn_sinf(du);
+            float qr, qi;
+            if constexpr (SHQ) {
+                const float2 q = sq32[rd.step * kSqStride + i];
+                qr = q.x;
+                qi = q.y;
+            } else {
+                float du = fmaf(dkx, nd.xf, fmaf(dky, nd.yf, fmaf(dkz, nd.zf, -nd.dwcf * dtf)));
+                du -= __builtin_rintf(du);
+                const float ds = __builtin_amdgcn_exp2f(-dtf * nd.r2f * 1.44269504f);
+                qr = ds * __builtin_amdgcn_cosf(du);
+                qi = -ds * __builtin_amdgcn_sinf(du);
+            }
             if constexpr (VB == 1) {
                 // y = e c advances by q: one complex multiply and one complex
                 // add per sample
@@ -492,13 +516,15 @@ struct alignas(16) RunNode64 {
     double c[2 * VB];
 };
 
-template <int VB>
+template <int VB, bool SHQ = false>
 __global__ __launch_bounds__(kDirectThreads) void k_signal_runs_f64(
     const double *__restrict__ packed, int64_t N, int V, int v0, const RunDesc *__restrict__ runs,
     int64_t n_runs, int64_t stride, int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S,
-    int write_final)
+    int write_final, const double *__restrict__ run_steps = nullptr, const int32_t *__restrict__ run_nsteps = nullptr)
 {
     __shared__ RunNode64<VB> snode[kDirectThreads];
+    extern __shared__ double2 sq64[];  // SHQ: (step, node) factors, rows of kSqStride
+    const int nst = SHQ ? run_nsteps[blockIdx.x] : 0;
     const int64_t split = blockIdx.y;
     const int64_t n_begin = split * nodes_per_split;
     const int64_t n_end = (n_begin + nodes_per_split < N) ? n_begin + nodes_per_split : N;
@@ -536,6 +562,17 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs_f64(
                 for (int v = 0; v < 2 * VB; ++v) nd.c[v] = 0.0;
             }
             snode[tid] = nd;
+            if constexpr (SHQ) {
+                for (int j = 0; j < nst; ++j) {
+                    const double *st = &run_steps[((int64_t)blockIdx.x * kRunSteps + j) * 4];
+                    double du = fma(st[0], nd.x, fma(st[1], nd.y, fma(st[2], nd.z, -nd.dwc * st[3])));
+                    du -= __builtin_rint(du);
+                    double qs, qc;
+                    sincospi(2.0 * du, &qs, &qc);
+                    const double ds = exp(-st[3] * nd.r2);
+                    sq64[j * kSqStride + tid] = make_double2(ds * qc, -ds * qs);
+                }
+            }
         }
         __syncthreads();
         const int cnt = (n_end - base < kDirectThreads) ? (int)(n_end - base) : kDirectThreads;
@@ -549,12 +586,20 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs_f64(
             const double d0 = exp(-rd.t0 * nd.r2);
             double er = d0 * cs, ei = -d0 * sn;
             // step factor
-            double du = fma(rd.dkx, nd.x, fma(rd.dky, nd.y, fma(rd.dkz, nd.z, -nd.dwc * rd.dt)));
-            du -= __builtin_rint(du);
-            double qs, qc;
-            sincospi(2.0 * du, &qs, &qc);
-            const double ds = exp(-rd.dt * nd.r2);
-            const double qr = ds * qc, qi = -ds * qs;
+            double qr, qi;
+            if constexpr (SHQ) {
+                const double2 q = sq64[rd.step * kSqStride + i];
+                qr = q.x;
+                qi = q.y;
+            } else {
+                double du = fma(rd.dkx, nd.x, fma(rd.dky, nd.y, fma(rd.dkz, nd.z, -nd.dwc * rd.dt)));
+                du -= __builtin_rint(du);
+                double qs, qc;
+                sincospi(2.0 * du, &qs, &qc);
+                const double ds = exp(-rd.dt * nd.r2);
+                qr = ds * qc;
+                qi = -ds * qs;
+            }
             if constexpr (VB == 1) {
                 // y = e c advances by q, as in k_signal_runs
                 double yr = fma(er, nd.c[0], -ei * nd.c[1]);
@@ -791,9 +836,18 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
         // keeps the kernel at ~130 VGPRs (3 waves per SIMD)
         for (int v0 = 0; v0 < V; ++v0) {
             if (precision == FM_PREC_F64)
-                k_signal_runs_f64<1><<<grid, kDirectThreads, 0, st>>>(packed, N, V, v0, p->d_runs, p->n_runs,
-                                                                      p->run_stride, L.nodes_per_split, dst, S,
-                                                                      write_final);
+                if (p->run_steps > 0)
+                    k_signal_runs_f64<1, true><<<grid, kDirectThreads, sizeof(double2) * kSqStride * p->run_steps, st>>>(
+                        packed, N, V, v0, p->d_runs, p->n_runs, p->run_stride, L.nodes_per_split, dst, S, write_final,
+                        p->d_run_steps, p->d_run_nsteps);
+                else
+                    k_signal_runs_f64<1><<<grid, kDirectThreads, 0, st>>>(packed, N, V, v0, p->d_runs, p->n_runs,
+                                                                          p->run_stride, L.nodes_per_split, dst, S,
+                                                                          write_final);
+            else if (p->run_steps > 0)
+                k_signal_runs<1, true><<<grid, kDirectThreads, sizeof(float2) * kSqStride * p->run_steps, st>>>(
+                    packed, N, V, v0, p->d_runs, p->n_runs, p->run_stride, L.nodes_per_split, dst, S, write_final,
+                    p->d_run_steps, p->d_run_nsteps);
             else
                 k_signal_runs<1><<<grid, kDirectThreads, 0, st>>>(packed, N, V, v0, p->d_runs, p->n_runs,
                                                                   p->run_stride, L.nodes_per_split, dst, S,
@@ -844,6 +898,8 @@ struct PlanHost {
     std::vector<SignalTile> tiles_g, tiles_c, tiles_p;
     std::vector<double4> ks;
     std::vector<RunDesc> runs;
+    std::vector<double> run_steps;    // (tiles, kRunSteps, 4)
+    std::vector<int32_t> run_nsteps;  // (tiles)
 };
 
 // Host analysis of a trajectory: distinct-t groups, Cartesian
@@ -966,16 +1022,16 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
         double devk = 0.0, devt = 0.0;  // largest departure from the linear model
         runs.reserve((size_t)(stride * cdiv(n_meas, kRunLen)));
         for (int64_t line = 0; line < stride && linear; ++line) {
+            // one slope per line (first to last sample), so that the runs of
+            // a line share their step factor
+            const double4 f = ks[(size_t)line], l = ks[(size_t)((n_meas - 1) * stride + line)];
+            const double inv = 1.0 / (double)(n_meas - 1);
+            const double4 d = make_double4((l.x - f.x) * inv, (l.y - f.y) * inv, (l.z - f.z) * inv, (l.w - f.w) * inv);
             for (int64_t i0 = 0; i0 < n_meas && linear; i0 += kRunLen) {
                 const int64_t len = std::min<int64_t>(kRunLen, n_meas - i0);
                 const int64_t s0 = i0 * stride + line;
                 const double4 a = ks[(size_t)s0];
-                double4 d = make_double4(0.0, 0.0, 0.0, 0.0);
-                if (len > 1) {
-                    const double4 b = ks[(size_t)(s0 + stride)];
-                    d = make_double4(b.x - a.x, b.y - a.y, b.z - a.z, b.w - a.w);
-                }
-                for (int64_t m = 2; m < len && linear; ++m) {
+                for (int64_t m = 1; m < len && linear; ++m) {
                     const double4 c = ks[(size_t)(s0 + m * stride)];
                     const double ek = std::max({std::fabs(c.x - a.x - m * d.x), std::fabs(c.y - a.y - m * d.y),
                                                 std::fabs(c.z - a.z - m * d.z)});
@@ -1002,6 +1058,37 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
             p->n_runs = (int64_t)runs.size();
             p->run_stride = stride;
             p->runs_exact = devk <= 1e-2 * tolk && devt <= 1e-2 * tolt;
+            // the distinct steps of each 256-run tile (bitwise)
+            const int64_t n_tiles = cdiv(p->n_runs, kDirectThreads);
+            h.run_steps.assign((size_t)(n_tiles * kRunSteps * 4), 0.0);
+            h.run_nsteps.assign((size_t)n_tiles, 0);
+            int most = 0;
+            for (int64_t b = 0; b < n_tiles && most <= kRunSteps; ++b) {
+                int cnt = 0;
+                for (int64_t r = b * kDirectThreads; r < std::min<int64_t>(p->n_runs, (b + 1) * kDirectThreads); ++r) {
+                    RunDesc &rd = runs[(size_t)r];
+                    const double v[4] = {rd.dkx, rd.dky, rd.dkz, rd.dt};
+                    int j = 0;
+                    for (; j < cnt; ++j)
+                        if (std::memcmp(&h.run_steps[(size_t)((b * kRunSteps + j) * 4)], v, sizeof v) == 0) break;
+                    if (j == cnt) {
+                        if (cnt == kRunSteps) {
+                            most = kRunSteps + 1;
+                            break;
+                        }
+                        std::memcpy(&h.run_steps[(size_t)((b * kRunSteps + j) * 4)], v, sizeof v);
+                        ++cnt;
+                    }
+                    rd.step = j;
+                }
+                h.run_nsteps[(size_t)b] = cnt;
+                most = std::max(most, cnt);
+            }
+            p->run_steps = most <= kRunSteps ? most : 0;
+            if (p->run_steps == 0) {
+                h.run_steps.clear();
+                h.run_nsteps.clear();
+            }
         } else {
             runs.clear();
         }
@@ -1074,6 +1161,12 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         if (p->n_runs > 0) {
             if ((e = hipMalloc(&p->d_runs, sizeof(RunDesc) * runs.size())) != hipSuccess) return fail(e, "hipMalloc");
             if ((e = hipMemcpy(p->d_runs, runs.data(), sizeof(RunDesc) * runs.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        }
+        if (p->run_steps > 0) {
+            if ((e = hipMalloc(&p->d_run_steps, sizeof(double) * h.run_steps.size())) != hipSuccess) return fail(e, "hipMalloc");
+            if ((e = hipMemcpy(p->d_run_steps, h.run_steps.data(), sizeof(double) * h.run_steps.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+            if ((e = hipMalloc(&p->d_run_nsteps, sizeof(int32_t) * h.run_nsteps.size())) != hipSuccess) return fail(e, "hipMalloc");
+            if ((e = hipMemcpy(p->d_run_nsteps, h.run_nsteps.data(), sizeof(int32_t) * h.run_nsteps.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         }
         if (p->method == FM_METHOD_CARTESIAN) {
             const int rc = cartesian_upload(p);
@@ -1169,6 +1262,8 @@ extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
     (void)hipFree(p->d_perm);
     (void)hipFree(p->d_tiles_grouped);
     (void)hipFree(p->d_runs);
+    (void)hipFree(p->d_run_steps);
+    (void)hipFree(p->d_run_nsteps);
     (void)hipFree(p->d_tiles_consec);
     (void)hipFree(p->d_tiles_packed);
     (void)hipFree(p->d_tg_t);

@@ -25,11 +25,14 @@ constexpr int kRunLen = 16;
 
 struct RunDesc {
     double k0x, k0y, k0z, t0;  // first sample (k' in cycles/m, t in ms)
-    double dkx, dky, dkz, dt;  // step per sample
+    double dkx, dky, dkz, dt;  // step per sample (the line's slope)
     int32_t s0;                // first sample index; sample m is s0 + m * run_stride
     int32_t len;               // samples in the run (<= kRunLen)
-    int32_t pad[2];
+    int32_t step;              // index of (dk, dt) in its tile's step table
+    int32_t pad;
 };
+// distinct (dk, dt) steps a 256-run tile may share through LDS
+constexpr int kRunSteps = 8;
 
 struct SignalTile {
     int32_t group;   // t-group index (shared-t kernels) or -1
@@ -83,6 +86,12 @@ struct fm_signal_plan {
     // 1e-12 of run detection): the FP64 form may then take the runs too
     bool runs_exact = false;
     fm::RunDesc *d_runs = nullptr;
+    // per 256-run tile, its distinct steps (kRunSteps x (dkx, dky, dkz, dt))
+    // and their count; run_steps = the largest count, 0 when some tile has
+    // more than kRunSteps (each lane then forms its own step factor)
+    int run_steps = 0;
+    double *d_run_steps = nullptr;
+    int32_t *d_run_nsteps = nullptr;
     fm::CartesianTables cart;
     // optional timing of the dominant kernel (fm_signal_plan_set_timing)
     std::vector<hipEvent_t> ev_start, ev_stop;
