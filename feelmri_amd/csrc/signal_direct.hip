@@ -207,11 +207,23 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                             for (int v = 0; v < 2 * VB; ++v) nd.c[j][v] = nd.c[0][v];
                             continue;
                         }
-                        // c = b exp(-t/T2) exp(i phi t)   (MRI.cpp:347-354)
-                        const double d = exp(-tg[j] * r2);
-                        double so, co;
-                        sincos(phi * tg[j], &so, &co);
-                        const double fr = d * co, fi = d * so;
+                        // c = b exp(-t/T2) exp(i phi t)   (MRI.cpp:347-354); the
+                        // mixed form reduces phi t / 2 pi in FP64 and takes the
+                        // FP32 hardware sine, cosine and exp2
+                        double fr, fi;
+                        if constexpr (MIXED) {
+                            const double u = phi * tg[j] * (1.0 / kTwoPi);
+                            const float r = (float)(u - __builtin_rint(u));
+                            const float d = __builtin_amdgcn_exp2f((float)(-tg[j] * r2 * 1.4426950408889634));
+                            fr = (double)(d * __builtin_amdgcn_cosf(r));
+                            fi = (double)(d * __builtin_amdgcn_sinf(r));
+                        } else {
+                            const double d = exp(-tg[j] * r2);
+                            double so, co;
+                            sincos(phi * tg[j], &so, &co);
+                            fr = d * co;
+                            fi = d * so;
+                        }
 #pragma unroll
                         for (int v = 0; v < VB; ++v) {
                             const double br = pb[(int64_t)(2 * (v0 + v)) * N + n];
