@@ -46,10 +46,14 @@ def _run(kloc, t, n, v, seed, method="cartesian"):
 # Column tiles of 128 (lines x kz x encodings): the tiles whose distinct Ey
 # and Ezb rows fit the LDS staging take the staged kernel, the others (many
 # lines per tile with one kz, or more than 128 kz x encodings) the per-lane
-# kernel; both kinds are covered below.
+# kernel; both kinds are covered below.  Shapes with 97 or more readout rows
+# use 4 row blocks per wave: (128, 9, 1), (100, 20, 6) (ragged rows and
+# columns), (256, 10, 4) (two row tiles) on the staged kernel, (128, 200, 1)
+# on the per-lane kernel.
 @pytest.mark.parametrize("res,n,v", [((16, 8, 2), 3000, 1), ((40, 12, 3), 2049, 2), ((128, 9, 1), 700, 3),
                                      ((8, 6, 1), 1, 1), ((33, 17, 5), 5000, 1), ((32, 16, 32), 4000, 1),
-                                     ((16, 200, 1), 3000, 1), ((12, 3, 50), 2000, 3)])
+                                     ((16, 200, 1), 3000, 1), ((12, 3, 50), 2000, 3), ((100, 20, 6), 5000, 2),
+                                     ((256, 10, 4), 3000, 1), ((128, 200, 1), 2000, 1)])
 def test_gemm_shapes(gpu, res, n, v):
     kloc, t = trajectories.cartesian((0.22, 0.22, 0.22), res, t_first=1.0, t_readout=3.0)
     _run(kloc, t, n, v, seed=sum(res) + n)
