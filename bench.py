@@ -511,14 +511,18 @@ def row_benchmarks(wl, dev) -> dict:
     nodes_d = torch.from_numpy(m.nodes).to(dev)
     out = torch.empty(N, dtype=torch.float64, device=dev)
     ms = _time_cuda(lambda: device.lumped_mass(col, nodes_d, out=out), 5)
+    ms_coloured = _time_cuda(lambda: device.lumped_mass_coloured(col, nodes_d, out=out), 5)
     E = m.n_elems
-    by = E * (16 + 4 + 4 * 24) + N * 16          # connectivity, permutation, gathered coordinates; weights
+    by = E * (16 + 16 + 64) + N * (24 + 4 + 8)  # connectivity, slot indices, slot values out+in; coords, degree, w
     rows["lumped_mass_p1"] = {"elements_per_s": E / (ms * 1e-3), "ms": ms, "elements": E, "colours": col.n_colours,
+                              "coloured_scatter_ms": ms_coloured,
                               "roofline": {"bound": "hbm", "achieved": by / (ms * 1e-3) / 1e9,
                                            "peak": PEAKS["hbm_gbs"], "unit": "GB/s",
                                            "frac": by / (ms * 1e-3) / 1e9 / PEAKS["hbm_gbs"],
-                                           "per_unit": "116 B per element (16 B nodes, 4 B colour permutation, "
-                                                       "4 x 24 B coordinates) + 16 B per node; one launch per colour"}}
+                                           "per_unit": "96 B per element (16 B nodes, 16 B slot indices, 4 x 8 B "
+                                                       "|det J| written and read back) + 36 B per node (24 B "
+                                                       "coordinates, 4 B degree, 8 B weight); two launches: "
+                                                       "per-element slot writes, per-node gather in colour order"}}
     # POD contraction at 2 M nodes, 10 modes (float32)
     P, R = 2_000_000, 10
     modes = torch.randn((P, 3, R), dtype=torch.float32, device=dev)

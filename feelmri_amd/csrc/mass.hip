@@ -11,6 +11,16 @@
 // weights of its four nodes with plain loads and stores.  The order in which
 // a node receives its contributions (colour order) is fixed, so results are
 // reproducible.
+//
+// The gather form does the same sums in two launches over a host-built
+// incidence in sliced-ELL layout: node n's j-th incident element in colour
+// order owns slot chunk_offsets[n/64] + 64 j + n%64.  Launch 1, per element
+// in element order (coalesced connectivity, local coordinate reads), writes
+// |det J_e| = 6 |T_e| into its four vertex slots (each slot has one writer).
+// Launch 2, per node, adds its deg(n) slots: a wave reads 64 consecutive
+// values per step.  A node has at most one element per colour, so it
+// performs the scatter's additions in the scatter's order; both forms add
+// with an explicit fma(|det J_e|, 1/24, w), so they give the same bits.
 #include <algorithm>
 #include <vector>
 
@@ -49,9 +59,38 @@ __global__ void k_lumped_colour(int64_t begin, int64_t end, const int32_t *__res
     if (k >= end) return;
     const int64_t e = perm[k];
     int32_t ids[4];
-    const double q = tet_abs_det<T>(elems + 4 * e, nodes, ids) * (1.0 / 24.0);  // |T|/4
+    const double det = tet_abs_det<T>(elems + 4 * e, nodes, ids);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) w[ids[i]] += q;
+    for (int i = 0; i < 4; ++i) w[ids[i]] = fma(det, 1.0 / 24.0, w[ids[i]]);  // += |T|/4
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_abs_dets(int64_t E, const int32_t *__restrict__ elems,
+                                                  const T *__restrict__ nodes, const int32_t *__restrict__ vertex_slots,
+                                                  double *__restrict__ slot_vals)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    int32_t ids[4];
+    const double det = tet_abs_det<T>(elems + 4 * e, nodes, ids);
+    const int4 sl = reinterpret_cast<const int4 *>(vertex_slots)[e];
+    slot_vals[sl.x] = det;
+    slot_vals[sl.y] = det;
+    slot_vals[sl.z] = det;
+    slot_vals[sl.w] = det;
+}
+
+__global__ __launch_bounds__(256) void k_lumped_gather(int64_t n_nodes, const int64_t *__restrict__ chunk_offsets,
+                                                       const int32_t *__restrict__ degree,
+                                                       const double *__restrict__ slot_vals, double *__restrict__ w)
+{
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const double *v = slot_vals + chunk_offsets[n / 64] + n % 64;
+    const int d = degree[n];
+    double acc = 0.0;
+    for (int j = 0; j < d; ++j) acc = fma(v[64 * j], 1.0 / 24.0, acc);  // += |T|/4
+    w[n] = acc;
 }
 
 template <typename T>
@@ -147,6 +186,82 @@ extern "C" int fm_mass_p1_lumped(int64_t n_elems, int64_t n_nodes, const int32_t
             k_lumped_colour<float><<<blocks, 256, 0, st>>>(b, e, perm, elems, (const float *)nodes, w);
         FM_LAUNCH_CHECK("k_lumped_colour");
     }
+    return FM_OK;
+}
+
+extern "C" int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int32_t *elems, const int32_t *perm,
+                                    int64_t *chunk_offsets, int32_t *degree, int32_t *vertex_slots)
+{
+    FM_CHECK_ARG(n_elems >= 0 && n_nodes >= 0, "negative size");
+    FM_CHECK_ARG(chunk_offsets && (n_nodes == 0 || degree) && (n_elems == 0 || (elems && perm && vertex_slots)),
+                 "NULL argument");
+    for (int64_t n = 0; n < n_nodes; ++n) degree[n] = 0;
+    for (int64_t e = 0; e < n_elems; ++e)
+        for (int i = 0; i < 4; ++i) {
+            const int64_t n = elems[4 * e + i];
+            if (n < 0 || n >= n_nodes) {
+                set_error("element %lld references node %lld outside [0, %lld)", (long long)e, (long long)n,
+                          (long long)n_nodes);
+                return FM_ERR_ARG;
+            }
+            degree[n]++;
+        }
+    const int64_t n_chunks = cdiv(n_nodes, 64);
+    chunk_offsets[0] = 0;
+    for (int64_t c = 0; c < n_chunks; ++c) {
+        int32_t wmax = 0;
+        for (int64_t n = 64 * c; n < std::min<int64_t>(n_nodes, 64 * c + 64); ++n) wmax = std::max(wmax, degree[n]);
+        chunk_offsets[c + 1] = chunk_offsets[c] + 64 * (int64_t)wmax;
+    }
+    if (chunk_offsets[n_chunks] > 0x7fffffffLL) {
+        set_error("incidence needs %lld slots, more than int32 slot indices address",
+                  (long long)chunk_offsets[n_chunks]);
+        return FM_ERR_UNSUPPORTED;
+    }
+    // slots in perm (colour) order: node n's j-th element takes slot j
+    std::vector<int32_t> next((size_t)n_nodes, 0);
+    std::vector<uint8_t> seen((size_t)n_elems, 0);
+    for (int64_t k = 0; k < n_elems; ++k) {
+        const int64_t e = perm[k];
+        if (e < 0 || e >= n_elems || seen[(size_t)e]) {
+            set_error("perm is not a permutation of [0, %lld) (entry %lld)", (long long)n_elems, (long long)k);
+            return FM_ERR_ARG;
+        }
+        seen[(size_t)e] = 1;
+        for (int i = 0; i < 4; ++i) {
+            const int64_t n = elems[4 * e + i];
+            vertex_slots[4 * e + i] = (int32_t)(chunk_offsets[n / 64] + 64 * (int64_t)next[(size_t)n]++ + n % 64);
+        }
+    }
+    return FM_OK;
+}
+
+extern "C" int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const int32_t *elems, int dtype,
+                                        const void *nodes, const int64_t *chunk_offsets, const int32_t *degree,
+                                        const int32_t *vertex_slots, int64_t n_slots, double *slot_scratch,
+                                        double *w, void *stream)
+{
+    FM_CHECK_ARG(n_elems >= 0 && n_nodes >= 0 && n_slots >= 0, "negative size");
+    FM_CHECK_ARG(dtype == FM_DTYPE_F32 || dtype == FM_DTYPE_F64, "unknown dtype %d", dtype);
+    FM_CHECK_ARG(n_slots >= 4 * n_elems, "n_slots (%lld) below 4 n_elems", (long long)n_slots);
+    if (n_nodes == 0) return FM_OK;
+    FM_CHECK_ARG(w && chunk_offsets && degree, "NULL array");
+    hipStream_t st = as_stream(stream);
+    if (n_elems > 0) {
+        FM_CHECK_ARG(elems && nodes && vertex_slots && slot_scratch, "NULL array");
+        FM_CHECK_ARG(((uintptr_t)vertex_slots & 15) == 0, "vertex_slots must be 16-byte aligned");
+        FM_CHECK_ARG(n_elems <= 0x7fffffffLL * 256, "too many elements for one launch");
+        const unsigned eblocks = (unsigned)cdiv(n_elems, 256);
+        if (dtype == FM_DTYPE_F64)
+            k_abs_dets<double><<<eblocks, 256, 0, st>>>(n_elems, elems, (const double *)nodes, vertex_slots,
+                                                        slot_scratch);
+        else
+            k_abs_dets<float><<<eblocks, 256, 0, st>>>(n_elems, elems, (const float *)nodes, vertex_slots,
+                                                       slot_scratch);
+        FM_LAUNCH_CHECK("k_abs_dets");
+    }
+    k_lumped_gather<<<(unsigned)cdiv(n_nodes, 256), 256, 0, st>>>(n_nodes, chunk_offsets, degree, slot_scratch, w);
+    FM_LAUNCH_CHECK("k_lumped_gather");
     return FM_OK;
 }
 

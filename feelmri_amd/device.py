@@ -363,7 +363,8 @@ def bloch_store_spoil(mxy_state, mz_state, mxy_store=None, mz_store=None, col: i
 # P1 mass
 # ---------------------------------------------------------------------------
 class P1Colouring:
-    """Host greedy colouring of a tet mesh, resident on the device."""
+    """Host greedy colouring of a tet mesh and each node's incident elements
+    in colour order, resident on the device."""
 
     def __init__(self, elems, n_nodes: int, device=None):
         device = torch.device(device) if device is not None else default_device()
@@ -380,19 +381,49 @@ class P1Colouring:
         self.n_colours = nc.value
         self.offsets = np.ascontiguousarray(offs[: self.n_colours + 1])
         self.n_elems, self.n_nodes = E, int(n_nodes)
+        chunk_offsets = np.empty((self.n_nodes + 63) // 64 + 1, dtype=np.int64)
+        degree = np.empty(self.n_nodes, dtype=np.int32)
+        vertex_slots = np.empty(4 * E, dtype=np.int32)
+        _lib.call("fm_mass_p1_incidence", E, self.n_nodes, el.ctypes.data_as(ctypes.c_void_p),
+                  perm.ctypes.data_as(ctypes.c_void_p), chunk_offsets.ctypes.data_as(ctypes.c_void_p),
+                  degree.ctypes.data_as(ctypes.c_void_p), vertex_slots.ctypes.data_as(ctypes.c_void_p))
+        self.n_slots = int(chunk_offsets[-1])
         self.elems = torch.from_numpy(el).to(device)
         self.perm = torch.from_numpy(perm).to(device)
+        self.chunk_offsets = torch.from_numpy(chunk_offsets).to(device)
+        self.degree = torch.from_numpy(degree).to(device)
+        self.vertex_slots = torch.from_numpy(vertex_slots).to(device)
         self.device = device
 
 
-def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None) -> torch.Tensor:
-    """w_i = sum_{e ni i} |T_e| / 4 (float64, (N,))."""
+def _mass_args(colouring: P1Colouring, nodes: torch.Tensor, out):
     nodes = nodes.contiguous()
     if nodes.dtype not in (torch.float32, torch.float64) or nodes.shape != (colouring.n_nodes, 3):
         raise ValueError("nodes must be float32/float64 of shape (n_nodes, 3)")
     if out is None:
         out = torch.empty(colouring.n_nodes, dtype=torch.float64, device=nodes.device)
     dt_code = _lib.DTYPE_F64 if nodes.dtype == torch.float64 else _lib.DTYPE_F32
+    return nodes, out, dt_code
+
+
+def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None) -> torch.Tensor:
+    """w_i = sum_{e ni i} |T_e| / 4 (float64, (N,)): each element writes
+    |det J_e| into its vertices' incidence slots, then each node adds its
+    slots in colour order."""
+    nodes, out, dt_code = _mass_args(colouring, nodes, out)
+    q = torch.empty(max(colouring.n_slots, 1), dtype=torch.float64, device=nodes.device)
+    with torch.cuda.device(nodes.device):
+        _lib.call("fm_mass_p1_lumped_gather", colouring.n_elems, colouring.n_nodes, ptr(colouring.elems),
+                  dt_code, ptr(nodes), ptr(colouring.chunk_offsets), ptr(colouring.degree),
+                  ptr(colouring.vertex_slots), colouring.n_slots, ptr(q), ptr(out), stream_handle(stream))
+    if isinstance(stream, torch.cuda.Stream):
+        q.record_stream(stream)  # the scratch is freed here but used on that stream
+    return out
+
+
+def lumped_mass_coloured(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None) -> torch.Tensor:
+    """The same weights by the coloured scatter: one launch per colour."""
+    nodes, out, dt_code = _mass_args(colouring, nodes, out)
     with torch.cuda.device(nodes.device):
         _lib.call("fm_mass_p1_lumped", colouring.n_elems, colouring.n_nodes, ptr(colouring.elems),
                   dt_code, ptr(nodes), ptr(colouring.perm),

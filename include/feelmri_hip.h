@@ -220,6 +220,27 @@ int fm_mass_p1_lumped(int64_t n_elems, int64_t n_nodes, const int32_t *elems, in
                       const void *nodes, const int32_t *perm, const int64_t *color_offsets,
                       int n_colors, double *w, void *stream);
 
+/* Host: each node's incident elements in the order of perm (colour order),
+ * as slots of a sliced-ELL table: node n's j-th element owns slot
+ * chunk_offsets[n / 64] + 64 j + n % 64.  Outputs (host): chunk_offsets
+ * (ceil(n_nodes / 64) + 1; the last entry is the slot count), degree
+ * (n_nodes), vertex_slots (4 n_elems: the slot of element e's vertex i).
+ * elems and perm as for fm_mass_p1_color (host). */
+int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int32_t *elems, const int32_t *perm,
+                         int64_t *chunk_offsets, int32_t *degree, int32_t *vertex_slots);
+
+/* Lumped weights w_i = sum_{e ni i} |T_e| / 4 in two launches: each element
+ * writes |det J_e| = 6 |T_e| into its four slots of slot_scratch, then node
+ * i adds its degree[i] slots -- the additions of fm_mass_p1_lumped, in the
+ * same order and with the same rounding, so the result is bitwise the same.
+ * chunk_offsets, degree, vertex_slots (16-byte aligned) from
+ * fm_mass_p1_incidence; n_slots = the last chunk offset; slot_scratch
+ * (n_slots float64, overwritten); elems, nodes, w: all device. */
+int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const int32_t *elems, int dtype,
+                             const void *nodes, const int64_t *chunk_offsets, const int32_t *degree,
+                             const int32_t *vertex_slots, int64_t n_slots, double *slot_scratch,
+                             double *w, void *stream);
+
 /* Per-element 4x4 consistent mass matrices |T_e|/20 (1 + delta_ij),
  * (n_elems, 16) float64 -- the triplet values of Assemble.cpp:148-150. */
 int fm_mass_p1_element_matrices(int64_t n_elems, const int32_t *elems, int dtype,

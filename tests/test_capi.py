@@ -111,6 +111,52 @@ def test_colouring_is_a_valid_partition():
         assert len(ids) == len(set(ids.tolist())), "two elements of one colour share a node"
 
 
+def test_incidence_lists_each_nodes_elements_in_colour_order():
+    """The host incidence slots behind the gather lumped mass: decoded, node
+    n's slots hold exactly the elements containing n, in the order they
+    appear in the colour permutation (brute force on a small mesh)."""
+    import numpy as np
+
+    from feelmri_amd import _lib, phantoms
+
+    m = phantoms.box_mesh((0, 0, 0), (1, 1, 1), (5, 4, 3))
+    E, N = m.n_elems, m.n_nodes + 1  # one node no element references
+    el = np.ascontiguousarray(m.elems)
+    perm = np.empty(E, np.int32)
+    offs = np.zeros(65, np.int64)
+    nc = ctypes.c_int()
+    _lib.call("fm_mass_p1_color", E, N, el.ctypes.data_as(ctypes.c_void_p), perm.ctypes.data_as(ctypes.c_void_p),
+              offs.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nc))
+    chunk_offsets = np.empty((N + 63) // 64 + 1, np.int64)
+    degree = np.empty(N, np.int32)
+    slots = np.empty(4 * E, np.int32)
+    args = lambda p: (E, N, el.ctypes.data_as(ctypes.c_void_p), p.ctypes.data_as(ctypes.c_void_p),
+                      chunk_offsets.ctypes.data_as(ctypes.c_void_p), degree.ctypes.data_as(ctypes.c_void_p),
+                      slots.ctypes.data_as(ctypes.c_void_p))
+    _lib.call("fm_mass_p1_incidence", *args(perm))
+    # decode every slot back to (node, j) and rebuild each node's list
+    lists = {n: {} for n in range(N)}
+    for e in range(E):
+        for i in range(4):
+            s_ = int(slots[4 * e + i])
+            c = int(np.searchsorted(chunk_offsets, s_, side="right")) - 1
+            j, lane = divmod(s_ - int(chunk_offsets[c]), 64)
+            assert 64 * c + lane == el[e, i], (e, i)
+            assert j not in lists[64 * c + lane]
+            lists[64 * c + lane][j] = e
+    for n in range(N):
+        want = [int(e) for e in perm if n in el[e]]
+        assert degree[n] == len(want)
+        assert [lists[n][j] for j in range(degree[n])] == want, n
+        c = n // 64
+        assert degree[n] <= (chunk_offsets[c + 1] - chunk_offsets[c]) // 64
+    assert degree[N - 1] == 0  # the unreferenced node
+    bad = perm.copy()
+    bad[0] = bad[1]
+    with pytest.raises(RuntimeError, match="permutation"):
+        _lib.call("fm_mass_p1_incidence", *args(bad))
+
+
 def _declared_arity():
     """name -> number of parameters, from the prototypes in include/*.h."""
     out = {}

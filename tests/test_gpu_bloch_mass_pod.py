@@ -282,6 +282,54 @@ def test_lumped_mass_c0(gpu):
     np.testing.assert_allclose(w.sum(), 1e-4, rtol=1e-12)
 
 
+@pytest.mark.parametrize("mesh,dtype", [("slab", np.float64), ("slab", np.float32), ("ellipsoid", np.float64),
+                                        ("box_spare_node", np.float64)])
+def test_lumped_gather_equals_coloured_scatter(gpu, mesh, dtype):
+    """The gather does the coloured scatter's additions in the same order
+    and rounding: bitwise the same weights, and the oracle's to 1e-12.  The
+    box mesh gets one node no element references (weight 0)."""
+    import torch
+
+    import oracle
+    from feelmri_amd import device
+
+    if mesh == "slab":
+        m = phantoms.slab_mesh()
+        nodes = m.nodes
+    elif mesh == "ellipsoid":
+        m = phantoms.ellipsoid_mesh(n=14)
+        nodes = m.nodes
+    else:
+        m = phantoms.box_mesh((0, 0, 0), (0.02, 0.02, 0.01), (9, 8, 5))
+        nodes = np.vstack([m.nodes, [[0.5, 0.5, 0.5]]])
+    nodes = nodes.astype(dtype)
+    col = device.P1Colouring(m.elems, nodes.shape[0], device=gpu)
+    nd = torch.from_numpy(nodes).to(gpu)
+    g = device.lumped_mass(col, nd)
+    c = device.lumped_mass_coloured(col, nd)
+    torch.cuda.synchronize()
+    assert torch.equal(g, c)
+    ref = oracle.mass_p1_lumped(m.elems, nodes.astype(np.float64))
+    w = g.cpu().numpy()
+    assert ref.shape == w.shape or mesh == "box_spare_node"
+    if mesh == "box_spare_node":
+        assert w[-1] == 0.0
+        w = w[:-1]
+        ref = ref[: w.size]
+    assert np.max(np.abs(w - ref) / ref) <= 1e-12
+
+
+def test_lumped_gather_without_elements(gpu):
+    import torch
+
+    from feelmri_amd import device
+
+    col = device.P1Colouring(np.zeros((0, 4), np.int32), 7, device=gpu)
+    w = device.lumped_mass(col, torch.ones((7, 3), dtype=torch.float64, device=gpu))
+    torch.cuda.synchronize()
+    assert torch.equal(w.cpu(), torch.zeros(7, dtype=torch.float64))
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_consistent_mass_assembly(gpu, dtype):
     from feelmri_amd import Assemble
