@@ -283,11 +283,13 @@ def test_lumped_mass_c0(gpu):
 
 
 @pytest.mark.parametrize("mesh,dtype", [("slab", np.float64), ("slab", np.float32), ("ellipsoid", np.float64),
-                                        ("box_spare_node", np.float64)])
+                                        ("ellipsoid_shuffled", np.float64), ("box_spare_node", np.float64)])
 def test_lumped_gather_equals_coloured_scatter(gpu, mesh, dtype):
     """The gather does the coloured scatter's additions in the same order
     and rounding: bitwise the same weights, and the oracle's to 1e-12.  The
-    box mesh gets one node no element references (weight 0)."""
+    box mesh gets one node no element references (weight 0); the shuffled
+    ellipsoid lists its elements in random order (another colouring, slots
+    scattered over the whole table)."""
     import torch
 
     import oracle
@@ -296,9 +298,11 @@ def test_lumped_gather_equals_coloured_scatter(gpu, mesh, dtype):
     if mesh == "slab":
         m = phantoms.slab_mesh()
         nodes = m.nodes
-    elif mesh == "ellipsoid":
+    elif mesh.startswith("ellipsoid"):
         m = phantoms.ellipsoid_mesh(n=14)
         nodes = m.nodes
+        if mesh.endswith("shuffled"):
+            m.elems = m.elems[np.random.default_rng(5).permutation(m.n_elems)]
     else:
         m = phantoms.box_mesh((0, 0, 0), (0.02, 0.02, 0.01), (9, 8, 5))
         nodes = np.vstack([m.nodes, [[0.5, 0.5, 0.5]]])
