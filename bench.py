@@ -2,27 +2,37 @@
 """Signal-generation throughput on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mixed|f64]
-                    [--method auto|direct|cartesian] [--config c1|c0|c2-radial|c2-spiral]
+                    [--config c3|c1|c0|c2-radial|c2-spiral|c4] [--method auto|direct|cartesian]
 
 A step is one evaluation of the k-space signal of the workload (all
-samples, all encodings) from node data already resident in HBM.  The
-default workload is c1: a 512 000-node P1 box phantom with spherical
-inclusions, 3-D Cartesian 128 x 128 x 32, 3 ms readout, after a spoiled-GRE
-Bloch solve (15 deg, TR 5 ms, 16 dummy TRs) and lumped-mass weighting,
-all run on the GPU during set-up.
+samples, all encodings, all cardiac phases) from node data already resident
+in HBM.  The default workload is c3, the largest single-GPU configuration:
+4-D phase-contrast flow on a 1 948 800-node P1 curved tube with 10 POD
+displacement modes, 4 velocity encodings and 20 cardiac phases of a
+96 x 96 x 24 Cartesian k-space, the nodes moving with every readout.  Set-up
+on the GPU: lumped-mass weights and the whole phase-contrast acquisition
+through the Bloch integrator with the nodes moving along a Poiseuille
+velocity field.
 
 Prints one JSON line: node*k-sample/s of the chosen form, its relative L2
-error against the FP64 GPU form over the full k-space and against the FP64
-oracle on a sample of k-space points, the roofline of the dominant kernel
-(timed live with HIP events on the launch stream) and the FP64 CPU oracle
-timed on the host on a bounded sample of the same workload.
+error against the FP64 GPU form over the full k-space, against the FP64
+oracle on sampled k-points and against the CPU baseline's readouts, the
+roofline of the dominant kernel (timed live with HIP events on the launch
+stream, traffic from committed hardware counters), and the FP64 CPU oracle
+run the reference's way (forked ranks over an element partition, partial
+lumped weights, a final sum) on the host on a bounded sample of readouts.
+The c1 3-D Cartesian workload follows as a secondary block, then the other
+rows of the path (Bloch, lumped mass, POD) and a serving stream.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -36,6 +46,8 @@ PEAKS = {  # MI355X published dense peaks
     "fp32_valu_tflops": 157.3,
     "hbm_gbs": 8000.0,
 }
+CPU_RANKS = 16   # the GPU box's host share for one GPU
+METRIC = "node*k-sample/s at relL2<=1e-4 vs FP64 oracle"
 
 
 def log(msg: str) -> None:
@@ -63,7 +75,6 @@ def build_workload(name: str, scale: float):
 
 def prepare_nodes(wl, dev):
     """Lumped weights, spoiled-GRE steady state, packed signal nodes (GPU)."""
-    import numpy as np
     import torch
 
     from feelmri_amd import device
@@ -85,8 +96,6 @@ def prepare_nodes(wl, dev):
 
 def time_form(plan, nodes, form, steps, warmup):
     import torch
-
-    from feelmri_amd import device
 
     out = torch.empty((plan.n_samples, nodes.n_enc), dtype=torch.complex128, device=plan.device)
     ws = torch.empty(max(plan.workspace_bytes(form, nodes.n_nodes, nodes.n_enc), 1), dtype=torch.uint8,
@@ -140,8 +149,8 @@ def time_host_io(plan, wl, w_h, mxy_h, form, steps):
 def time_shim_per_line(wl, w_h, mxy_h, form, reps: int = 3):
     """The reference examples' call pattern: one ``MRI.Signal`` call per
     phase-encoding line from host arrays (free_running.py:203-209), through
-    the drop-in shim (plans cached across calls).  Returns ms per full
-    k-space."""
+    the drop-in shim (plans and packed nodes cached across calls).  Returns
+    ms per full k-space."""
     import numpy as np
     import scipy.sparse as sp
 
@@ -243,34 +252,49 @@ def load_traffic(kernel: str, config: str, form: str):
     return None if rec is None else rec.get("bytes_per_launch")
 
 
-def cpu_baseline(wl, w, mxy_host, target_s: float):
-    """FP64 oracle (fast-math build, OpenMP) on a subset of readout indices."""
+def cpu_baseline(wl, w_h, mxy_h, target_s: float, motion=None):
+    """The FP64 oracle run the reference's way (``oracle.cpu_ranks``): forked
+    ranks, one per host core, each owning one block of elements and the
+    nodes they touch, with partial lumped weights, summed at the end.  Runs
+    as a child process on evenly spaced readout indices (all lines and kz
+    rows of each), sized to about ``target_s`` seconds.
+
+    ``motion``: (modes (N,3,R) float32, coef_t (G,), coef (G,R+3)) for
+    moving nodes.  Returns (baseline dict, readout indices, reduced k-space
+    (n_readouts * lines * kz, V))."""
     import numpy as np
 
-    import oracle
-
-    threads = min(16, os.cpu_count() or 1)
-    kx, ky, kz = (np.asarray(k) for k in wl.kloc)
-    t = np.asarray(wl.t)
-    n_ro = t.shape[0]
-    mm = w[:, None] * mxy_host
-    args = (wl.mesh.nodes, wl.tissue.delta_omega, wl.tissue.T2star, mm)
-    # probe: one readout index, a few lines
-    sl = (slice(0, 1), slice(0, min(8, t.shape[1])), slice(None))
-    t0 = time.perf_counter()
-    oracle.signal_samples(kx[sl], ky[sl], kz[sl], t[sl], *args, n_threads=threads, fast=True)
-    probe = time.perf_counter() - t0
-    per_sample = probe / t[sl].size
-    per_ro = per_sample * t[0].size
-    n_pick = int(max(1, min(n_ro, round(target_s / max(per_ro, 1e-9)))))
-    idx = np.unique(np.linspace(0, n_ro - 1, n_pick).round().astype(int))
-    t0 = time.perf_counter()
-    oracle.signal_samples(kx[idx], ky[idx], kz[idx], t[idx], *args, n_threads=threads, fast=True)
-    dt = time.perf_counter() - t0
-    work = wl.n_nodes * t[idx].size * wl.n_enc
-    return {"value": work / dt, "unit": "node*k-sample/s", "cores": threads, "kind": "port",
-            "sample": f"{idx.size} of {n_ro} readout indices ({t[idx].size} k-samples x {wl.n_nodes} nodes), "
-                      f"{dt:.1f} s on {threads} threads, fast-math FP64 oracle"}
+    tmp = Path(tempfile.mkdtemp(prefix="fm_cpu_"))
+    try:
+        arrays = dict(nodes=wl.mesh.nodes, elems=wl.mesh.elems, phi=wl.tissue.delta_omega, T2=wl.tissue.T2star,
+                      mxy=mxy_h, kx=np.asarray(wl.kloc[0]), ky=np.asarray(wl.kloc[1]), kz=np.asarray(wl.kloc[2]),
+                      t=np.asarray(wl.t))
+        if motion is not None:
+            arrays.update(modes=motion[0], coef_t=motion[1], coef=motion[2])
+        np.savez(tmp / "in.npz", **arrays)
+        ranks = min(CPU_RANKS, os.cpu_count() or 1)
+        cmd = [sys.executable, "-m", "oracle.cpu_ranks", str(tmp / "in.npz"), str(tmp / "out.json"),
+               "--ranks", str(ranks), "--method", "blocks", "--target-s", str(target_s)]
+        res = subprocess.run(cmd, cwd=str(ROOT), capture_output=True, text=True,
+                             env={**os.environ, "OMP_NUM_THREADS": "1"})
+        if res.returncode != 0:
+            raise RuntimeError(f"cpu_ranks failed ({res.returncode}): {res.stderr[-2000:]}")
+        info = json.loads((tmp / "out.json").read_text())
+        ks = np.load(tmp / "out.npy")
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    ro = info["readouts"]
+    work = wl.n_nodes * info["samples"] * mxy_h.shape[1]
+    cpu = {"value": work / info["seconds"], "unit": "node*k-sample/s", "cores": info["ranks"],
+           "kind": "port",
+           "sample": (f"{len(ro)} of {info['n_readouts_total']} readout indices ({info['samples']} k-samples x "
+                      f"{mxy_h.shape[1]} encodings x {wl.n_nodes} nodes), {info['ranks']} forked ranks over "
+                      f"contiguous element blocks ({info['nodes_stored']} stored nodes incl. interface copies), "
+                      f"partial lumped weights per rank, rank-order sum; max rank {info['max_rank_s']:.2f} s + "
+                      f"reduce {info['reduce_s'] * 1e3:.1f} ms; FP64 oracle at -O3 -ffast-math, 1 thread per rank; "
+                      "no MPI in this image"),
+           "seconds": info["seconds"], "ranks": info["ranks"], "readouts": ro}
+    return cpu, ro, ks
 
 
 def serving_run(dev, n_requests: int, rate: float, max_active: int = 8):
@@ -301,7 +325,7 @@ def serving_run(dev, n_requests: int, rate: float, max_active: int = 8):
     out.update({"rate_per_s": rate, "max_active": max_active, "resident_setup_s": prep,
                 "mean_latency_s": float(lat.mean()) if lat.size else None,
                 "p99_latency_by_form_s": {k: float(np.percentile(v, 99)) for k, v in by.items()},
-                "stream": "Poisson arrivals; N log-uniform 10k-2M served by 9 resident box-phantom classes; "
+                "stream": "Poisson arrivals; N log-uniform 10k-2M; "
                           "Cartesian 2-D 64^2-256^2, 3-D 64^2-256^2 x 32, radial 64-512 spokes x 256-512; "
                           "FP64 or mixed"})
     return out
@@ -323,10 +347,10 @@ def _arithmetic(method: str, form: str) -> str:
     return "FP64 phase and range reduction, FP32 sine/cosine and accumulation, FP64 chunk sums"
 
 
-def run_flow(args, dev):
+def run_flow(args, dev) -> dict:
     """c3: 4-D flow.  Set-up on the GPU: lumped weights, the whole
     phase-contrast acquisition through the Bloch integrator with nodes moving
-    along the Poiseuille PODVelocity field, then per cardiac phase the
+    along the Poiseuille velocity field, then per cardiac phase the
     packed nodes (4 encodings) and the POD displacement tables.  A step is
     the k-space of every phase (all encodings) with moving nodes."""
     import numpy as np
@@ -339,30 +363,51 @@ def run_flow(args, dev):
     wl = workloads.c3(args.scale)
     n_ph = wl.n_phases if args.phases <= 0 else min(args.phases, wl.n_phases)
     m, tis = wl.mesh, wl.tissue
-    log(f"c3: {m.n_nodes} nodes, {wl.n_samples} k-samples x 4 encodings x {n_ph} phases; Bloch set-up on GPU")
+    log(f"c3: {m.n_nodes} nodes, {m.n_elems} tets, {wl.n_samples} k-samples x 4 encodings x {n_ph} phases; "
+        "Bloch set-up on GPU")
     col = device.P1Colouring(m.elems, m.n_nodes, device=dev)
     nodes_d = torch.from_numpy(m.nodes).to(dev)
-    w = device.lumped_mass(col, nodes_d)
     blocks = workloads.flow_sequence(wl, n_dummy=8, n_phases=n_ph)
     solver = BlochSolver(blocks, m.nodes, tis.T1, tis.T2, delta_B=tis.delta_B_mT, pod_trajectory=wl.velocity,
                          dev=dev)
-    mxy_store, _ = solver.solve_device()                       # (N, 4 n_ph) complex64
-    # Cartesian plan: with moving nodes the mixed form runs the per-readout
-    # GEMM and the FP64 form the direct kernels
+    mxy0, mz0 = solver.mxy.clone(), solver.mz.clone()
+    # Cartesian plan: with moving nodes both forms run the per-readout GEMM
     plan = device.SignalPlan(wl.kloc, wl.t, method="auto", device=dev)
-    modes = device.modes_soa(torch.from_numpy(Motion.motion_modes(wl.motion)).to(dev))
+    modes_pcr = Motion.motion_modes(wl.motion)
+    modes = device.modes_soa(torch.from_numpy(modes_pcr).to(dev))
     M0 = torch.from_numpy(tis.M0).to(dev)
     phi = torch.from_numpy(tis.delta_omega).to(dev)
     T2s = torch.from_numpy(tis.T2star).to(dev)
-    phases = []
+    coefs = []
     for p in range(n_ph):
         wl.motion.update_timeshift(p * wl.period / wl.n_phases)
-        coef = torch.from_numpy(Motion.motion_coefficients(wl.motion, plan.tgroup_t)).to(dev)
+        coefs.append(torch.from_numpy(Motion.motion_coefficients(wl.motion, plan.tgroup_t)).to(dev))
+    wl.motion.update_timeshift(0.0)
+
+    # the whole scan once untimed (kernels loaded), then timed: weights, the
+    # moving-node Bloch acquisition, and the packed nodes of every phase
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    scan_ms = None
+    for rep in range(2):
+        solver.mxy.copy_(mxy0)
+        solver.mz.copy_(mz0)
+        torch.cuda.synchronize()
+        ev[0].record()
+        w = device.lumped_mass(col, nodes_d)
+        ev[1].record()
+        mxy_store, _ = solver.solve_device()                       # (N, 4 n_ph) complex64
+        ev[2].record()
+        torch.cuda.synchronize()
+        scan_ms = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]))
+    phases = []
+    for p in range(n_ph):
         mxy = mxy_store[:, 4 * p:4 * p + 4].to(torch.complex128) * M0[:, None]
         nodes = device.SignalNodes.from_arrays(nodes_d, phi, T2s, mxy, weights=w, device=dev)
-        phases.append((nodes, coef, mxy))
+        phases.append((nodes, coefs[p], mxy))
     torch.cuda.synchronize()
     work = m.n_nodes * wl.n_samples * 4 * n_ph
+    log(f"set-up: lumped mass {scan_ms[0]:.3f} ms, moving-node Bloch acquisition ({len(blocks)} blocks) "
+        f"{scan_ms[1]:.1f} ms")
 
     def time_it(form, steps, warmup):
         outs = [torch.empty((plan.n_samples, 4), dtype=torch.complex128, device=dev) for _ in phases]
@@ -388,28 +433,35 @@ def run_flow(args, dev):
 
     outs, ms, k_ms = time_it(args.form, args.steps, args.warmup)
     value = work / (ms * 1e-3)
-    log(f"{args.form}: {ms:.1f} ms/step ({n_ph} phases), {value:.3e} node*k/s")
+    log(f"{args.form}: {ms:.1f} ms/step ({n_ph} phases), {value:.3e} node*k/s, dominant kernel {k_ms:.2f} ms/phase")
     forms = {args.form: {"ms_per_step": ms, "node_ksamples_per_s": value, "kernel_ms_per_phase": k_ms}}
     err64 = None
+    out0 = outs[0].cpu().numpy()
+    o64_0 = None
     if not args.no_f64 and args.form != "f64":
         o64, ms64, k64 = time_it("f64", 1, 0)
         forms["f64"] = {"ms_per_step": ms64, "node_ksamples_per_s": work / (ms64 * 1e-3), "kernel_ms_per_phase": k64}
-        a = torch.stack(outs).cpu().numpy()
-        b = torch.stack(o64).cpu().numpy()
-        err64 = rel_l2(a, b)
-        log(f"f64: {ms64:.1f} ms/step; {args.form} vs f64 relL2 {err64:.2e}; speed-up {ms64 / ms:.2f}x")
+        num = den = 0.0
+        for a, b in zip(outs, o64):
+            num += float(torch.linalg.vector_norm(a - b)) ** 2
+            den += float(torch.linalg.vector_norm(b)) ** 2
+        err64 = (num / den) ** 0.5
+        o64_0 = o64[0].cpu().numpy()
+        del o64
+        log(f"f64: {ms64:.1f} ms/step; {args.form} vs f64 relL2 {err64:.2e} over all phases; speed-up {ms64 / ms:.2f}x")
     # FP64 oracle on sampled k-points of phase 0: positions at each sample's t
     rng = np.random.default_rng(7)
     idx = np.sort(rng.choice(wl.n_samples, size=12, replace=False))
     kx, ky, kz = (np.asarray(k).ravel()[idx] for k in wl.kloc)
     tt = np.asarray(wl.t).ravel()[idx]
-    wl.motion.update_timeshift(0.0)
-    mm = w.cpu().numpy()[:, None] * phases[0][2].cpu().numpy()
+    w_h = w.cpu().numpy()
+    mxy0_h = phases[0][2].cpu().numpy()
+    mm = w_h[:, None] * mxy0_h
     ref = np.stack([oracle.signal_samples(kx[i:i + 1], ky[i:i + 1], kz[i:i + 1], tt[i:i + 1],
                                           m.nodes + np.asarray(wl.motion(tt[i])), tis.delta_omega, tis.T2star,
                                           mm, n_threads=min(16, os.cpu_count() or 1))[0]
                     for i in range(idx.size)])
-    err_or = rel_l2(outs[0].cpu().numpy()[idx], ref)
+    err_or = rel_l2(out0[idx], ref)
     log(f"relL2 vs FP64 oracle on {idx.size} sampled k-points of phase 0: {err_or:.2e}")
     k_s = (k_ms or ms / n_ph) * 1e-3   # dominant kernel, per phase
     if plan.method == "cartesian" and args.form == "mixed":
@@ -418,15 +470,18 @@ def run_flow(args, dev):
         achieved = 8.0 * m.n_nodes * wl.n_samples * 4 / k_s / 1e12
         roof = {"bound": "mfma", "kernel": "k_cart_moving_bf16", "achieved": achieved,
                 "peak": PEAKS["bf16_mfma_tflops"], "unit": "TFLOP/s",
-                "frac": achieved / PEAKS["bf16_mfma_tflops"], "traffic": None, "kernel_ms": k_ms,
+                "frac": achieved / PEAKS["bf16_mfma_tflops"],
+                "traffic": load_traffic("k_cart_moving_bf16", "c3", "mixed"), "kernel_ms": k_ms,
                 "executed": 3.0 * achieved,
-                "per_unit": "8 flops per node*k-sample*encoding (complex MAC); operands generated in the "
-                            "kernel (FP64 phase, FP32 sin/cos per node and line / kz, bf16 hi+lo split)"}
+                "per_unit": "8 flops per node*k-sample*encoding (complex MAC); one launch is one cardiac phase "
+                            "(1948800 nodes x 221184 k-samples x 4 encodings); operands generated in the kernel "
+                            "(FP64 phase, FP32 sin/cos per node and line / kz, bf16 hi+lo split)"}
     elif plan.method == "cartesian":
         achieved = 8.0 * m.n_nodes * wl.n_samples * 4 / k_s / 1e12
         roof = {"bound": "mfma", "kernel": "k_cart_moving_f64", "achieved": achieved,
                 "peak": PEAKS["fp64_mfma_tflops"], "unit": "TFLOP/s",
-                "frac": achieved / PEAKS["fp64_mfma_tflops"], "traffic": None, "kernel_ms": k_ms,
+                "frac": achieved / PEAKS["fp64_mfma_tflops"],
+                "traffic": load_traffic("k_cart_moving_f64", "c3", "f64"), "kernel_ms": k_ms,
                 "per_unit": "8 flops per node*k-sample*encoding (complex MAC) on FP64 MFMA; operands "
                             "generated in the kernel (FP64 sincospi per node and line / kz)"}
     else:
@@ -435,17 +490,139 @@ def run_flow(args, dev):
         roof = {"bound": "valu", "kernel": "k_signal_direct (moving nodes)", "achieved": achieved, "peak": peak,
                 "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                 "per_unit": "8 flops + 8 per encoding and 2 transcendentals per node*k-sample"}
-    print(json.dumps({
-        "metric": "node*k-sample/s at relL2<=1e-4 vs FP64 oracle", "value": value, "unit": "node*k-sample/s",
+
+    cpu = None
+    err_cpu = {}
+    if not args.no_cpu:
+        log(f"CPU baseline: {CPU_RANKS} forked ranks (~{args.cpu_seconds:.0f} s)")
+        mot = (modes_pcr, plan.tgroup_t, coefs[0].cpu().numpy())
+        cpu, ro, ks = cpu_baseline(wl, w_h, mxy0_h, args.cpu_seconds, motion=mot)
+        shape = (wl.t.shape[0], -1, 4)
+        sel = lambda o: o.reshape(shape)[ro].reshape(-1, 4)  # noqa: E731
+        err_cpu[args.form] = rel_l2(sel(out0), ks)
+        if o64_0 is not None:
+            err_cpu["f64"] = rel_l2(sel(o64_0), ks)
+        log(f"CPU: {cpu['value']:.3e} node*k/s on {cpu['cores']} ranks; GPU vs CPU on its {ks.shape[0]} k-samples "
+            f"x 4 encodings: {err_cpu}")
+    return {
+        "metric": METRIC, "value": value, "unit": "node*k-sample/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": _dtype(plan.method, args.form), "data": "synthetic",
-        "config": {"workload": wl.info["desc"], "config": "c3", "nodes": m.n_nodes, "k_samples": wl.n_samples,
-                   "encodings": 4, "phases": n_ph, "method": plan.method, "form": args.form,
-                   "arithmetic": _arithmetic(plan.method, args.form)},
-        "relL2_vs_f64_gpu": err64, "relL2_vs_oracle_sampled": err_or, "forms": forms,
-        "roofline": roof, "cpu_baseline": None, "serving": None}), flush=True)
-    return 0
+        "config": {"workload": wl.info["desc"], "config": "c3", "nodes": m.n_nodes, "elements": m.n_elems,
+                   "k_samples": wl.n_samples, "encodings": 4, "phases": n_ph, "method": plan.method,
+                   "form": args.form, "arithmetic": _arithmetic(plan.method, args.form)},
+        "relL2_vs_f64_gpu": err64, "relL2_vs_oracle_sampled": err_or, "relL2_vs_cpu_baseline_readouts": err_cpu,
+        "forms": forms, "roofline": roof, "cpu_baseline": cpu,
+        "setup_ms": {"lumped_mass": scan_ms[0], "bloch_moving_acquisition": scan_ms[1],
+                     "bloch_blocks": len(blocks)},
+    }
+
+
+def measure_static(args, dev, wl, form: str, with_cpu: bool) -> dict:
+    """c0 / c1 / c2: the static-node signal of one Bloch steady state."""
+    from feelmri_amd import device
+
+    log(f"{wl.name}: {wl.n_nodes} nodes, {wl.mesh.n_elems} tets, {wl.n_samples} k-samples; preparing on GPU")
+    nodes, w, mxy = prepare_nodes(wl, dev)
+    plan = device.SignalPlan(wl.kloc, wl.t, method=args.method, device=dev)
+    log(f"plan: method {plan.method}, {plan.n_tgroups} distinct t")
+    work = wl.node_ksamples
+    steps = max(1, args.steps)
+    out, ms_ev, ms_wall, ms_kernel = time_form(plan, nodes, form, steps, args.warmup)
+    value = work / (ms_ev * 1e-3)
+    log(f"{form}: {ms_ev:.3f} ms/step (events), {ms_wall:.3f} ms/step (wall), {value:.3e} node*k/s")
+    out_h = out.cpu().numpy()
+
+    forms = {form: {"ms_per_step": ms_ev, "node_ksamples_per_s": value, "kernel_ms": ms_kernel}}
+    err_f64 = err_img = err_img_tukey = None
+    out64_h = None
+    if not args.no_f64 and form != "f64":
+        ref_steps = 1 if plan.method == "direct" else steps
+        out64, ms64, _, ms64_kernel = time_form(plan, nodes, "f64", ref_steps, 1)
+        forms["f64"] = {"ms_per_step": ms64, "node_ksamples_per_s": work / (ms64 * 1e-3),
+                        "kernel_ms": ms64_kernel}
+        out64_h = out64.cpu().numpy()
+        err_f64 = rel_l2(out_h, out64_h)
+        if wl.name in ("c0", "c1"):
+            from feelmri_amd.recon import image_rel_l2
+
+            shape = (*wl.t.shape, wl.n_enc)
+            err_img = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape))
+            # CartesianRecon's Tukey-windowed image (Recon.py:13-90, before its resize)
+            err_img_tukey = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape), "Tukey")
+        elif wl.name.startswith("c2"):
+            from feelmri_amd.recon import adjoint_image_rel_l2
+
+            # 2-D radial / spiral: direct adjoint DFT with ramp density compensation, 128^2 over the FOV
+            err_img = adjoint_image_rel_l2(wl.kloc[0], wl.kloc[1], out_h[:, 0], out64_h[:, 0], 0.24, n=128)
+        log(f"f64: {ms64:.3f} ms/step; {form} vs f64 relL2 {err_f64:.2e} (image {err_img}); "
+            f"speed-up {ms64 / ms_ev:.2f}x")
+    w_h = w.cpu().numpy()
+    mxy_h = mxy.cpu().numpy()
+    ms_host = time_host_io(plan, wl, w_h, mxy_h, form, max(3, steps // 4))
+    log(f"{form} from host arrays (H2D nodes + pack + execute + D2H k-space): {ms_host:.3f} ms/step")
+    forms[form]["host_io_ms_per_step"] = ms_host
+    forms[form]["host_io_node_ksamples_per_s"] = work / (ms_host * 1e-3)
+    if wl.name == "c0":
+        ms_lines, n_lines = time_shim_per_line(wl, w_h, mxy_h, form)
+        log(f"{form} per-line drop-in calls ({n_lines} x MRI.Signal from host arrays): {ms_lines:.2f} ms "
+            f"per k-space, {ms_lines / n_lines:.3f} ms per call")
+        forms[form]["per_line_shim_ms_per_kspace"] = ms_lines
+        forms[form]["per_line_shim_calls"] = n_lines
+    scan = time_scan(wl, dev, plan, form)
+    log(f"end-to-end scan (weights + Bloch + pack + signal): {scan['ms']:.3f} ms, "
+        f"{scan['node_ksamples_per_s']:.3e} node*k/s, parts {scan['parts_ms']}")
+    err_oracle = oracle_subset_error(wl, w_h, mxy_h, out_h)
+    log(f"relL2 vs FP64 oracle on sampled k-points: {err_oracle:.2e}")
+
+    # roofline of the dominant kernel
+    if plan.method == "cartesian":
+        bound, unit = "mfma", "TFLOP/s"
+        flops = 8.0 * work
+        peak = PEAKS["bf16_mfma_tflops"] if form == "mixed" else PEAKS["fp64_mfma_tflops"]
+        dom = "k_cart_gemm"
+    elif form == "mixed" and plan.n_runs > 0:
+        bound, unit = "valu", "TFLOP/s"
+        flops = (8.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
+        peak = PEAKS["fp32_valu_tflops"]
+        dom = "k_signal_runs"
+    else:
+        bound, unit = "valu", "TFLOP/s"
+        flops = (8.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
+        peak = PEAKS["fp64_valu_tflops"] if form == "f64" else PEAKS["fp32_valu_tflops"]
+        dom = "k_signal_direct"
+    ms_dom = ms_kernel if ms_kernel else ms_ev
+    achieved = flops / (ms_dom * 1e-3) / 1e12
+    roof = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": load_traffic(dom, wl.name, form), "kernel_ms": ms_dom,
+            "per_unit": ("8 flops per node*k-sample*encoding (complex MAC); the mixed GEMM executes 3 bf16 "
+                         "products (hi*hi + hi*lo + lo*hi) per real product, i.e. 24" if plan.method == "cartesian"
+                         else "8 flops (k.x 3 FMA + phi t 1 FMA) + 8 per encoding (complex MAC) and 2 "
+                              "transcendentals per node*k-sample")}
+    if plan.method == "cartesian" and form == "mixed":
+        roof["executed"] = 3.0 * achieved
+    cpu = None
+    err_cpu = {}
+    if with_cpu:
+        log(f"CPU baseline: {CPU_RANKS} forked ranks (~{args.cpu_seconds:.0f} s)")
+        cpu, ro, ks = cpu_baseline(wl, w_h, mxy_h, args.cpu_seconds)
+        V = wl.n_enc
+        sel = lambda o: o.reshape(wl.t.shape[0], -1, V)[ro].reshape(-1, V)  # noqa: E731
+        err_cpu[form] = rel_l2(sel(out_h), ks)
+        if out64_h is not None:
+            err_cpu["f64"] = rel_l2(sel(out64_h), ks)
+        log(f"CPU: {cpu['value']:.3e} node*k/s on {cpu['cores']} ranks; GPU vs CPU: {err_cpu}")
+    return {
+        "value": value, "ms_per_step": ms_ev, "dtype": _dtype(plan.method, form),
+        "config": {"workload": wl.info["desc"], "config": wl.name, "nodes": wl.n_nodes,
+                   "k_samples": wl.n_samples, "encodings": wl.n_enc, "method": plan.method, "form": form,
+                   "arithmetic": _arithmetic(plan.method, form)},
+        "relL2_vs_f64_gpu": err_f64, "image_relL2_vs_f64_gpu": err_img,
+        "image_relL2_tukey_vs_f64_gpu": err_img_tukey, "relL2_vs_oracle_sampled": err_oracle,
+        "relL2_vs_cpu_baseline_readouts": err_cpu, "forms": forms, "roofline": roof, "cpu_baseline": cpu,
+        "scan": scan,
+    }
 
 
 def _time_cuda(fn, reps: int) -> float:
@@ -462,14 +639,21 @@ def _time_cuda(fn, reps: int) -> float:
     return e0.elapsed_time(e1) / reps
 
 
+BLOCH_FLOPS = 80.0   # per node*event: the rotation, relaxation and |B| of BlochSimulator.cpp:231-271 (static)
+
+
 def row_benchmarks(wl, dev) -> dict:
     """Throughput and roofline of the other rows of the path on the workload's
-    mesh: the Bloch integrator (final state and full history), P1 lumped
-    mass, and the POD contraction at the 4-D flow size (2 M nodes, 10 modes)."""
+    mesh: the Bloch integrator (RF-sparse event rows, the RF-heavy scan chain
+    and full history), P1 lumped mass, and the POD contraction at the 4-D
+    flow size (2 M nodes, 10 modes).  Rooflines count the algorithmic work
+    per unit: 80 flops per node*event for Bloch whatever the event, 16 B per
+    element + 32 B per node for the lumped mass."""
     import numpy as np
     import torch
 
     from feelmri_amd import device
+    from feelmri_amd.Bloch import BlochSolver
     from feelmri_amd.sequences import GAMMA
 
     m, tis = wl.mesh, wl.tissue
@@ -488,15 +672,37 @@ def row_benchmarks(wl, dev) -> dict:
     mz0 = torch.ones(N, dtype=torch.float64, device=dev)
     for prec, peak, key in (("f64", PEAKS["fp64_valu_tflops"], "bloch_f64"), ("f32", PEAKS["fp32_valu_tflops"], "bloch_f32")):
         ms = _time_cuda(lambda: device.bloch(bn, ev, 1.0, GAMMA, mxy0, mz0, precision=prec), 5)
-        ach = 70.0 * N * T / (ms * 1e-3) / 1e12
-        n_rf = int(np.count_nonzero(rf[1:]))   # step i uses event i + 1
-        executed = (70.0 * n_rf + 18.0 * (T - 1 - n_rf)) * N / (ms * 1e-3) / 1e12
-        rows[key] = {"node_events_per_s": N * T / (ms * 1e-3), "ms": ms, "nodes": N, "events": T,
+        ach = BLOCH_FLOPS * N * (T - 1) / (ms * 1e-3) / 1e12
+        rows[key] = {"node_events_per_s": N * (T - 1) / (ms * 1e-3), "ms": ms, "nodes": N, "events": T,
+                     "rf_events": 20,
                      "roofline": {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-                                  "frac": ach / peak, "executed": executed,
-                                  "per_unit": "~70 flops + sqrt, rcp, sin, cos per node*event (final state kept); "
-                                              "events without RF execute the precession the rotation reduces to, "
-                                              "~18 flops + sin, cos"}}
+                                  "frac": ach / peak,
+                                  "per_unit": "80 flops + sqrt, rcp, sin, cos per node*event (final state kept); "
+                                              "979 of 999 steps carry no RF and take the precession the rotation "
+                                              "reduces to (~18 flops + sin, cos), so this row flatters the kernel; "
+                                              "see bloch_scan_chain for an RF-heavy chain"}}
+    # the RF-heavy scan chain: the workload's spoiled-GRE blocks in one chained launch
+    blocks = wl.blocks
+    n_ev = sum(len(b.dt) - 1 for b in blocks)
+    n_rf = sum(int(np.count_nonzero(np.asarray(b.rf)[1:])) for b in blocks)
+    for prec, peak in (("f32", PEAKS["fp32_valu_tflops"]), ("f64", PEAKS["fp64_valu_tflops"])):
+        solver = BlochSolver(blocks, m.nodes, tis.T1, tis.T2, delta_B=tis.delta_B_mT, precision=prec, dev=dev)
+        x0, z0 = solver.mxy.clone(), solver.mz.clone()
+
+        def run():
+            solver.mxy.copy_(x0)
+            solver.mz.copy_(z0)
+            solver.solve_device()
+        ms_copy = _time_cuda(lambda: (solver.mxy.copy_(x0), solver.mz.copy_(z0)), 5)
+        ms = _time_cuda(run, 5) - ms_copy
+        ach = BLOCH_FLOPS * N * n_ev / (ms * 1e-3) / 1e12
+        rows[f"bloch_scan_chain_{prec}"] = {
+            "node_events_per_s": N * n_ev / (ms * 1e-3), "ms": ms, "nodes": N, "events": n_ev, "rf_events": n_rf,
+            "blocks": len(blocks),
+            "roofline": {"bound": "valu", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                         "per_unit": f"80 flops + sqrt, rcp, sin, cos per node*event; {n_rf} of {n_ev} steps carry "
+                                     "RF; the spoiled-GRE scan chain (16 dummy TRs + 1) as one chained launch"}}
+        del solver
     Th = 200
     evh = device.EventTable(rf[:Th], G[:Th], np.full(Th, 0.01), device=dev)
     ms = _time_cuda(lambda: device.bloch(bn, evh, 1.0, GAMMA, mxy0, mz0, precision="f32", history=True), 3)
@@ -513,16 +719,16 @@ def row_benchmarks(wl, dev) -> dict:
     ms = _time_cuda(lambda: device.lumped_mass(col, nodes_d, out=out), 5)
     ms_coloured = _time_cuda(lambda: device.lumped_mass_coloured(col, nodes_d, out=out), 5)
     E = m.n_elems
-    by = E * (16 + 16 + 64) + N * (24 + 4 + 8)  # connectivity, slot indices, slot values out+in; coords, degree, w
+    by = E * 16 + N * 32   # compulsory: connectivity read, coordinates read, weights written
     rows["lumped_mass_p1"] = {"elements_per_s": E / (ms * 1e-3), "ms": ms, "elements": E, "colours": col.n_colours,
-                              "coloured_scatter_ms": ms_coloured,
+                              "coloured_scatter_ms": ms_coloured, "method": col.method,
                               "roofline": {"bound": "hbm", "achieved": by / (ms * 1e-3) / 1e9,
                                            "peak": PEAKS["hbm_gbs"], "unit": "GB/s",
                                            "frac": by / (ms * 1e-3) / 1e9 / PEAKS["hbm_gbs"],
-                                           "per_unit": "96 B per element (16 B nodes, 16 B slot indices, 4 x 8 B "
-                                                       "|det J| written and read back) + 36 B per node (24 B "
-                                                       "coordinates, 4 B degree, 8 B weight); two launches: "
-                                                       "per-element slot writes, per-node gather in colour order"}}
+                                           "traffic": load_traffic("lumped_mass", "c1", "f64"),
+                                           "per_unit": "compulsory bytes: 16 B per element (4 x int32 connectivity) "
+                                                       "+ 24 B per node coordinates read + 8 B per node weight "
+                                                       "written"}}
     # POD contraction at 2 M nodes, 10 modes (float32)
     P, R = 2_000_000, 10
     modes = torch.randn((P, 3, R), dtype=torch.float32, device=dev)
@@ -556,17 +762,18 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=10,
+    ap.add_argument("--warmup", type=int, default=5,
                     help="untimed steps first; the clock ramps up over the first few launches")
     ap.add_argument("--form", default="mixed", choices=["mixed", "f64"])
     ap.add_argument("--method", default="auto", choices=["auto", "direct", "cartesian"])
-    ap.add_argument("--config", default="c1")
+    ap.add_argument("--config", default="c3")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--phases", type=int, default=0, help="c3: cardiac phases per step (0: all)")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison form")
     ap.add_argument("--no-rows", action="store_true", help="skip the Bloch / mass / POD row measurements")
+    ap.add_argument("--no-secondary", action="store_true", help="c3: skip the c1 secondary block")
     ap.add_argument("--serve-requests", type=int, default=60,
                     help="requests in the serving stream reported beside the throughput (0: none)")
     ap.add_argument("--rate", type=float, default=20.0, help="serving stream arrival rate (requests/s)")
@@ -582,10 +789,7 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
-
-    from feelmri_amd import device
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -609,139 +813,32 @@ def main() -> int:
             "config": {"workload": f"serving stream, {srv['requests']} requests at {args.rate}/s", "config": "c4"},
             "serving": srv}), flush=True)
         return 0
+
     if args.config == "c3":
-        return run_flow(args, dev)
-    log(f"building workload {args.config} (scale {args.scale})")
-    wl = build_workload(args.config, args.scale)
-    log(f"{wl.n_nodes} nodes, {wl.mesh.n_elems} tets, {wl.n_samples} k-samples; preparing on GPU")
-    nodes, w, mxy = prepare_nodes(wl, dev)
-    plan = device.SignalPlan(wl.kloc, wl.t, method=args.method, device=dev)
-    log(f"plan: method {plan.method}, {plan.n_tgroups} distinct t")
-    work = wl.node_ksamples
-
-    out, ms_ev, ms_wall, ms_kernel = time_form(plan, nodes, args.form, args.steps, args.warmup)
-    value = work / (ms_ev * 1e-3)
-    log(f"{args.form}: {ms_ev:.3f} ms/step (events), {ms_wall:.3f} ms/step (wall), {value:.3e} node*k/s")
-    out_h = out.cpu().numpy()
-
-    forms = {args.form: {"ms_per_step": ms_ev, "node_ksamples_per_s": value, "kernel_ms": ms_kernel}}
-    err_f64 = err_img = err_img_tukey = None
-    if not args.no_f64 and args.form != "f64":
-        ref_steps = 1 if plan.method == "direct" else max(1, args.steps)
-        out64, ms64, _, ms64_kernel = time_form(plan, nodes, "f64", ref_steps, 1)
-        forms["f64"] = {"ms_per_step": ms64, "node_ksamples_per_s": work / (ms64 * 1e-3),
-                        "kernel_ms": ms64_kernel}
-        out64_h = out64.cpu().numpy()
-        err_f64 = rel_l2(out_h, out64_h)
-        if wl.name in ("c0", "c1"):
-            from feelmri_amd.recon import image_rel_l2
-
-            shape = (*wl.t.shape, wl.n_enc)
-            err_img = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape))
-            # CartesianRecon's Tukey-windowed image (Recon.py:13-90, before its resize)
-            err_img_tukey = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape), "Tukey")
-        elif wl.name.startswith("c2"):
-            from feelmri_amd.recon import adjoint_image_rel_l2
-
-            # 2-D radial / spiral: direct adjoint DFT with ramp density compensation, 128^2 over the FOV
-            err_img = adjoint_image_rel_l2(wl.kloc[0], wl.kloc[1], out_h[:, 0], out64_h[:, 0], 0.24, n=128)
-        log(f"f64: {ms64:.3f} ms/step; {args.form} vs f64 relL2 {err_f64:.2e} (image {err_img}); "
-            f"speed-up {ms64 / ms_ev:.2f}x")
-    w_h = w.cpu().numpy()
-    mxy_h = mxy.cpu().numpy()
-    ms_host = time_host_io(plan, wl, w_h, mxy_h, args.form, max(3, args.steps // 4))
-    log(f"{args.form} from host arrays (H2D nodes + pack + execute + D2H k-space): {ms_host:.3f} ms/step")
-    forms[args.form]["host_io_ms_per_step"] = ms_host
-    forms[args.form]["host_io_node_ksamples_per_s"] = work / (ms_host * 1e-3)
-    if wl.name == "c0":
-        ms_lines, n_lines = time_shim_per_line(wl, w_h, mxy_h, args.form)
-        log(f"{args.form} per-line drop-in calls ({n_lines} x MRI.Signal from host arrays): {ms_lines:.2f} ms "
-            f"per k-space, {ms_lines / n_lines:.3f} ms per call")
-        forms[args.form]["per_line_shim_ms_per_kspace"] = ms_lines
-        forms[args.form]["per_line_shim_calls"] = n_lines
-    scan = time_scan(wl, dev, plan, args.form)
-    log(f"end-to-end scan (weights + Bloch + pack + signal): {scan['ms']:.3f} ms, "
-        f"{scan['node_ksamples_per_s']:.3e} node*k/s, parts {scan['parts_ms']}")
-    err_oracle = oracle_subset_error(wl, w_h, mxy_h, out_h)
-    log(f"relL2 vs FP64 oracle on sampled k-points: {err_oracle:.2e}")
-
-    # roofline of the dominant kernel
-    if plan.method == "cartesian":
-        bound, unit = "mfma", "TFLOP/s"
-        flops = 8.0 * work
-        peak = PEAKS["bf16_mfma_tflops"] if args.form == "mixed" else PEAKS["fp64_mfma_tflops"]
-        dom = "k_cart_gemm"
-    elif args.form == "mixed" and plan.n_runs > 0:
-        # one complex step (6 flops) and one complex MAC per encoding (8) per
-        # node*k-sample, in FP32
-        bound, unit = "valu", "TFLOP/s"
-        flops = (6.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
-        peak = PEAKS["fp32_valu_tflops"]
-        dom = "k_signal_runs"
+        line = run_flow(args, dev)
+        torch.cuda.empty_cache()
+        aux_wl = None
+        if not args.no_secondary or not args.no_rows:
+            aux_wl = build_workload("c1", args.scale)
+        if not args.no_secondary:
+            line["secondary"] = {"c1": measure_static(args, dev, aux_wl, args.form, with_cpu=False)}
     else:
-        bound, unit = "valu", "TFLOP/s"
-        flops = (8.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
-        peak = PEAKS["fp64_valu_tflops"] if args.form == "f64" else PEAKS["fp32_valu_tflops"]
-        dom = "k_signal_direct"
-    ms_dom = ms_kernel if ms_kernel else ms_ev
-    achieved = flops / (ms_dom * 1e-3) / 1e12
-    traffic = load_traffic(dom, wl.name, args.form)
-    roof = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": traffic, "kernel_ms": ms_dom,
-            "executed": (3.0 * achieved if (plan.method == "cartesian" and args.form == "mixed")
-                         else (8.0 * wl.n_enc / (6.0 + 8.0 * wl.n_enc) * achieved if dom == "k_signal_runs"
-                               else achieved)),
-            "per_unit": "8 flops per node*k-sample*encoding (complex MAC); the mixed GEMM executes 3 bf16 "
-                        "products (hi*hi + hi*lo + lo*hi) per real product, i.e. 24" if plan.method == "cartesian"
-            else ("6 flops (complex step) + 8 per encoding (complex MAC) per node*k-sample, FP32; one FP64-anchored "
-                  "sincos per node and 16-sample readout run; the kernel advances y = e*c per encoding, executing "
-                  "a complex multiply and a complex add (8 flops) per node*k-sample*encoding" if dom == "k_signal_runs"
-                  else "8 flops + 8 per encoding and 2 transcendentals per node*k-sample")}
-
-    cpu = None
-    if not args.no_cpu:
-        log(f"CPU baseline (~{args.cpu_seconds:.0f} s)")
-        cpu = cpu_baseline(wl, w_h, mxy_h, args.cpu_seconds)
-        log(f"CPU: {cpu['value']:.3e} node*k/s on {cpu['cores']} threads")
-
-    rows = None
-    if not args.no_rows:
-        rows = row_benchmarks(wl, dev)
+        aux_wl = build_workload(args.config, args.scale)
+        sec = measure_static(args, dev, aux_wl, args.form, with_cpu=not args.no_cpu)
+        line = {"metric": METRIC, "value": sec.pop("value"), "unit": "node*k-sample/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec.pop("ms_per_step"),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": sec.pop("dtype"),
+                "data": "synthetic", **sec}
+    if not args.no_rows and aux_wl is not None:
+        rows = row_benchmarks(aux_wl, dev)
         log("rows: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['roofline']['frac']:.2f} of {v['roofline']['bound']})"
                                   for k, v in rows.items()))
-    srv = None
+        line["rows"] = rows
     if args.serve_requests > 0:
         srv = serving_run(dev, args.serve_requests, args.rate, args.serve_slots)
         log(f"serving: p50 {srv['p50_latency_s']:.3f} s, p99 {srv['p99_latency_s']:.3f} s, "
             f"{srv['node_ksamples_per_s']:.3e} node*k/s")
-
-    line = {
-        "metric": "node*k-sample/s at relL2<=1e-4 vs FP64 oracle",
-        "value": value,
-        "unit": "node*k-sample/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_ev,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": _dtype(plan.method, args.form),
-        "data": "synthetic",
-        "config": {"workload": wl.info["desc"], "config": wl.name, "nodes": wl.n_nodes,
-                   "k_samples": wl.n_samples, "encodings": wl.n_enc, "method": plan.method, "form": args.form,
-                   "arithmetic": _arithmetic(plan.method, args.form)},
-        "relL2_vs_f64_gpu": err_f64,
-        "image_relL2_vs_f64_gpu": err_img,
-        "image_relL2_tukey_vs_f64_gpu": err_img_tukey,
-        "relL2_vs_oracle_sampled": err_oracle,
-        "forms": forms,
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "serving": srv,
-        "rows": rows,
-        "scan": scan,
-    }
+        line["serving"] = srv
     print(json.dumps(line), flush=True)
     return 0
 

@@ -6,6 +6,14 @@ Reference-shaped drop-ins (numpy in, numpy out):
 Device-resident API: ``feelmri_amd.device``.  Kernels: libfeelmri_hip.so
 (C ABI in include/feelmri_hip.h), built by ``python -m feelmri_amd.build``.
 """
-__version__ = "0.1.0"
+__version__ = "0.2.0"
 
-from . import _lib  # noqa: F401  (binding only; loads nothing until first call)
+
+def __getattr__(name):
+    # the binding (and torch, whose HIP runtime it binds to) is imported on
+    # first use, so host-only modules such as ``partition`` import without it
+    if name == "_lib":
+        import importlib
+
+        return importlib.import_module(f"{__name__}._lib")
+    raise AttributeError(f"module 'feelmri_amd' has no attribute {name!r}")

@@ -6,8 +6,8 @@ TEST INFRASTRUCTURE ONLY.  This package is the checker for the HIP path in
 
 It wraps ``oracle/_build/liboracle.so`` (plain C, strict IEEE FP64, built by
 ``oracle/Makefile``) and adds the host-side pieces the reference keeps in
-Python: the ``M * Mxy`` product (cpp/feelmri/MRI.cpp:326), the per-``t``
-position update through a motion callable (MRI.cpp:192-217), and the block
+Python: the ``M * Mxy`` product (cpp/feelmri/MRI.cpp:243), the per-``t``
+position update through a motion callable (MRI.cpp:109-134), and the block
 chaining / artificial spoiling of ``BlochSolver.solve``
 (python/feelmri/Bloch.py:298-380).
 
@@ -81,7 +81,7 @@ def max_threads(fast: bool = False) -> int:
 
 
 # ---------------------------------------------------------------------------
-# M * Mxy  (cpp/feelmri/MRI.cpp:326)
+# M * Mxy  (cpp/feelmri/MRI.cpp:243)
 # ---------------------------------------------------------------------------
 def weighted_mxy(M, Mxy) -> np.ndarray:
     """Return ``M @ Mxy`` in complex128.
@@ -112,7 +112,7 @@ def weighted_mxy(M, Mxy) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
-# Signal  (cpp/feelmri/MRI.cpp:121-242 moving, :279-381 static)
+# Signal  (cpp/feelmri/MRI.cpp:38-159 moving, :196-297 static)
 # ---------------------------------------------------------------------------
 def signal_samples(kx, ky, kz, t, r, phi_dB0, T2, mmxy, n_threads: int = 0,
                    fast: bool = False) -> np.ndarray:
@@ -138,7 +138,7 @@ def Signal(MPI_rank, M, kloc, t, r0, phi_dB0, T2, Mxy, pod_trajectory=None,
     Returns (n_meas, n_lines, n_kz, n_enc) complex128.  With a motion
     callable, positions are ``r0 + pod_trajectory(t)`` re-evaluated whenever
     ``t`` changes between consecutive samples in (i, j, k) order
-    (MRI.cpp:192-217).
+    (MRI.cpp:109-134).
     """
     t = _f64(t)
     shape = t.shape
@@ -161,7 +161,7 @@ def Signal(MPI_rank, M, kloc, t, r0, phi_dB0, T2, Mxy, pod_trajectory=None,
 
 
 # ---------------------------------------------------------------------------
-# Bloch  (cpp/feelmri/BlochSimulator.cpp:92-228 moving, :231-357 static)
+# Bloch  (cpp/feelmri/BlochSimulator.cpp:12-148 moving, :151-276 static)
 # ---------------------------------------------------------------------------
 def solve_mri(r0, T1, T2, delta_B, M0, gamma, rf_all, G_all, dt, regime_idx,
               Mxy_initial, Mz_initial, pod_trajectory=None, history: bool = True,
@@ -182,7 +182,7 @@ def solve_mri(r0, T1, T2, delta_B, M0, gamma, rf_all, G_all, dt, regime_idx,
     mz0 = _f64(Mz_initial).ravel()
     disp = None
     if pod_trajectory is not None:
-        pod_trajectory(0.0)  # BlochSimulator.cpp:124 evaluates once at t=0
+        pod_trajectory(0.0)  # BlochSimulator.cpp:44 evaluates once at t=0
         disp = np.empty((max(T - 1, 0), N, 3), dtype=np.float64)
         ti = 0.0
         for i in range(T - 1):
@@ -230,7 +230,7 @@ def bloch_solver_solve(blocks, r0, T1, T2, delta_B, M0, gamma, initial_Mxy, init
 
 
 # ---------------------------------------------------------------------------
-# P1 mass  (cpp/feelmri/Assemble.cpp:54-155, python/feelmri/Phantom.py:379-399)
+# P1 mass  (cpp/feelmri/Assemble.cpp:15-116, python/feelmri/Phantom.py:379-399)
 # ---------------------------------------------------------------------------
 def mass_p1_element_matrices(elems, nodes) -> np.ndarray:
     elems = np.ascontiguousarray(elems, dtype=np.int32).reshape(-1, 4)
@@ -252,11 +252,11 @@ def MassAssemble(elems, nodes):
     return sp.csc_matrix((vals.ravel(), (rows, cols)), shape=(N, N))
 
 
-def mass_p1_lumped(elems, nodes) -> np.ndarray:
+def mass_p1_lumped(elems, nodes, fast: bool = False) -> np.ndarray:
     elems = np.ascontiguousarray(elems, dtype=np.int32).reshape(-1, 4)
     nodes = _f64(nodes).reshape(-1, 3)
     w = np.empty(nodes.shape[0], dtype=np.float64)
-    _lib().orc_mass_p1_lumped(elems.shape[0], nodes.shape[0], _ptr(elems), _ptr(nodes), _ptr(w))
+    _lib(fast).orc_mass_p1_lumped(elems.shape[0], nodes.shape[0], _ptr(elems), _ptr(nodes), _ptr(w))
     return w
 
 

@@ -35,6 +35,65 @@ def test_alone_equals_mix(gpu):
     assert s["p50_latency_s"] is not None and s["p99_latency_s"] >= s["p50_latency_s"]
 
 
+def _c4_mix():
+    """Requests at the serving stream's stated sizes: N log-uniform in
+    10k-2M (here both ends and the middle), Cartesian 2-D 64^2-256^2, 3-D
+    up to 256^2 x 32, radial up to 512 spokes x 512 samples, FP64 and mixed;
+    Poisson arrivals at 20/s."""
+    from feelmri_amd.serving import poisson_stream
+
+    drawn = poisson_stream(40, rate=20.0, seed=1004)
+    # the largest FP64 3-D request and a mixed 512-sample radial request
+    # are fixed; four more come from the stream as drawn
+    pick = [r for r in drawn if r.n_nodes < 400_000][:4]
+    t = [r.arrival for r in pick]
+    reqs = [ScanRequest(100, 0.0, 1_100_000, "cart3d", (256, 256, 32), "f64", 11),
+            ScanRequest(101, 0.01, 800_000, "radial", (512, 256, 1), "mixed", 12)]
+    for k, r in enumerate(pick):
+        reqs.append(ScanRequest(102 + k, 0.02 + 0.05 * k, r.n_nodes, r.traj, r.shape, r.precision, r.seed))
+    assert t == sorted(t)
+    return reqs
+
+
+def test_c4_sizes_alone_equals_mix_and_oracle(gpu, oracle):
+    """c4 at its stated sizes: a 1.1M-node FP64 256^2 x 32 request, a mixed
+    512-sample radial one and stream draws run concurrently; each k-space is
+    bitwise the one the request gets alone, and the FP64 request matches the
+    FP64 oracle chain (lumped mass, spoiled-GRE steady state stored as
+    complex64 as BlochSolver keeps it, signal) at 16 sampled k-points."""
+    from feelmri_amd import sequences
+    from feelmri_amd.serving import request_trajectory
+
+    reqs = _c4_mix()
+    be = GpuBackend(dev=gpu)
+    mix = Scheduler(be, max_active=8).run(reqs)
+    assert all(r.error is None for r in mix.results.values()), [r.error for r in mix.results.values()]
+    assert mix.max_active >= 2
+    for req in reqs:
+        req.arrival = 0.0
+        alone = Scheduler(be, max_active=1).run([req])
+        a, b = alone.results[req.rid].kspace, mix.results[req.rid].kspace
+        assert a.shape == b.shape == (*req.shape, 1)
+        assert np.array_equal(a, b), f"request {req.rid} ({req.n_nodes} nodes, {req.traj} {req.shape}, " \
+                                     f"{req.precision}) differs alone vs in the mix"
+    big = reqs[0]
+    ph = be.store.get(big.n_nodes)
+    m, tis = ph["mesh"], ph["tissue"]
+    w = oracle.mass_p1_lumped(m.elems, m.nodes)
+    blocks = [b.as_dict() for b in be._steady_blocks()]
+    mxy, _, _, _ = oracle.bloch_solver_solve(blocks, m.nodes, tis.T1, tis.T2, tis.delta_B_mT, 1.0,
+                                             sequences.GAMMA, np.zeros(m.n_nodes), np.ones(m.n_nodes))
+    mxy = mxy[:, -1].astype(np.complex64).astype(np.complex128) * tis.M0
+    kloc, t = request_trajectory(big)
+    rng = np.random.default_rng(5)
+    idx = np.sort(rng.choice(t.size, 16, replace=False))
+    ref = oracle.signal_samples(*(np.asarray(k).ravel()[idx] for k in kloc), t.ravel()[idx], m.nodes,
+                                tis.delta_omega, tis.T2star, w[:, None] * mxy[:, None], n_threads=16)
+    got = mix.results[big.rid].kspace.reshape(-1, 1)[idx]
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err < 1e-8, err
+
+
 def test_budget_queues_requests(gpu):
     be = GpuBackend(dev=gpu, chunk_rows=16, n_dummy=2)
     reqs = _mix()[:3]
