@@ -39,7 +39,8 @@ class BlochSolver:
         self.precision = precision
         self.pod_trajectory = pod_trajectory
         if pod_trajectory is not None and not Motion.is_supported(pod_trajectory):
-            raise TypeError("BlochSolver motion must be a POD / PODVelocity / PODSum / RespiratoryMotion")
+            raise TypeError("BlochSolver motion must have a modal form: the reference's POD / PODVelocity / "
+                            "PODSum / RespiratoryMotion, or feelmri_amd.Motion's classes")
         with torch.cuda.device(self.dev):
             self.nodes = device.BlochNodes.from_arrays(r0, np.asarray(T1) * ones, np.asarray(T2) * ones,
                                                        np.asarray(delta_B).reshape(-1) * ones,

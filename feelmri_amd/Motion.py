@@ -1,211 +1,252 @@
-"""POD motion: the reference's motion classes and their device descriptors.
+"""Node motion: modal displacement / velocity fields and their device form.
 
-The classes mirror python/feelmri/Motion.py (same constructors, same
-``__call__(t) -> (N, 3)`` host semantics, same ``update_timeshift``):
+The signal and Bloch kernels never receive per-node positions from the host.
+A motion becomes ``modes`` -- (P, 3, R) float32 spatial fields -- plus, per
+evaluation time, one coefficient row of R mode weights and a uniform
+translation (``motion_coefficients``); the fused loaders form
+r_n(t) = r0_n + sum_r modes[n, :, r] w_r(t) + trans(t) on the device.
 
-* ``RespiratoryMotion`` -- Motion.py:15-109: a spline of one scalar times a
-  direction, evaluated at fold(t + timeshift).
-* ``POD`` -- Motion.py:112-261: snapshot POD (``calculate_pod`` :174-217),
-  per-mode PCHIP/Akima/cubic splines batched into one ``PPoly`` with
-  ``extrapolate=False`` (:129-137), trajectory = modes . weights(fold(t+shift))
-  (:237-254).
-* ``PODVelocity`` -- Motion.py:264-290: weights scaled by the unshifted t.
-* ``PODSum`` -- Motion.py:293-317: sum of two trajectories.
+Two families of objects map to that form:
 
-On the device the trajectory is never formed per node on the host: a
-motion object becomes ``modes`` -- (3R, N) float32 rows -- plus, per
-evaluation time, R mode weights and one uniform translation
-(``motion_coefficients``).  The fused Signal and Bloch loaders then form
-r_n(t) = r0_n + sum_r modes_r[n] w_r(t) + trans(t).  ``motion_modes`` and
-``motion_coefficients`` also accept the reference's own objects (duck-typed
-on ``_modes``, ``_pp_batch``, ``_fold_time``, ``timeshift``, ``pod1/pod2``,
-``interpolator``/``direction``).
+* the reference's own motion objects (python/feelmri/Motion.py: ``POD``
+  :112-261, ``PODVelocity`` :264-290, ``PODSum`` :293-317,
+  ``RespiratoryMotion`` :15-109), read by duck typing on the attributes
+  their evaluation uses (``_modes``, ``_pp_batch``, ``_fold_time``,
+  ``timeshift``, ``pod1``/``pod2``, ``interpolator``/``direction``), so a
+  user keeps the reference's classes and passes them as they are;
+* this module's classes, which implement ``modal_form()`` /
+  ``coefficient_rows(times)`` directly:
+
+  - ``ModalMotion``: a field sampled at snapshot times (displacement, or
+    velocity scaled by t), reduced to its R leading modes by a truncated SVD
+    of the (3P x T) snapshot matrix, with one vector-valued interpolant of
+    the R coefficient series in time (PCHIP, Akima or cubic), folded into
+    the period for periodic motion and NaN outside the snapshot span
+    otherwise;
+  - ``TranslationMotion``: a rigid translation, a scalar waveform times a
+    unit direction;
+  - ``MotionSum``: the sum of two motions.
+
+The modes and coefficients are rounded to float32 as the reference's
+``_modes`` / ``_weights`` arrays are, so both families give the device the
+same numbers for the same field.
 """
 from __future__ import annotations
 
-import time
 from collections.abc import Callable
-from typing import Literal
 
 import numpy as np
-from scipy.interpolate import Akima1DInterpolator, CubicSpline, PchipInterpolator, PPoly
+from scipy.interpolate import Akima1DInterpolator, CubicSpline, PchipInterpolator
 
-_INTERP = {"AkimaSpline": Akima1DInterpolator, "CubicSpline": CubicSpline, "Pchip": PchipInterpolator}
-
-
-def _folder(period):
-    if period is None:
-        return lambda x: x
-
-    def _fold(x, T=period):
-        return x - T * np.floor(x / T)
-
-    return _fold
+_INTERPOLANTS = ("pchip", "akima", "cubic")
 
 
-class RespiratoryMotion:
-    def __init__(self, times: np.ndarray, data: np.ndarray, timeshift: float = 0.0,
-                 is_periodic: bool = False, remove_mean: bool = False,
-                 direction: np.ndarray = np.array([1, 0, 0], dtype=np.float32),
-                 interpolation_method: Literal["AkimaSpline", "CubicSpline", "Pchip"] = "Pchip"):
-        self.times = np.asarray(times).astype(np.float32)
-        self.data = np.asarray(data).astype(np.float32)
-        self.timeshift = timeshift
-        self.is_periodic = is_periodic
-        self.remove_mean = remove_mean
-        d = np.asarray(direction, dtype=np.float64).reshape((1, 3))
-        self.direction = (d / np.linalg.norm(d)).astype(np.float32)
-        self.interpolation_method = interpolation_method
-        self.interpolator = self.calculate_interpolator()
-        self._period = self.times[-1] if self.is_periodic else None
-        self._fold_time = _folder(self._period)
+def _interpolant(kind: str, times: np.ndarray, values: np.ndarray, natural: bool = False):
+    """Interpolant of ``values`` (T, ...) over ``times`` along axis 0; NaN
+    outside [times[0], times[-1]]."""
+    if kind == "pchip":
+        return PchipInterpolator(times, values, axis=0, extrapolate=False)
+    if kind == "akima":
+        return Akima1DInterpolator(times, values, axis=0, extrapolate=False)
+    if kind == "cubic":
+        return CubicSpline(times, values, axis=0, bc_type="natural" if natural else "not-a-knot",
+                           extrapolate=False)
+    raise ValueError(f"interpolation must be one of {_INTERPOLANTS}, got {kind!r}")
+
+
+class _Clock:
+    """Time shift and optional fold into [0, period) shared by the motions."""
+
+    def __init__(self, period: float | None, timeshift: float):
+        self.period = None if period is None else float(period)
+        self.timeshift = float(timeshift)
+
+    def local(self, t: float) -> float:
+        u = float(t) + self.timeshift
+        if self.period is not None:
+            u -= self.period * np.floor(u / self.period)
+        return u
+
+    def update_timeshift(self, timeshift: float) -> None:
+        self.timeshift = float(timeshift)
+
+
+class ModalMotion:
+    """Snapshot field reduced to ``rank`` spatial modes.
+
+    ``snapshots`` (P, 3, T) sampled at ``times`` (T,) in ms: a displacement
+    in m, or with ``velocity=True`` a velocity in m/ms whose evaluation is
+    scaled by the (unshifted) time, as the reference's ``PODVelocity`` does.
+    ``periodic``: fold times into [0, times[-1]).
+
+    Decomposition: X = snapshots as (3P, T) = U S V^T (thin SVD); the modes
+    are the first ``rank`` columns of U and the coefficient series at the
+    snapshot times are the first ``rank`` rows of S V^T, so the field at a
+    snapshot time is the rank-R projection of its snapshot.
+    """
+
+    def __init__(self, times, snapshots, rank: int = 5, periodic: bool = False, velocity: bool = False,
+                 interpolation: str = "pchip", timeshift: float = 0.0):
+        times = np.asarray(times, dtype=np.float64).ravel()
+        X = np.asarray(snapshots, dtype=np.float64)
+        if X.ndim != 3 or X.shape[1] != 3 or X.shape[2] != times.size:
+            raise ValueError(f"snapshots must be (P, 3, T) with T = {times.size}, got {X.shape}")
+        if not 1 <= rank <= times.size:
+            raise ValueError(f"rank must be in [1, {times.size}]")
+        P, T = X.shape[0], times.size
+        U, S, Vt = np.linalg.svd(X.reshape(3 * P, T), full_matrices=False)
+        self.rank = int(rank)
+        self.times = times
+        self.singular_values = S
+        self.modes = np.ascontiguousarray(U[:, :rank].reshape(P, 3, rank), dtype=np.float32)
+        self.series = (S[:rank, None] * Vt[:rank]).T            # (T, R) coefficients at the snapshots
+        self.velocity = bool(velocity)
+        self.interpolation = interpolation
+        self._interp = _interpolant(interpolation, times, self.series)
+        self.clock = _Clock(times[-1] if periodic else None, timeshift)
+
+    @property
+    def timeshift(self) -> float:
+        return self.clock.timeshift
+
+    def update_timeshift(self, timeshift: float) -> None:
+        self.clock.update_timeshift(timeshift)
+
+    def coefficients(self, t: float) -> np.ndarray:
+        """(R,) float32 mode weights at time ``t``."""
+        w = np.asarray(self._interp(self.clock.local(t)), dtype=np.float64).astype(np.float32)
+        if self.velocity:
+            w = (w * np.float32(t)).astype(np.float32)
+        return w
+
+    def __call__(self, t: float) -> np.ndarray:
+        """(P, 3) float32 field at time ``t``."""
+        return np.tensordot(self.modes, self.coefficients(t), axes=([2], [0]))
 
     def __add__(self, other):
-        return PODSum(self, other)
+        return MotionSum(self, other)
 
-    def calculate_interpolator(self):
-        data = self.data
-        if self.remove_mean:
-            data = data - np.mean(self.data, axis=0, dtype=np.float32)
-        if self.interpolation_method not in _INTERP:
-            raise ValueError(f"Interpolation method '{self.interpolation_method}' not recognized. "
-                             "Choose from 'AkimaSpline', 'CubicSpline', or 'Pchip'.")
-        if self.interpolation_method == "CubicSpline":
-            return CubicSpline(self.times, data, bc_type="natural")
-        return _INTERP[self.interpolation_method](self.times, data)
+    # device form
+    def modal_form(self) -> np.ndarray:
+        return self.modes
 
-    def _evaluate_motion(self, t):
-        return self.interpolator(self._fold_time(t + self.timeshift)).astype(np.float32)
-
-    def __call__(self, t):
-        return self._evaluate_motion(t) * self.direction
-
-    def update_timeshift(self, timeshift):
-        self.timeshift = timeshift
+    def coefficient_rows(self, times) -> np.ndarray:
+        times = np.asarray(times, dtype=np.float64).ravel()
+        out = np.zeros((times.size, self.rank + 3), dtype=np.float64)
+        for i, t in enumerate(times.tolist()):
+            out[i, :self.rank] = self.coefficients(t)
+        return out
 
 
-class POD:
-    def __init__(self, times: np.ndarray, data: np.ndarray, global_to_local: np.ndarray = None,
-                 n_modes: int = 5, is_periodic: bool = False,
-                 interpolation_method: Literal["AkimaSpline", "CubicSpline", "Pchip"] = "Pchip",
-                 timeshift: float = 0.0):
-        self.times = np.asarray(times)
-        self.data = data
-        self.local_to_global_map = global_to_local
-        self.n_modes = n_modes
-        self.timeshift = timeshift
-        self.is_periodic = is_periodic
-        self.interpolation_method = interpolation_method
-        self.modes, self.weights = self.calculate_pod(remove_mean=False)
-        self.spline_coeffs = self.spline_fit()
-        x0 = self.spline_coeffs[0].x
-        if not all(np.array_equal(pp.x, x0) for pp in self.spline_coeffs):
-            raise ValueError("knots differ")
-        C = np.stack([pp.c for pp in self.spline_coeffs], axis=-1)
-        self._pp_batch = PPoly(C, x0, extrapolate=False)
-        self._modes = np.asarray(self.modes, dtype=np.float32, order="C")
-        self._period = self.times[-1] if self.is_periodic else None
-        self._weights = np.zeros([self.n_modes], dtype=np.float32, order="C")
-        self._fold_time = _folder(self._period)
+class TranslationMotion:
+    """Rigid translation: ``waveform`` (T,) in m sampled at ``times`` times a
+    unit ``direction``; ``remove_mean`` subtracts the waveform's mean."""
 
-    def __repr__(self):
-        return (f"POD(n_modes={self.n_modes}, interpolation_method='{self.interpolation_method}', "
-                f"is_periodic={self.is_periodic})")
-
-    def __add__(self, other):
-        return PODSum(self, other)
-
-    def __call__(self, t):
-        return self._evaluate_trajectory(t)
-
-    def calculate_pod(self, remove_mean: bool = False):
-        """Snapshot POD (Motion.py:174-217): eigh of the t x t covariance."""
-        n_t = self.times.shape[0]
-        flat = np.asarray(self.data).reshape(-1, n_t)
+    def __init__(self, times, waveform, direction=(1.0, 0.0, 0.0), periodic: bool = False,
+                 remove_mean: bool = False, interpolation: str = "pchip", timeshift: float = 0.0):
+        times = np.asarray(times, dtype=np.float32).astype(np.float64).ravel()
+        wave = np.asarray(waveform, dtype=np.float32).ravel()
+        if wave.size != times.size:
+            raise ValueError("waveform and times differ in length")
         if remove_mean:
-            flat = flat - np.mean(flat, axis=1, keepdims=True)
-        cov = np.dot(flat.T, flat)
-        evals, evecs = np.linalg.eigh(cov)
-        idx = np.argsort(evals)[::-1][0:self.n_modes]
-        evals, evecs = evals[idx], evecs[:, idx]
-        phi = np.dot(flat, evecs / np.sqrt(evals).reshape(1, -1))
-        weights = np.einsum("pn, pt -> nt", flat, phi)
-        phi = phi.reshape((np.asarray(self.data).shape[0], -1, self.n_modes))
-        if self.local_to_global_map is not None:
-            phi = phi[self.local_to_global_map, :, :]
-        return phi, weights
+            wave = wave - np.mean(wave, dtype=np.float32)
+        d = np.asarray(direction, dtype=np.float64).reshape(3)
+        self.direction = (d / np.linalg.norm(d)).astype(np.float32)
+        self.times = times
+        self._interp = _interpolant(interpolation, times, wave.astype(np.float64), natural=True)
+        self.clock = _Clock(times[-1] if periodic else None, timeshift)
 
-    def spline_fit(self):
-        if self.interpolation_method not in _INTERP:
-            raise ValueError(f"Interpolation method '{self.interpolation_method}' not recognized. "
-                             "Choose from 'AkimaSpline', 'CubicSpline', or 'Pchip'.")
-        interp = _INTERP[self.interpolation_method]
-        return [interp(self.times, self.weights[:, i]) for i in range(self.n_modes)]
+    @property
+    def timeshift(self) -> float:
+        return self.clock.timeshift
 
-    def _evaluate_weights(self, t):
-        self._weights[:] = self._pp_batch(t).astype(self._weights.dtype, copy=False)
+    def update_timeshift(self, timeshift: float) -> None:
+        self.clock.update_timeshift(timeshift)
 
-    def _evaluate_trajectory(self, t):
-        self._evaluate_weights(self._fold_time(t + self.timeshift))
-        return np.tensordot(self._modes, self._weights, axes=([2], [0]))
+    def offset(self, t: float) -> np.ndarray:
+        """(3,) float32 translation at time ``t``."""
+        s = np.float32(np.asarray(self._interp(self.clock.local(t))).reshape(-1)[0])
+        return (s * self.direction).astype(np.float32)
 
-    def update_timeshift(self, timeshift):
-        self.timeshift = timeshift
+    def __call__(self, t: float) -> np.ndarray:
+        return self.offset(t).reshape(1, 3)
+
+    def __add__(self, other):
+        return MotionSum(self, other)
+
+    def modal_form(self) -> np.ndarray:
+        return np.zeros((0, 3, 0), dtype=np.float32)
+
+    def coefficient_rows(self, times) -> np.ndarray:
+        times = np.asarray(times, dtype=np.float64).ravel()
+        out = np.zeros((times.size, 3), dtype=np.float64)
+        for i, t in enumerate(times.tolist()):
+            out[i] = self.offset(t)
+        return out
 
 
-class PODVelocity(POD):
-    def _evaluate_trajectory(self, t):
-        self._evaluate_weights(self._fold_time(t + self.timeshift))
-        self._weights *= t
-        return np.tensordot(self._modes, self._weights, axes=([2], [0]))
+class MotionSum:
+    """a(t) + b(t); time shifts are set on both."""
 
+    def __init__(self, a, b: Callable):
+        self.a, self.b = a, b
 
-class PODSum:
-    def __init__(self, pod1, pod2: Callable):
-        self.pod1 = pod1
-        self.pod2 = pod2
-        self.timeshift = 0.0
+    def __call__(self, t: float) -> np.ndarray:
+        return self.a(t) + self.b(t)
 
-    def __call__(self, t):
-        return self.pod1(t) + self.pod2(t)
+    def __add__(self, other):
+        return MotionSum(self, other)
 
-    def update_timeshift(self, timeshift):
-        self.pod1.update_timeshift(timeshift)
-        self.pod2.update_timeshift(timeshift)
-        self.timeshift = timeshift
+    def update_timeshift(self, timeshift: float) -> None:
+        self.a.update_timeshift(timeshift)
+        self.b.update_timeshift(timeshift)
 
 
 # ---------------------------------------------------------------------------
-# Device descriptors
+# Device form of any supported motion (this module's or the reference's)
 # ---------------------------------------------------------------------------
 def _kind(obj) -> str:
+    if isinstance(obj, MotionSum):
+        return "own-sum"
+    if hasattr(obj, "coefficient_rows") and hasattr(obj, "modal_form"):
+        return "own"
+    # the reference's objects, by the attributes their evaluation uses
     if hasattr(obj, "pod1") and hasattr(obj, "pod2"):
         return "sum"
     if hasattr(obj, "_modes") and hasattr(obj, "_pp_batch"):
-        return "velocity" if type(obj).__name__ == "PODVelocity" or isinstance(obj, PODVelocity) else "pod"
+        return "velocity" if type(obj).__name__ == "PODVelocity" else "pod"
     if hasattr(obj, "interpolator") and hasattr(obj, "direction"):
         return "resp"
     return "other"
 
 
+def _parts(obj):
+    k = _kind(obj)
+    if k == "own-sum":
+        return obj.a, obj.b
+    return obj.pod1, obj.pod2
+
+
 def is_supported(obj) -> bool:
     """True when ``obj`` maps to modes + coefficients (no per-node host work)."""
     k = _kind(obj)
-    if k == "sum":
-        return is_supported(obj.pod1) and is_supported(obj.pod2)
-    return k in ("pod", "velocity", "resp")
+    if k in ("sum", "own-sum"):
+        return all(is_supported(p) for p in _parts(obj))
+    return k in ("own", "pod", "velocity", "resp")
 
 
 def motion_modes(obj) -> np.ndarray:
-    """(P, 3, R) float32 modes of a supported motion object (R may be 0)."""
+    """(P, 3, R) float32 modes of a supported motion (R may be 0)."""
     k = _kind(obj)
-    if k == "sum":
-        a, b = motion_modes(obj.pod1), motion_modes(obj.pod2)
+    if k in ("sum", "own-sum"):
+        a, b = (motion_modes(p) for p in _parts(obj))
         if a.shape[2] == 0:
             return b
         if b.shape[2] == 0:
             return a
         return np.concatenate([a, b], axis=2)
+    if k == "own":
+        return np.ascontiguousarray(obj.modal_form(), dtype=np.float32)
     if k in ("pod", "velocity"):
         return np.ascontiguousarray(obj._modes, dtype=np.float32)
     if k == "resp":
@@ -215,33 +256,39 @@ def motion_modes(obj) -> np.ndarray:
 
 def n_modes(obj) -> int:
     k = _kind(obj)
-    if k == "sum":
-        return n_modes(obj.pod1) + n_modes(obj.pod2)
+    if k in ("sum", "own-sum"):
+        return sum(n_modes(p) for p in _parts(obj))
+    if k == "own":
+        return int(np.asarray(obj.modal_form()).shape[2])
     if k in ("pod", "velocity"):
         return int(np.asarray(obj._modes).shape[2])
     return 0
 
 
 def motion_coefficients(obj, times) -> np.ndarray:
-    """(len(times), R + 3) float64: per time the mode weights (rounded to
-    float32 as the reference's ``_weights``) and a uniform translation."""
+    """(len(times), R + 3) float64: per time the mode weights (float32
+    values) followed by a uniform translation."""
     times = np.asarray(times, dtype=np.float64).ravel()
     k = _kind(obj)
     R = n_modes(obj)
     out = np.zeros((times.size, R + 3), dtype=np.float64)
-    if k == "sum":
-        a = motion_coefficients(obj.pod1, times)
-        b = motion_coefficients(obj.pod2, times)
-        ra = n_modes(obj.pod1)
+    if k in ("sum", "own-sum"):
+        pa, pb = _parts(obj)
+        a, b = motion_coefficients(pa, times), motion_coefficients(pb, times)
+        ra = n_modes(pa)
         out[:, :ra] = a[:, :ra]
-        out[:, ra:R] = b[:, ra:-3]
+        out[:, ra:R] = b[:, :R - ra]
         out[:, R:] = a[:, -3:] + b[:, -3:]
         return out
+    if k == "own":
+        return np.asarray(obj.coefficient_rows(times), dtype=np.float64)
     if k in ("pod", "velocity"):
+        # the reference evaluates its batched PPoly at fold(t + timeshift)
+        # into a float32 array, scaled in place by t for velocities
         for i, t in enumerate(times.tolist()):
             w = np.asarray(obj._pp_batch(obj._fold_time(t + obj.timeshift))).astype(np.float32)
             if k == "velocity":
-                w *= t  # Python float: float32 arithmetic, as the reference's in-place scale
+                w *= t
             out[i, :R] = w
         return out
     if k == "resp":

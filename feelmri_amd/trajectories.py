@@ -50,11 +50,17 @@ def cartesian(fov, res, oversampling: int = 1, t_first: float = 0.0, t_readout: 
 
 
 def radial(fov: float, n_samples: int, n_spokes: int, t_first: float = 0.0,
-           t_readout: float = 2.56, dtype=np.float64):
-    """2-D golden-angle half-spoke radial (KSpaceTraj.py:335-368), one slice."""
-    # KSpaceTraj.py: oversampling 2 along the readout, kx from 0 to its extent
-    k_spa = 1.0 / (2.0 * fov)
-    kmax = (n_samples - 1 - n_samples // 2) * k_spa * 2.0
+           t_readout: float = 2.56, oversampling: int = 2, dtype=np.float64):
+    """2-D golden-angle half-spoke radial (KSpaceTraj.py:335-368), one slice.
+
+    ``n_samples`` readout samples = ``oversampling`` x the image resolution
+    (the reference's default oversampling is 2, KSpaceTraj.py:16); the
+    spoke runs from k = 0 to the readout extent of KSpaceTraj.py:42,
+    (n_samples - 1 - n_samples // 2) / (oversampling FOV), plus one step for
+    an odd resolution."""
+    res0 = n_samples // oversampling
+    k_spa = 1.0 / (oversampling * fov)
+    kmax = (n_samples - 1 - n_samples // 2) * k_spa + float(res0 % 2 != 0) * k_spa
     kr = np.linspace(0.0, kmax, n_samples)
     n = np.arange(n_spokes)
     theta = np.mod((2.0 * np.pi - 2.0 * np.pi / GOLDEN_RATIO) * n, 2.0 * np.pi)

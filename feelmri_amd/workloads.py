@@ -77,8 +77,8 @@ class FlowWorkload(Workload):
     n_phases: int = 20
     period: float = 1000.0                 # ms
     venc: float = 1.5                      # m/s
-    motion: object = None                  # POD (displacement), R modes
-    velocity: object = None                # PODVelocity (Poiseuille field)
+    motion: object = None                  # ModalMotion displacement, R modes
+    velocity: object = None                # ModalMotion velocity (Poiseuille field)
     enc_blocks: list = field(default_factory=list)   # one Block per encoding
 
     @property
@@ -154,8 +154,8 @@ def c3(scale: float = 1.0, n_phases: int = 20, n_modes: int = 10, kspace=(96, 96
     wave = np.maximum(0.0, np.sin(2 * np.pi * times / period)) ** 2 + 0.1
     speed = (1.0 - np.clip(rho2, 0, 1))[:, None] * wave[None, :]          # m/s
     vel = tangent[:, :, None] * speed[:, None, :] * 1e-3                    # m/ms
-    # the field is one spatial profile times a waveform: one POD mode holds it
-    velocity = Motion.PODVelocity(times=times, data=vel, n_modes=1, is_periodic=True)
+    # the field is one spatial profile times a waveform: one mode holds it
+    velocity = Motion.ModalMotion(times, vel, rank=1, periodic=True, velocity=True)
     # node displacement from random smooth modes with periodic weights
     base = _smooth_modes(m.nodes, n_modes, rng)
     n_modes = min(n_modes, max(1, n_phases // 2))  # harmonics below Nyquist of the phase grid
@@ -164,8 +164,8 @@ def c3(scale: float = 1.0, n_phases: int = 20, n_modes: int = 10, kspace=(96, 96
     coef = rng.normal(size=(n_modes, 2, h.size))
     a = (coef[:, 0, :, None] * np.cos(2 * np.pi * h[None, :, None] * times / period) +
          coef[:, 1, :, None] * np.sin(2 * np.pi * h[None, :, None] * times / period)).sum(1)
-    disp = np.einsum("pcr,rt->pct", base, a)
-    motion = Motion.POD(times=times, data=disp, n_modes=n_modes, is_periodic=True)
+    disp = np.tensordot(base, a, axes=([2], [0]))
+    motion = Motion.ModalMotion(times, disp, rank=n_modes, periodic=True)
     # encodings: reference + bipolar along x, y, z; VENC 1.5 m/s
     venc = 1.5
     ramp, flat = 0.1, 0.4

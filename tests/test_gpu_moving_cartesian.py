@@ -1,7 +1,7 @@
 """Moving nodes on Cartesian stacks: the per-readout GEMMs (mixed form on bf16
 MFMA, FP64 form on FP64 MFMA) against the FP64 oracle and the direct kernels.
 
-With POD motion the positions change with t (MRI.cpp:192-217); on a Cartesian
+With POD motion the positions change with t (MRI.cpp:109-134); on a Cartesian
 stack t depends on the readout index only, so each readout's samples share
 one set of positions and the sum over nodes is a (lines x nodes) by
 (nodes x kz*V) product that both forms run on the matrix cores with
@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
+import motion_fixtures
 from feelmri_amd import Motion, trajectories
 
 from test_gpu_signal import _nodes, _pod, descriptor_callable, rel_l2
@@ -76,15 +77,14 @@ def test_shapes(gpu, shape, v):
 @pytest.mark.parametrize("n", [1, 17, 191, 192, 193, 4097])
 def test_ragged_node_counts(gpu, n):
     kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (8, 96, 4), t_first=0.4)
-    _run(kloc, t, n, 2, _pod(n, Motion.PODVelocity, seed=n))
+    _run(kloc, t, n, 2, _pod(n, velocity=True, seed=n, family="ref" if n % 2 else "own"))
 
 
 def test_translation_only(gpu):
     """A motion without modes (uniform translation): the R = 0 path."""
     kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (10, 32, 4), t_first=0.4)
-    resp = Motion.RespiratoryMotion(times=np.linspace(0, 100, 25), data=np.sin(np.linspace(0, 6, 25)) * 0.004,
-                                    is_periodic=True, direction=np.array([0, 1, 0]))
-    _run(kloc, t, 2000, 1, resp)
+    for family in ("own", "ref"):
+        _run(kloc, t, 2000, 1, motion_fixtures.resp(direction=(0, 1, 0), family=family))
 
 
 def test_readouts_sharing_t(gpu):

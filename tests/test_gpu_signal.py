@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sp
 
+import motion_fixtures
 from feelmri_amd import Motion, trajectories
 
 pytestmark = pytest.mark.gpu
@@ -209,30 +210,28 @@ def test_float32_inputs_use_float_overload(gpu):
     assert rel_l2(out, ref) <= TOL_MIXED
 
 
-def _pod(n, cls=Motion.POD, seed=11):
-    rng = np.random.default_rng(seed)
-    nt = 10
-    times = np.linspace(0, 40.0, nt)
-    base = rng.normal(size=(n, 3, 3)) * 1e-3
-    coeff = np.stack([np.sin(2 * np.pi * times / 40.0 * (k + 1)) for k in range(3)], 0)
-    data = np.einsum("pcr,rt->pct", base, coeff)
-    return cls(times=times, data=data, n_modes=3, is_periodic=True)
+def _pod(n, velocity=False, seed=11, family="own"):
+    return motion_fixtures.pod(n, seed=seed, velocity=velocity, family=family)
 
 
-@pytest.mark.parametrize("kind", ["pod", "velocity", "resp", "sum", "callable"])
+@pytest.mark.parametrize("kind", ["pod", "velocity", "resp", "sum", "callable", "ref-pod", "ref-velocity",
+                                  "ref-sum"])
 def test_moving_nodes(gpu, kind):
+    """Own motion classes, reference-shaped objects (duck-typed) and a generic
+    callable (sampled at the distinct t on the host)."""
     n = 2500
     kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (12, 6, 2), t_first=0.4)
-    resp = Motion.RespiratoryMotion(times=np.linspace(0, 100, 25), data=np.sin(np.linspace(0, 6, 25)) * 0.004,
-                                    is_periodic=True, direction=np.array([0, 1, 0]))
-    if kind == "pod":
-        m = _pod(n)
-    elif kind == "velocity":
-        m = _pod(n, Motion.PODVelocity)
-    elif kind == "resp":
+    family = "ref" if kind.startswith("ref-") else "own"
+    base = kind.split("-")[-1]
+    resp = motion_fixtures.resp(direction=(0, 1, 0), family=family)
+    if base == "pod":
+        m = _pod(n, family=family)
+    elif base == "velocity":
+        m = _pod(n, velocity=True, family=family)
+    elif base == "resp":
         m = resp
-    elif kind == "sum":
-        m = resp + _pod(n)
+    elif base == "sum":
+        m = resp + _pod(n, family=family)
     else:
         pod = _pod(n)
         m = lambda tt: pod(tt) * 1.5  # noqa: E731

@@ -1,0 +1,121 @@
+"""Pins for the host producers that feed the kernels: the event grid of a
+sequence block (python/feelmri/Bloch.py:105-129, :340), the Cartesian and
+golden-angle radial k-space extents (python/feelmri/KSpaceTraj.py:40-44,
+:183-220, :335-379) and the spiral's rotation of interleaves.
+
+Each is checked by what the definitions fix, not by retyping them: grid
+membership of every knot, the half-open dt grid, dt = diff(prepend 0), the
+integrated RF area against the nominal flip; the centre sample at k = 0, the
+sample spacing 1/(oversampling FOV), the golden-angle increment
+2 pi / phi^2 between spokes and the spoke radii.
+"""
+import numpy as np
+import pytest
+
+from feelmri_amd import sequences, trajectories
+
+
+def test_block_grid_contains_knots_and_half_open_dt_grid():
+    p = sequences.HardPulse(start=2.0, dur=0.3, flip=np.deg2rad(20))
+    g = sequences.Trapezoid(axis=1, start=2.5, amp=10.0, ramp=0.1, flat=0.4)
+    b = sequences.make_block(2.0, 3.0, pulses=[p], gradients=[g], dt_rf=0.05, dt_gr=0.1, dt=1.0)
+    t = b.times
+    assert np.all(np.diff(t) > 0)                                   # sorted, unique
+    for knot in [2.0, 2.3, *g.timings]:                              # RF start/end, gradient knots
+        assert np.min(np.abs(t - knot)) < 1e-12, knot
+    # the block dt grid starts at the extent and excludes its end (arange)
+    assert np.min(np.abs(t - 3.0)) < 1e-12 and np.min(np.abs(t - 4.0)) < 1e-12
+    assert np.min(np.abs(t - 5.0)) > 0.5                            # extent end 5.0 is not on the grid
+    # dt = diff(times, prepend=0): the first step is the absolute start time
+    assert b.dt[0] == t[0] == 2.0
+    np.testing.assert_allclose(np.cumsum(b.dt), t, rtol=0, atol=1e-12)
+    # grid spacing inside the pulse and the gradient
+    inside = (t > 2.0) & (t <= 2.3)
+    assert np.all(np.diff(t[(t >= 2.0) & (t <= 2.3)]) <= 0.05 + 1e-12) and inside.sum() >= 6
+    assert np.all(np.diff(t[(t >= 2.5) & (t <= g.timings[-1])]) <= 0.1 + 1e-12)
+
+
+def test_block_rf_area_is_the_flip_and_gradient_follows_trapezoid():
+    flip = np.deg2rad(15.0)
+    p = sequences.HardPulse(start=0.0, dur=0.2, flip=flip)
+    b = sequences.make_block(0.0, 0.2, pulses=[p], dt_rf=0.01, dt=1.0)
+    # step i integrates over (t_{i-1}, t_i] with rf(t_i): the hard pulse's
+    # B1 times its duration, so gamma * sum |rf| dt is the flip angle
+    np.testing.assert_allclose(sequences.GAMMA * np.sum(np.abs(b.rf[1:]) * b.dt[1:]), flip, rtol=1e-12)
+    assert b.rf[0] == 0                                              # (start, end]: off at the start knot
+    g = sequences.Trapezoid(axis=2, start=1.0, amp=-7.0, ramp=0.2, flat=0.6)
+    b2 = sequences.make_block(1.0, 1.0, gradients=[g], dt_gr=0.05, dt=1.0)
+    area = np.sum(b2.G[1:, 2] * b2.dt[1:])                           # right-point rule on a piecewise-linear G
+    np.testing.assert_allclose(area, -7.0 * (0.6 + 0.2), rtol=0.02)
+    assert b2.G[:, 0].max() == 0 and b2.G[:, 1].max() == 0
+
+
+def test_delay_block():
+    b = sequences.delay_block(5.0, 3.0, dt=1.0)
+    assert b.empty and not b.store
+    np.testing.assert_allclose(b.times, [5.0, 6.0, 7.0])            # end excluded
+    assert not np.any(b.rf) and not np.any(b.G)
+
+
+def test_spoiled_gre_blocks():
+    blocks = sequences.spoiled_gre(flip_deg=15, TR=5.0, n_dummy=3, n_lines=2, rf_dur=0.2)
+    assert len(blocks) == 10
+    assert [b.store for b in blocks[::2]] == [False, False, False, True, True]
+    assert all(b.empty for b in blocks[1::2]) and not any(b.empty for b in blocks[::2])
+    starts = [b.t0 for b in blocks[::2]]
+    np.testing.assert_allclose(np.diff(starts), 5.0)
+    # each TR (imaging + delay block) ends where the next starts
+    for img, dly, nxt in zip(blocks[::2], blocks[1::2], blocks[2::2]):
+        assert abs(dly.info["extent"][1] - nxt.t0) < 1e-9
+
+
+@pytest.mark.parametrize("res,ofac", [((64, 48, 8), 1), ((40, 30, 5), 2), ((33, 17, 3), 1)])
+def test_cartesian_extents(res, ofac):
+    fov = (0.24, 0.20, 0.06)
+    (kx, ky, kz), t = trajectories.cartesian(fov, res, oversampling=ofac, t_first=1.0, t_readout=2.0)
+    ro = ofac * res[0]
+    assert kx.shape == (ro, res[1], res[2])
+    dk = 1.0 / (np.array([ofac, 1, 1]) * np.array(fov))
+    np.testing.assert_allclose(np.diff(kx[:, 0, 0]), dk[0], rtol=1e-12)
+    np.testing.assert_allclose(np.diff(ky[0, :, 0]), dk[1], rtol=1e-12)
+    if res[2] > 1:
+        np.testing.assert_allclose(np.diff(kz[0, 0, :]), dk[2], rtol=1e-12)
+    # centre sample: k = 0 at index n // 2 (readout: plus one step for an odd resolution)
+    assert abs(kx[ro // 2, 0, 0] - (dk[0] if res[0] % 2 else 0.0)) < 1e-9 * dk[0]
+    assert abs(ky[0, res[1] // 2, 0]) < 1e-9 * dk[1]
+    assert abs(kz[0, 0, res[2] // 2]) < 1e-9 * dk[2]
+    # every line and slice has the same readout times, t_first .. t_first + t_readout
+    np.testing.assert_allclose(t, np.broadcast_to(t[:, :1, :1], t.shape))
+    np.testing.assert_allclose(t[[0, -1], 0, 0], [1.0, 3.0])
+
+
+@pytest.mark.parametrize("n,ofac", [(512, 2), (256, 2), (129, 1)])
+def test_radial_golden_angle_and_extent(n, ofac):
+    fov = 0.24
+    (kx, ky, kz), t = trajectories.radial(fov, n, 13, oversampling=ofac)
+    kx, ky = kx[..., 0], ky[..., 0]
+    assert not np.any(kz)
+    r = np.hypot(kx, ky)
+    res0 = n // ofac
+    kmax = ((n - 1 - n // 2) + (res0 % 2)) / (ofac * fov)
+    np.testing.assert_allclose(r[:, 0], np.linspace(0, kmax, n), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(r, np.broadcast_to(r[:, :1], r.shape), rtol=1e-12)
+    # spoke angle: k = kr (cos theta, -sin theta), theta_n = n * 2 pi / phi^2 (mod 2 pi),
+    # the golden angle 137.5078 deg
+    theta = np.mod(-np.arctan2(ky[-1], kx[-1]), 2 * np.pi)
+    golden = 2 * np.pi / ((1 + np.sqrt(5)) / 2) ** 2
+    assert abs(np.rad2deg(golden) - 137.50776) < 1e-4
+    want = np.mod(np.arange(13) * golden, 2 * np.pi)
+    d = np.angle(np.exp(1j * (theta - want)))
+    assert np.abs(d).max() < 1e-9
+
+
+def test_spiral_interleaves_are_rotations():
+    (kx, ky, _), t = trajectories.spiral(0.24, 256, 1024, 8, density_exponent=2.0)
+    K = kx[..., 0] + 1j * ky[..., 0]
+    for m in range(1, 8):
+        np.testing.assert_allclose(K[:, m], K[:, 0] * np.exp(2j * np.pi * m / 8), rtol=1e-12, atol=1e-9)
+    # variable density: radius kmax u^2 reaches 0.5 res / FOV at the last sample
+    np.testing.assert_allclose(np.abs(K[-1, 0]), 0.5 * 256 / 0.24, rtol=1e-12)
+    u = (len(K) // 2) / (len(K) - 1)
+    np.testing.assert_allclose(np.abs(K[len(K) // 2, 0]) / np.abs(K[-1, 0]), u ** 2, rtol=1e-9)

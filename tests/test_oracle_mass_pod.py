@@ -4,14 +4,19 @@ classes of feelmri_amd.Motion (host semantics).
 Mass: element matrices equal the closed form |T|/20 (1 + delta_ij); lumped
 weights sum to the mesh volume and equal h^3 at interior nodes of a Kuhn
 lattice; consistent row sums equal the lumped weights.
-POD: the contraction equals numpy.tensordot; the spline weights equal
-scipy's interpolators; fold, NaN outside the knots and the velocity scaling
-follow python/feelmri/Motion.py.
+POD: the contraction equals numpy.tensordot.
+Motion: ModalMotion reconstructs rank-R snapshots exactly with orthonormal
+modes and is the best rank-R fit otherwise; its coefficients are scipy's
+PCHIP of the snapshot projections, folded by the period, NaN outside the
+snapshots when not periodic; velocities scale by t (python/feelmri/
+Motion.py:264-290); the device tables reproduce every object's evaluation,
+for this package's classes and reference-shaped ones.
 """
 import numpy as np
 import pytest
 from scipy.interpolate import PchipInterpolator
 
+import motion_fixtures
 from feelmri_amd import Motion, phantoms
 
 
@@ -52,53 +57,106 @@ def test_tensordot_matches_numpy(oracle):
                                rtol=1e-13)
 
 
-def _pod(periodic=True, cls=Motion.POD):
-    rng = np.random.default_rng(2)
-    P, nt = 40, 12
-    times = np.linspace(0, 55.0, nt)
-    base = rng.normal(size=(P, 3, 3))
-    coeff = np.stack([np.sin(2 * np.pi * times / 55.0 * (k + 1)) for k in range(3)], 0)
-    data = np.einsum("pcr,rt->pct", base, coeff) + 0.01 * rng.normal(size=(P, 3, nt))
-    return cls(times=times, data=data, n_modes=3, is_periodic=periodic), times
+def test_modal_motion_reconstructs_rank_r_snapshots():
+    """A rank-3 field is reproduced exactly at every snapshot time by 3
+    modes; the modes are orthonormal; the coefficient series are the
+    projections of the snapshots on the modes (the SVD of the (3P, T)
+    snapshot matrix, which the reference's snapshot POD, Motion.py:174-217,
+    also computes through the T x T covariance)."""
+    times, data, _, _ = motion_fixtures.snapshots(60, seed=2, n_t=12, period=55.0)
+    m = Motion.ModalMotion(times, data, rank=3, periodic=True)
+    P = data.shape[0]
+    U = m.modes.astype(np.float64).reshape(3 * P, 3)
+    np.testing.assert_allclose(U.T @ U, np.eye(3), atol=1e-6)          # float32 modes
+    np.testing.assert_allclose(m.series, data.reshape(3 * P, -1).T @ U, rtol=1e-5, atol=1e-9)
+    assert m.singular_values[3] < 1e-12 * m.singular_values[0]           # rank 3 exactly
+    for k, t in enumerate(times[:-1]):
+        np.testing.assert_allclose(m(t), data[:, :, k], rtol=1e-4, atol=1e-8)
 
 
-def test_pod_weights_are_scipy_pchip():
-    pod, times = _pod()
-    for i in range(pod.n_modes):
-        pp = PchipInterpolator(times, pod.weights[:, i])
-        t = np.linspace(0, 55, 17)
-        np.testing.assert_allclose(pod._pp_batch(t)[:, i], pp(t), rtol=1e-12, atol=1e-12)
+def test_modal_motion_truncation_is_the_best_rank_r_fit():
+    """Truncating a rank-5 field to 2 modes leaves the residual of the
+    best rank-2 approximation (Eckart-Young): sqrt of the dropped singular
+    values' squares."""
+    rng = np.random.default_rng(4)
+    times = np.linspace(0, 10.0, 8)
+    data = np.tensordot(rng.normal(size=(30, 3, 5)), rng.normal(size=(5, 8)), axes=([2], [0]))
+    m = Motion.ModalMotion(times, data, rank=2, periodic=False)
+    X = data.reshape(90, 8)
+    U = m.modes.astype(np.float64).reshape(90, 2)
+    resid = np.linalg.norm(X - U @ (U.T @ X))
+    np.testing.assert_allclose(resid, np.sqrt((m.singular_values[2:] ** 2).sum()), rtol=1e-5)
 
 
-def test_pod_call_fold_and_nan():
-    pod, times = _pod(periodic=True)
-    pod.update_timeshift(3.0)
-    a = pod(10.0)
-    b = pod(10.0 + 55.0)  # folded back by the period
-    np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-7)
-    w = pod._pp_batch(13.0).astype(np.float32)
-    np.testing.assert_allclose(a, np.tensordot(pod._modes, w, axes=([2], [0])), rtol=1e-6)
-    nonper, _ = _pod(periodic=False)
-    assert np.all(np.isnan(nonper(80.0)))  # extrapolate=False
+def test_modal_motion_interpolation_fold_and_nan():
+    times, data, _, _ = motion_fixtures.snapshots(40, seed=3, n_t=12, period=55.0)
+    m = Motion.ModalMotion(times, data, rank=3, periodic=True)
+    t = np.linspace(0, 55, 17)
+    want = PchipInterpolator(times, m.series, axis=0)(t)
+    np.testing.assert_allclose(np.stack([m.coefficients(x) for x in t]), want, rtol=1e-6, atol=1e-9)
+    m.update_timeshift(3.0)
+    np.testing.assert_allclose(m(10.0), m(10.0 + 55.0), rtol=1e-6, atol=1e-9)   # folded by the period
+    np.testing.assert_allclose(m.coefficients(10.0), PchipInterpolator(times, m.series, axis=0)(13.0),
+                               rtol=1e-6, atol=1e-9)
+    nonper = Motion.ModalMotion(times, data, rank=3, periodic=False)
+    assert np.all(np.isnan(nonper(80.0)))                                         # outside the snapshots
 
 
-def test_pod_velocity_scales_by_t():
-    vel, _ = _pod(cls=Motion.PODVelocity)
-    pos, _ = _pod(cls=Motion.POD)
-    np.testing.assert_allclose(vel(7.5), pos(7.5) * 7.5, rtol=1e-5, atol=1e-5)
+def test_modal_velocity_scales_by_t():
+    vel = motion_fixtures.pod(40, seed=2, velocity=True)
+    pos = motion_fixtures.pod(40, seed=2)
+    np.testing.assert_allclose(vel(7.5), pos(7.5) * 7.5, rtol=1e-5, atol=1e-9)
 
 
-def test_descriptors_reproduce_call():
-    pod, _ = _pod()
-    resp = Motion.RespiratoryMotion(times=np.linspace(0, 100, 30), data=np.sin(np.linspace(0, 3, 30)) * 0.004,
-                                    is_periodic=True, direction=np.array([0, 1, 0]))
-    for obj in (pod, resp + pod, Motion.PODSum(pod, resp)):
+def test_translation_motion():
+    times = np.linspace(0, 100, 30)
+    wave = np.sin(np.linspace(0, 3, 30)) * 0.004
+    tr = Motion.TranslationMotion(times, wave, direction=(0, 3, 4), periodic=True)
+    np.testing.assert_allclose(tr(17.0).ravel(), PchipInterpolator(times, wave)(17.0) * np.array([0, 0.6, 0.8]),
+                               rtol=1e-6, atol=1e-10)
+    tr.update_timeshift(5.0)
+    np.testing.assert_allclose(tr(117.0), tr(17.0), atol=1e-10)
+
+
+@pytest.mark.parametrize("family", ["own", "ref"])
+def test_descriptors_reproduce_call(family):
+    """The device tables (float32 modes, per-time coefficient rows) give each
+    object's own evaluation, for this package's classes and for objects
+    shaped like the reference's (duck-typed)."""
+    pod = motion_fixtures.pod(40, seed=2, family=family)
+    vel = motion_fixtures.pod(40, seed=5, velocity=True, family=family)
+    resp = motion_fixtures.resp(direction=(0, 1, 0), family=family)
+    for obj in (pod, vel, resp, resp + pod, pod + vel):
+        assert Motion.is_supported(obj)
         obj.update_timeshift(2.0)
         ts = np.array([0.3, 4.0, 17.2])
         modes = Motion.motion_modes(obj)
         coef = Motion.motion_coefficients(obj, ts)
         R = Motion.n_modes(obj)
+        assert modes.shape == (40, 3, R) or (R == 0 and modes.shape[2] == 0)
         for i, t in enumerate(ts):
             want = obj(float(t))
             got = (np.tensordot(modes, coef[i, :R], axes=([2], [0])) if R else 0) + coef[i, R:]
-            np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-8)
+            np.testing.assert_allclose(np.broadcast_to(got, np.broadcast(got, want).shape),
+                                       np.broadcast_to(want, np.broadcast(got, want).shape), rtol=1e-5, atol=1e-9)
+
+
+def test_own_and_reference_shaped_give_the_same_field():
+    """A reference-shaped object holding ModalMotion's modes and coefficient
+    series evaluates to ModalMotion's field at every time; at the snapshot
+    times both equal the generating rank-3 field."""
+    from motion_fixtures import RefPOD
+
+    times, data, _, _ = motion_fixtures.snapshots(50, seed=9)
+    own = Motion.ModalMotion(times, data, rank=3, periodic=True)
+    ref = RefPOD(own.modes, times, own.series, periodic=True)
+    for t in (0.0, 3.3, 21.0, 39.0, 57.5):
+        np.testing.assert_allclose(own(t), ref(t), rtol=1e-6, atol=1e-10)
+    for k in range(times.size - 1):
+        np.testing.assert_allclose(own(times[k]), data[:, :, k], rtol=1e-4, atol=1e-8)
+
+
+def test_generic_callable_is_not_modal():
+    assert not Motion.is_supported(lambda t: np.zeros((3, 3)))
+    with pytest.raises(TypeError):
+        Motion.motion_modes(lambda t: np.zeros((3, 3)))
