@@ -54,7 +54,7 @@ _SIGNATURES = {
                        _p],
     "fm_mass_p1_color": [_i64, _i64, _p, _p, _p, _int, ctypes.POINTER(_int)],
     "fm_mass_p1_lumped": [_i64, _i64, _p, _int, _p, _p, _p, _int, _p, _p],
-    "fm_mass_p1_incidence": [_i64, _i64, _p, _p, _p, _p, _p],
+    "fm_mass_p1_incidence": [_i64, _i64, _p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)],
     "fm_mass_p1_lumped_gather": [_i64, _i64, _p, _int, _p, _p, _p, _p, _i64, _p, _p, _p],
     "fm_mass_p1_element_matrices": [_i64, _p, _int, _p, _p, _p],
     "fm_pod_tensordot": [_int, _i64, _i64, _i64, _p, _p, _p, _p],
@@ -93,12 +93,15 @@ def exported_symbols() -> list[str]:
     return list(_SIGNATURES)
 
 
+def check(rc: int, name: str) -> None:
+    """Raise for a non-zero status returned by entry point ``name``."""
+    if rc != FM_OK:
+        raise FeelmriHipError(name, rc, load().fm_last_error().decode(errors="replace"))
+
+
 def call(name: str, *args) -> None:
     lib = load()
-    rc = getattr(lib, name)(*args)
-    if rc != FM_OK:
-        msg = lib.fm_last_error().decode(errors="replace")
-        raise FeelmriHipError(name, rc, msg)
+    check(getattr(lib, name)(*args), name)
 
 
 def last_error() -> str:

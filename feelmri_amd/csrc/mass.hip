@@ -1,7 +1,7 @@
 // P1 tetrahedron mass: lumped weights by an atomic-free coloured scatter,
 // and the per-element consistent matrices.
 //
-// reference: cpp/feelmri/Assemble.cpp:54-155 (IsoMapping, LocalMassAssemble,
+// reference: cpp/feelmri/Assemble.cpp:15-116 (IsoMapping, LocalMassAssemble,
 // MassAssemble) and the lumping in python/feelmri/Phantom.py:379-399.  For
 // P1 tetrahedra the quadrature (exact at order 2) reduces to closed forms:
 // M^e_ij = |T_e|/20 (1 + delta_ij) and the row sums w_i = sum_{e ni i} |T_e|/4.
@@ -13,14 +13,18 @@
 // reproducible.
 //
 // The gather form does the same sums in two launches over a host-built
-// incidence in sliced-ELL layout: node n's j-th incident element in colour
-// order owns slot chunk_offsets[n/64] + 64 j + n%64.  Launch 1, per element
-// in element order (coalesced connectivity, local coordinate reads), writes
-// |det J_e| = 6 |T_e| into its four vertex slots (each slot has one writer).
-// Launch 2, per node, adds its deg(n) slots: a wave reads 64 consecutive
-// values per step.  A node has at most one element per colour, so it
-// performs the scatter's additions in the scatter's order; both forms add
-// with an explicit fma(|det J_e|, 1/24, w), so they give the same bits.
+// incidence table in sliced-ELL layout: node n's j-th incident element in
+// colour order is stored (as an element id) in slot chunk_offsets[n/64] +
+// 64 j + n%64.  Launch 1, per element in element order, writes |det J_e| =
+// 6 |T_e| once into a per-element array (coalesced connectivity reads and
+// stores).  Launch 2, per node, reads its deg(n) element ids -- a wave reads
+// 64 consecutive ids per step -- and adds the elements' determinants.  A
+// node has at most one element per colour, so it performs the scatter's
+// additions in the scatter's order; both forms add with an explicit
+// fma(|det J_e|, 1/24, w), so they give the same bits.  The determinants a
+// node chunk gathers belong to elements that share its nodes, which sit in a
+// few short runs of element ids on a mesh numbered along its grid, so the
+// gathers hit lines that neighbouring nodes fetch too.
 #include <algorithm>
 #include <vector>
 
@@ -66,30 +70,32 @@ __global__ void k_lumped_colour(int64_t begin, int64_t end, const int32_t *__res
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_abs_dets(int64_t E, const int32_t *__restrict__ elems,
-                                                  const T *__restrict__ nodes, const int32_t *__restrict__ vertex_slots,
-                                                  double *__restrict__ slot_vals)
+                                                  const T *__restrict__ nodes, double *__restrict__ dets)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     int32_t ids[4];
-    const double det = tet_abs_det<T>(elems + 4 * e, nodes, ids);
-    const int4 sl = reinterpret_cast<const int4 *>(vertex_slots)[e];
-    slot_vals[sl.x] = det;
-    slot_vals[sl.y] = det;
-    slot_vals[sl.z] = det;
-    slot_vals[sl.w] = det;
+    dets[e] = tet_abs_det<T>(elems + 4 * e, nodes, ids);
 }
 
-__global__ __launch_bounds__(256) void k_lumped_gather(int64_t n_nodes, const int64_t *__restrict__ chunk_offsets,
+// Elements with ids >= n_elems are skipped: a mesh whose elements are
+// ordered by their largest node id gives, for its first n_nodes nodes, the
+// sub-mesh of its first n_elems elements without a table of its own.
+__global__ __launch_bounds__(256) void k_lumped_gather(int64_t n_nodes, int64_t n_elems,
+                                                       const int64_t *__restrict__ chunk_offsets,
                                                        const int32_t *__restrict__ degree,
-                                                       const double *__restrict__ slot_vals, double *__restrict__ w)
+                                                       const int32_t *__restrict__ incident,
+                                                       const double *__restrict__ dets, double *__restrict__ w)
 {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
-    const double *v = slot_vals + chunk_offsets[n / 64] + n % 64;
+    const int32_t *ids = incident + chunk_offsets[n / 64] + n % 64;
     const int d = degree[n];
     double acc = 0.0;
-    for (int j = 0; j < d; ++j) acc = fma(v[64 * j], 1.0 / 24.0, acc);  // += |T|/4
+    for (int j = 0; j < d; ++j) {
+        const int32_t e = ids[64 * j];
+        if (e < n_elems) acc = fma(dets[e], 1.0 / 24.0, acc);  // += |T|/4
+    }
     w[n] = acc;
 }
 
@@ -190,10 +196,11 @@ extern "C" int fm_mass_p1_lumped(int64_t n_elems, int64_t n_nodes, const int32_t
 }
 
 extern "C" int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int32_t *elems, const int32_t *perm,
-                                    int64_t *chunk_offsets, int32_t *degree, int32_t *vertex_slots)
+                                    int64_t *chunk_offsets, int32_t *degree, int32_t *incident,
+                                    int64_t max_slots, int64_t *n_slots)
 {
     FM_CHECK_ARG(n_elems >= 0 && n_nodes >= 0, "negative size");
-    FM_CHECK_ARG(chunk_offsets && (n_nodes == 0 || degree) && (n_elems == 0 || (elems && perm && vertex_slots)),
+    FM_CHECK_ARG(chunk_offsets && n_slots && (n_nodes == 0 || degree) && (n_elems == 0 || (elems && perm)),
                  "NULL argument");
     for (int64_t n = 0; n < n_nodes; ++n) degree[n] = 0;
     for (int64_t e = 0; e < n_elems; ++e)
@@ -213,12 +220,16 @@ extern "C" int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int3
         for (int64_t n = 64 * c; n < std::min<int64_t>(n_nodes, 64 * c + 64); ++n) wmax = std::max(wmax, degree[n]);
         chunk_offsets[c + 1] = chunk_offsets[c] + 64 * (int64_t)wmax;
     }
-    if (chunk_offsets[n_chunks] > 0x7fffffffLL) {
-        set_error("incidence needs %lld slots, more than int32 slot indices address",
-                  (long long)chunk_offsets[n_chunks]);
-        return FM_ERR_UNSUPPORTED;
+    *n_slots = chunk_offsets[n_chunks];
+    if (incident == nullptr) return FM_OK;  // size query: chunk_offsets, degree and n_slots only
+    if (chunk_offsets[n_chunks] > max_slots) {
+        set_error("incidence needs %lld slots, the table holds %lld", (long long)chunk_offsets[n_chunks],
+                  (long long)max_slots);
+        return FM_ERR_ARG;
     }
-    // slots in perm (colour) order: node n's j-th element takes slot j
+    // slots in perm (colour) order: node n's j-th element takes slot j;
+    // padding slots (j >= degree) hold -1
+    for (int64_t q = 0; q < chunk_offsets[n_chunks]; ++q) incident[q] = -1;
     std::vector<int32_t> next((size_t)n_nodes, 0);
     std::vector<uint8_t> seen((size_t)n_elems, 0);
     for (int64_t k = 0; k < n_elems; ++k) {
@@ -230,7 +241,7 @@ extern "C" int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int3
         seen[(size_t)e] = 1;
         for (int i = 0; i < 4; ++i) {
             const int64_t n = elems[4 * e + i];
-            vertex_slots[4 * e + i] = (int32_t)(chunk_offsets[n / 64] + 64 * (int64_t)next[(size_t)n]++ + n % 64);
+            incident[chunk_offsets[n / 64] + 64 * (int64_t)next[(size_t)n]++ + n % 64] = (int32_t)e;
         }
     }
     return FM_OK;
@@ -238,29 +249,31 @@ extern "C" int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int3
 
 extern "C" int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const int32_t *elems, int dtype,
                                         const void *nodes, const int64_t *chunk_offsets, const int32_t *degree,
-                                        const int32_t *vertex_slots, int64_t n_slots, double *slot_scratch,
-                                        double *w, void *stream)
+                                        const int32_t *incident, int64_t n_slots, double *det_scratch, double *w,
+                                        void *stream)
 {
     FM_CHECK_ARG(n_elems >= 0 && n_nodes >= 0 && n_slots >= 0, "negative size");
     FM_CHECK_ARG(dtype == FM_DTYPE_F32 || dtype == FM_DTYPE_F64, "unknown dtype %d", dtype);
-    FM_CHECK_ARG(n_slots >= 4 * n_elems, "n_slots (%lld) below 4 n_elems", (long long)n_slots);
+    // every element has 4 incidence slots, so a table built for at least
+    // these elements by fm_mass_p1_incidence holds at least 4 n_elems
+    FM_CHECK_ARG(n_slots >= 4 * n_elems,
+                 "n_slots (%lld) below 4 n_elems: pass the slot count fm_mass_p1_incidence returned for this mesh",
+                 (long long)n_slots);
     if (n_nodes == 0) return FM_OK;
     FM_CHECK_ARG(w && chunk_offsets && degree, "NULL array");
     hipStream_t st = as_stream(stream);
     if (n_elems > 0) {
-        FM_CHECK_ARG(elems && nodes && vertex_slots && slot_scratch, "NULL array");
-        FM_CHECK_ARG(((uintptr_t)vertex_slots & 15) == 0, "vertex_slots must be 16-byte aligned");
+        FM_CHECK_ARG(elems && nodes && incident && det_scratch, "NULL array");
         FM_CHECK_ARG(n_elems <= 0x7fffffffLL * 256, "too many elements for one launch");
         const unsigned eblocks = (unsigned)cdiv(n_elems, 256);
         if (dtype == FM_DTYPE_F64)
-            k_abs_dets<double><<<eblocks, 256, 0, st>>>(n_elems, elems, (const double *)nodes, vertex_slots,
-                                                        slot_scratch);
+            k_abs_dets<double><<<eblocks, 256, 0, st>>>(n_elems, elems, (const double *)nodes, det_scratch);
         else
-            k_abs_dets<float><<<eblocks, 256, 0, st>>>(n_elems, elems, (const float *)nodes, vertex_slots,
-                                                       slot_scratch);
+            k_abs_dets<float><<<eblocks, 256, 0, st>>>(n_elems, elems, (const float *)nodes, det_scratch);
         FM_LAUNCH_CHECK("k_abs_dets");
     }
-    k_lumped_gather<<<(unsigned)cdiv(n_nodes, 256), 256, 0, st>>>(n_nodes, chunk_offsets, degree, slot_scratch, w);
+    k_lumped_gather<<<(unsigned)cdiv(n_nodes, 256), 256, 0, st>>>(n_nodes, n_elems, chunk_offsets, degree,
+                                                                  incident, det_scratch, w);
     FM_LAUNCH_CHECK("k_lumped_gather");
     return FM_OK;
 }

@@ -104,6 +104,7 @@ class SignalPlan:
                   tg.ctypes.data_as(ctypes.c_void_p))
         self.tgroup_t = tg
         self._t = tt
+        self.kloc, self.t = (kx, ky, kz), tt   # the host sampling, (n_meas, n_lines, n_kz) float64
         nr, rl, ex = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
         _lib.call("fm_signal_plan_runs", self._h, ctypes.byref(nr), ctypes.byref(rl), ctypes.byref(ex))
         # readout runs (k and t linear along every readout line): the mixed
@@ -167,11 +168,16 @@ class SignalPlan:
             if coef.dtype != torch.float64 or tuple(coef.shape) != (self.n_tgroups, R + 3):
                 raise ValueError("motion coefficients must be float64 of shape (n_tgroups, R+3)")
         need = self.workspace_bytes(prec, N, V, moving=R > 0)
-        if workspace is None or workspace.numel() < need:
+        own_ws = workspace is None or workspace.numel() < need
+        if own_ws:
             workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=self.device)
         with torch.cuda.device(self.device):
             _lib.call("fm_signal_execute", self._h, prec, N, V, ptr(nodes.packed), ptr(modes), R,
                       ptr(coef), ptr(workspace), workspace.numel(), ptr(out), stream_handle(stream))
+            if own_ws and isinstance(stream, torch.cuda.Stream) and stream != torch.cuda.current_stream():
+                # the workspace is freed on return: keep it from being reused
+                # before the kernels queued on `stream` have run
+                workspace.record_stream(stream)
         return out
 
 
@@ -364,7 +370,14 @@ def bloch_store_spoil(mxy_state, mz_state, mxy_store=None, mz_store=None, col: i
 # ---------------------------------------------------------------------------
 class P1Colouring:
     """Host greedy colouring of a tet mesh and each node's incident elements
-    in colour order, resident on the device."""
+    in colour order, resident on the device.
+
+    A mesh whose greedy colouring needs more than 64 colours (a node with
+    many incident elements, e.g. an unstructured Delaunay mesh) keeps the
+    elements in their own order instead (``method == "element-order"``):
+    the gather then adds each node's contributions in ascending element
+    order -- deterministic, but no longer bitwise equal to a coloured
+    scatter, which is unavailable for such a mesh (``n_colours == 0``)."""
 
     def __init__(self, elems, n_nodes: int, device=None):
         device = torch.device(device) if device is not None else default_device()
@@ -375,24 +388,34 @@ class P1Colouring:
         perm = np.empty(E, dtype=np.int32)
         offs = np.zeros(65, dtype=np.int64)
         nc = ctypes.c_int()
-        _lib.call("fm_mass_p1_color", E, int(n_nodes), el.ctypes.data_as(ctypes.c_void_p),
-                  perm.ctypes.data_as(ctypes.c_void_p), offs.ctypes.data_as(ctypes.c_void_p), 64,
-                  ctypes.byref(nc))
-        self.n_colours = nc.value
-        self.offsets = np.ascontiguousarray(offs[: self.n_colours + 1])
+        rc = _lib.load().fm_mass_p1_color(E, int(n_nodes), el.ctypes.data_as(ctypes.c_void_p),
+                                          perm.ctypes.data_as(ctypes.c_void_p),
+                                          offs.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nc))
+        if rc == _lib.FM_ERR_UNSUPPORTED:
+            perm = np.arange(E, dtype=np.int32)
+            self.method, self.n_colours = "element-order", 0
+            self.offsets = np.zeros(1, dtype=np.int64)
+        else:
+            _lib.check(rc, "fm_mass_p1_color")
+            self.method, self.n_colours = "colour-order", nc.value
+            self.offsets = np.ascontiguousarray(offs[: self.n_colours + 1])
         self.n_elems, self.n_nodes = E, int(n_nodes)
         chunk_offsets = np.empty((self.n_nodes + 63) // 64 + 1, dtype=np.int64)
         degree = np.empty(self.n_nodes, dtype=np.int32)
-        vertex_slots = np.empty(4 * E, dtype=np.int32)
-        _lib.call("fm_mass_p1_incidence", E, self.n_nodes, el.ctypes.data_as(ctypes.c_void_p),
-                  perm.ctypes.data_as(ctypes.c_void_p), chunk_offsets.ctypes.data_as(ctypes.c_void_p),
-                  degree.ctypes.data_as(ctypes.c_void_p), vertex_slots.ctypes.data_as(ctypes.c_void_p))
-        self.n_slots = int(chunk_offsets[-1])
+        n_slots = ctypes.c_int64()
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        # size query, then the table
+        _lib.call("fm_mass_p1_incidence", E, self.n_nodes, vp(el), vp(perm), vp(chunk_offsets), vp(degree), None,
+                  0, ctypes.byref(n_slots))
+        incident = np.empty(max(n_slots.value, 1), dtype=np.int32)
+        _lib.call("fm_mass_p1_incidence", E, self.n_nodes, vp(el), vp(perm), vp(chunk_offsets), vp(degree),
+                  vp(incident), incident.size, ctypes.byref(n_slots))
+        self.n_slots = int(n_slots.value)
         self.elems = torch.from_numpy(el).to(device)
         self.perm = torch.from_numpy(perm).to(device)
         self.chunk_offsets = torch.from_numpy(chunk_offsets).to(device)
         self.degree = torch.from_numpy(degree).to(device)
-        self.vertex_slots = torch.from_numpy(vertex_slots).to(device)
+        self.incident = torch.from_numpy(incident).to(device)
         self.device = device
 
 
@@ -408,14 +431,14 @@ def _mass_args(colouring: P1Colouring, nodes: torch.Tensor, out):
 
 def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None) -> torch.Tensor:
     """w_i = sum_{e ni i} |T_e| / 4 (float64, (N,)): each element writes
-    |det J_e| into its vertices' incidence slots, then each node adds its
-    slots in colour order."""
+    |det J_e| once, then each node adds its elements' determinants in colour
+    order (element order for a mesh without a colouring)."""
     nodes, out, dt_code = _mass_args(colouring, nodes, out)
-    q = torch.empty(max(colouring.n_slots, 1), dtype=torch.float64, device=nodes.device)
+    q = torch.empty(max(colouring.n_elems, 1), dtype=torch.float64, device=nodes.device)
     with torch.cuda.device(nodes.device):
         _lib.call("fm_mass_p1_lumped_gather", colouring.n_elems, colouring.n_nodes, ptr(colouring.elems),
                   dt_code, ptr(nodes), ptr(colouring.chunk_offsets), ptr(colouring.degree),
-                  ptr(colouring.vertex_slots), colouring.n_slots, ptr(q), ptr(out), stream_handle(stream))
+                  ptr(colouring.incident), colouring.n_slots, ptr(q), ptr(out), stream_handle(stream))
     if isinstance(stream, torch.cuda.Stream):
         q.record_stream(stream)  # the scratch is freed here but used on that stream
     return out
@@ -423,6 +446,8 @@ def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=No
 
 def lumped_mass_coloured(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None) -> torch.Tensor:
     """The same weights by the coloured scatter: one launch per colour."""
+    if colouring.method != "colour-order":
+        raise ValueError("this mesh has no colouring of at most 64 colours: use lumped_mass (the gather)")
     nodes, out, dt_code = _mass_args(colouring, nodes, out)
     with torch.cuda.device(nodes.device):
         _lib.call("fm_mass_p1_lumped", colouring.n_elems, colouring.n_nodes, ptr(colouring.elems),

@@ -56,7 +56,7 @@ int fm_version(void);
 
 /* ------------------------------------------------------------------------
  * Signal -- replaces MRI.Signal (cpp/feelmri/MRI.h:43-82 bindings;
- * cpp/feelmri/MRI.cpp:279-381 static nodes, :121-242 moving nodes):
+ * cpp/feelmri/MRI.cpp:196-297 static nodes, :38-159 moving nodes):
  *
  *   S[s, v] = sum_n exp(i(-2 pi_f k_s . r_n(t_s) + phi_dB0_n t_s))
  *                   exp(-t_s / T2_n) (M Mxy)[n, v]
@@ -69,7 +69,7 @@ typedef struct fm_signal_plan fm_signal_plan;
 
 /* kx, ky, kz, t: host arrays of shape (n_meas, n_lines, n_kz), C order --
  * the kloc and t arguments of MRI.Signal (MRI.h:19-20).  Samples with equal
- * t are grouped (the reference's t cache, MRI.cpp:343-359); the plan keeps
+ * t are grouped (the reference's t cache, MRI.cpp:260-276); the plan keeps
  * the sampling tables resident in device memory (allocated here, once). */
 int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int64_t n_lines, int64_t n_kz,
                           const double *kx, const double *ky, const double *kz,
@@ -119,7 +119,7 @@ int fm_signal_nodes_pack(int64_t n_nodes, int n_enc, int dtype, const void *r0,
 /* Evaluate the signal.  out: (n_samples, n_enc) complex128, sample order
  * (i, j, k) of the plan's arrays.  On a Cartesian plan, moving nodes with
  * n_enc <= 4 take the per-readout GEMM (bf16 or FP64 MFMA by precision),
- * otherwise the direct kernels.  Motion (MRI.cpp:192-204): modes (NULL
+ * otherwise the direct kernels.  Motion (MRI.cpp:109-121): modes (NULL
  * for static nodes) is (3 n_modes) rows of n_nodes float32, row c*n_modes+r
  * holding component c of mode r; motion_coef (device) is (n_tgroups,
  * n_modes + 3) doubles: per distinct t the mode weights followed by a
@@ -139,14 +139,14 @@ int fm_signal_plan_set_timing(fm_signal_plan *plan, int max_launches);
 int fm_signal_plan_kernel_ms(fm_signal_plan *plan, double *total_ms, int *n_timed);
 
 /* y = M x for a CSR matrix M (int64 indptr/indices, float64 data) and
- * x (n_cols, n_enc) complex128: the M * Mxy product of MRI.cpp:326 when M
+ * x (n_cols, n_enc) complex128: the M * Mxy product of MRI.cpp:243 when M
  * is the consistent mass matrix (examples/free_running.py:177). */
 int fm_csr_spmv_c128(int64_t n_rows, int n_enc, const int64_t *indptr, const int64_t *indices,
                      const double *data, const double *x, double *y, void *stream);
 
 /* ------------------------------------------------------------------------
  * Bloch -- replaces BlochSimulator.solve_mri (cpp/feelmri/BlochSimulator.h:
- * 50-80 bindings; BlochSimulator.cpp:231-357 static, :92-228 moving).
+ * 50-80 bindings; BlochSimulator.cpp:151-276 static, :12-148 moving).
  * One lane per node; the event table is staged through LDS.
  * ---------------------------------------------------------------------- */
 /* Pack r0 (n,3) row-major, T1, T2, delta_B (n) of dtype `dtype` into the
@@ -203,7 +203,7 @@ int fm_bloch_chain(int precision, int64_t n_pos, const double *packed, double M0
 
 /* ------------------------------------------------------------------------
  * P1 mass -- replaces Assemble.MassAssemble (cpp/feelmri/Assemble.h:21-39;
- * Assemble.cpp:116-155) followed by lumping (python/feelmri/
+ * Assemble.cpp:78-116) followed by lumping (python/feelmri/
  * Phantom.py:379-399).  P1 tetrahedra only.
  * ---------------------------------------------------------------------- */
 /* Host: greedy element colouring so that no two elements of a colour share
@@ -220,29 +220,37 @@ int fm_mass_p1_lumped(int64_t n_elems, int64_t n_nodes, const int32_t *elems, in
                       const void *nodes, const int32_t *perm, const int64_t *color_offsets,
                       int n_colors, double *w, void *stream);
 
-/* Host: each node's incident elements in the order of perm (colour order),
- * as slots of a sliced-ELL table: node n's j-th element owns slot
- * chunk_offsets[n / 64] + 64 j + n % 64.  Outputs (host): chunk_offsets
- * (ceil(n_nodes / 64) + 1; the last entry is the slot count), degree
- * (n_nodes), vertex_slots (4 n_elems: the slot of element e's vertex i).
+/* Host: each node's incident elements in the order of perm (colour order,
+ * or element order for perm = identity), as element ids in the slots of a
+ * sliced-ELL table: node n's j-th element is stored in slot
+ * chunk_offsets[n / 64] + 64 j + n % 64; padding slots hold -1.  Outputs
+ * (host): chunk_offsets (ceil(n_nodes / 64) + 1), degree (n_nodes), n_slots
+ * (the table length, chunk_offsets' last entry) and, unless incident is
+ * NULL (a size query), incident (max_slots >= n_slots int32 entries).
  * elems and perm as for fm_mass_p1_color (host). */
 int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int32_t *elems, const int32_t *perm,
-                         int64_t *chunk_offsets, int32_t *degree, int32_t *vertex_slots);
+                         int64_t *chunk_offsets, int32_t *degree, int32_t *incident, int64_t max_slots,
+                         int64_t *n_slots);
 
 /* Lumped weights w_i = sum_{e ni i} |T_e| / 4 in two launches: each element
- * writes |det J_e| = 6 |T_e| into its four slots of slot_scratch, then node
- * i adds its degree[i] slots -- the additions of fm_mass_p1_lumped, in the
- * same order and with the same rounding, so the result is bitwise the same.
- * chunk_offsets, degree, vertex_slots (16-byte aligned) from
- * fm_mass_p1_incidence; n_slots = the last chunk offset; slot_scratch
- * (n_slots float64, overwritten); elems, nodes, w: all device. */
+ * writes |det J_e| = 6 |T_e| into det_scratch[e], then node i adds the
+ * determinants of its degree[i] incident elements -- the additions of
+ * fm_mass_p1_lumped, in the same order and with the same rounding, so the
+ * result is bitwise the same when perm is a colouring.  chunk_offsets,
+ * degree, incident from fm_mass_p1_incidence (copied to the device);
+ * n_slots = the slot count it returned; det_scratch (n_elems float64,
+ * overwritten); elems, nodes, w: all device.  The tables may belong to a
+ * larger mesh whose elements are ordered by their largest node id: incident
+ * ids >= n_elems are skipped, so its first n_nodes nodes and first n_elems
+ * elements (those whose nodes all lie below n_nodes) are weighted as a mesh
+ * of their own. */
 int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const int32_t *elems, int dtype,
                              const void *nodes, const int64_t *chunk_offsets, const int32_t *degree,
-                             const int32_t *vertex_slots, int64_t n_slots, double *slot_scratch,
+                             const int32_t *incident, int64_t n_slots, double *det_scratch,
                              double *w, void *stream);
 
 /* Per-element 4x4 consistent mass matrices |T_e|/20 (1 + delta_ij),
- * (n_elems, 16) float64 -- the triplet values of Assemble.cpp:148-150. */
+ * (n_elems, 16) float64 -- the triplet values of Assemble.cpp:109-111. */
 int fm_mass_p1_element_matrices(int64_t n_elems, const int32_t *elems, int dtype,
                                 const void *nodes, double *vals, void *stream);
 

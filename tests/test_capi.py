@@ -112,9 +112,10 @@ def test_colouring_is_a_valid_partition():
 
 
 def test_incidence_lists_each_nodes_elements_in_colour_order():
-    """The host incidence slots behind the gather lumped mass: decoded, node
-    n's slots hold exactly the elements containing n, in the order they
-    appear in the colour permutation (brute force on a small mesh)."""
+    """The host incidence table behind the gather lumped mass: node n's slots
+    hold exactly the elements containing n, in the order they appear in the
+    colour permutation (brute force on a small mesh); padding slots hold -1;
+    the size query returns the same slot count."""
     import numpy as np
 
     from feelmri_amd import _lib, phantoms
@@ -125,36 +126,65 @@ def test_incidence_lists_each_nodes_elements_in_colour_order():
     perm = np.empty(E, np.int32)
     offs = np.zeros(65, np.int64)
     nc = ctypes.c_int()
-    _lib.call("fm_mass_p1_color", E, N, el.ctypes.data_as(ctypes.c_void_p), perm.ctypes.data_as(ctypes.c_void_p),
-              offs.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nc))
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    _lib.call("fm_mass_p1_color", E, N, vp(el), vp(perm), vp(offs), 64, ctypes.byref(nc))
     chunk_offsets = np.empty((N + 63) // 64 + 1, np.int64)
     degree = np.empty(N, np.int32)
-    slots = np.empty(4 * E, np.int32)
-    args = lambda p: (E, N, el.ctypes.data_as(ctypes.c_void_p), p.ctypes.data_as(ctypes.c_void_p),
-                      chunk_offsets.ctypes.data_as(ctypes.c_void_p), degree.ctypes.data_as(ctypes.c_void_p),
-                      slots.ctypes.data_as(ctypes.c_void_p))
+    n_slots = ctypes.c_int64()
+    _lib.call("fm_mass_p1_incidence", E, N, vp(el), vp(perm), vp(chunk_offsets), vp(degree), None, 0,
+              ctypes.byref(n_slots))
+    assert n_slots.value == chunk_offsets[-1] >= 4 * E
+    inc = np.empty(n_slots.value, np.int32)
+    args = lambda p: (E, N, vp(el), vp(p), vp(chunk_offsets), vp(degree), vp(inc), inc.size, ctypes.byref(n_slots))
     _lib.call("fm_mass_p1_incidence", *args(perm))
-    # decode every slot back to (node, j) and rebuild each node's list
-    lists = {n: {} for n in range(N)}
-    for e in range(E):
-        for i in range(4):
-            s_ = int(slots[4 * e + i])
-            c = int(np.searchsorted(chunk_offsets, s_, side="right")) - 1
-            j, lane = divmod(s_ - int(chunk_offsets[c]), 64)
-            assert 64 * c + lane == el[e, i], (e, i)
-            assert j not in lists[64 * c + lane]
-            lists[64 * c + lane][j] = e
     for n in range(N):
         want = [int(e) for e in perm if n in el[e]]
         assert degree[n] == len(want)
-        assert [lists[n][j] for j in range(degree[n])] == want, n
         c = n // 64
-        assert degree[n] <= (chunk_offsets[c + 1] - chunk_offsets[c]) // 64
+        width = (chunk_offsets[c + 1] - chunk_offsets[c]) // 64
+        got = [int(inc[chunk_offsets[c] + 64 * j + n % 64]) for j in range(width)]
+        assert got[:degree[n]] == want, n
+        assert all(g == -1 for g in got[degree[n]:]), n
     assert degree[N - 1] == 0  # the unreferenced node
     bad = perm.copy()
     bad[0] = bad[1]
     with pytest.raises(RuntimeError, match="permutation"):
         _lib.call("fm_mass_p1_incidence", *args(bad))
+    small = np.empty(10, np.int32)
+    with pytest.raises(RuntimeError, match="slots"):
+        _lib.call("fm_mass_p1_incidence", E, N, vp(el), vp(perm), vp(chunk_offsets), vp(degree), vp(small),
+                  small.size, ctypes.byref(n_slots))
+
+
+def test_colouring_fallback_for_high_valence_meshes():
+    """A Delaunay tet mesh of random points has nodes with more incident
+    elements than 64 colours can separate: the colouring reports
+    FM_ERR_UNSUPPORTED and P1Colouring falls back to element order (the
+    host tables only; the GPU weights are tested against the oracle)."""
+    import numpy as np
+    from scipy.spatial import Delaunay
+
+    from feelmri_amd import _lib
+
+    pts = np.random.default_rng(0).uniform(size=(6000, 3))
+    el = np.ascontiguousarray(Delaunay(pts).simplices, dtype=np.int32)
+    E, N = el.shape[0], pts.shape[0]
+    val = np.bincount(el.ravel(), minlength=N)
+    perm = np.empty(E, np.int32)
+    offs = np.zeros(65, np.int64)
+    nc = ctypes.c_int()
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    rc = _lib.load().fm_mass_p1_color(E, N, vp(el), vp(perm), vp(offs), 64, ctypes.byref(nc))
+    assert val.max() > 64
+    assert rc == _lib.FM_ERR_UNSUPPORTED and "64 colours" in _lib.last_error()
+    from feelmri_amd import device
+
+    col = device.P1Colouring(el, N, device="cpu")
+    assert col.method == "element-order" and col.n_colours == 0
+    inc, co, deg = col.incident.numpy(), col.chunk_offsets.numpy(), col.degree.numpy()
+    for n in range(0, N, 97):
+        ids = [int(inc[co[n // 64] + 64 * j + n % 64]) for j in range(deg[n])]
+        assert ids == sorted(np.flatnonzero((el == n).any(axis=1)).tolist())
 
 
 def _declared_arity():

@@ -5,6 +5,7 @@ Bloch FP64 relL2 <= 1e-10, FP32 <= 1e-4; mass and POD FP64 <= 1e-12.
 import numpy as np
 import pytest
 
+import motion_fixtures
 from feelmri_amd import Motion, phantoms, sequences
 
 pytestmark = pytest.mark.gpu
@@ -121,25 +122,22 @@ def test_bloch_final_state_f64_exact(gpu):
     assert rel_l2(mz.cpu().numpy(), ref[1]) <= 1e-10
 
 
-def _pod(n, cls=Motion.POD, seed=3):
-    rng = np.random.default_rng(seed)
-    times = np.linspace(0, 40.0, 10)
-    base = rng.normal(size=(n, 3, 3)) * 1e-3
-    coeff = np.stack([np.sin(2 * np.pi * times / 40.0 * (k + 1)) for k in range(3)], 0)
-    return cls(times=times, data=np.einsum("pcr,rt->pct", base, coeff), n_modes=3, is_periodic=True)
+def _pod(n, velocity=False, seed=3, family="own"):
+    return motion_fixtures.pod(n, seed=seed, velocity=velocity, family=family)
 
 
-@pytest.mark.parametrize("kind", ["pod", "velocity", "sum", "callable"])
+@pytest.mark.parametrize("kind", ["pod", "velocity", "sum", "callable", "ref-velocity", "ref-sum"])
 def test_bloch_moving(gpu, kind):
     from feelmri_amd import BlochSimulator
     import oracle
 
     n, T = 800, 90
     r0, T1, T2, dB, rf, G, dt, mxy0, mz0 = _bloch_inputs(n, T, seed=21)
-    resp = Motion.RespiratoryMotion(times=np.linspace(0, 100, 25), data=np.sin(np.linspace(0, 6, 25)) * 0.004,
-                                    is_periodic=True, direction=np.array([1, 1, 0]))
-    pod = _pod(n, Motion.PODVelocity if kind == "velocity" else Motion.POD)
-    m = {"pod": pod, "velocity": pod, "sum": resp + pod, "callable": (lambda tt: pod(tt) * 2.0)}[kind]
+    family = "ref" if kind.startswith("ref-") else "own"
+    base = kind.split("-")[-1]
+    resp = motion_fixtures.resp(direction=(1, 1, 0), family=family)
+    pod = _pod(n, velocity=base == "velocity", family=family)
+    m = {"pod": pod, "velocity": pod, "sum": resp + pod, "callable": (lambda tt: pod(tt) * 2.0)}[base]
     # kernel parity: the oracle moves the nodes with the device's float32 tables
     from test_gpu_signal import descriptor_callable
 
@@ -321,6 +319,29 @@ def test_lumped_gather_equals_coloured_scatter(gpu, mesh, dtype):
         w = w[:-1]
         ref = ref[: w.size]
     assert np.max(np.abs(w - ref) / ref) <= 1e-12
+
+
+def test_lumped_mass_unstructured_without_colouring(gpu):
+    """A Delaunay mesh of random points (node valence above 64, so no
+    colouring of at most 64 colours): the gather adds in element order and
+    matches the oracle, which sums in element order too, to 1e-13; the
+    weights sum to the convex hull's volume."""
+    import torch
+    from scipy.spatial import ConvexHull, Delaunay
+
+    import oracle
+    from feelmri_amd import device
+
+    pts = np.random.default_rng(0).uniform(-0.05, 0.05, size=(6000, 3))
+    el = np.ascontiguousarray(Delaunay(pts).simplices, dtype=np.int32)
+    col = device.P1Colouring(el, pts.shape[0], device=gpu)
+    assert col.method == "element-order"
+    w = device.lumped_mass(col, torch.from_numpy(pts).to(gpu)).cpu().numpy()
+    ref = oracle.mass_p1_lumped(el, pts)
+    assert np.max(np.abs(w - ref) / ref) <= 1e-13
+    np.testing.assert_allclose(w.sum(), ConvexHull(pts).volume, rtol=1e-12)
+    with pytest.raises(ValueError, match="colouring"):
+        device.lumped_mass_coloured(col, torch.from_numpy(pts).to(gpu))
 
 
 def test_lumped_gather_without_elements(gpu):
