@@ -10,7 +10,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import device
+from . import device, resident
 
 
 def tensordot_modes_weights(modes, weights, *, dev=None):
@@ -24,6 +24,9 @@ def tensordot_modes_weights(modes, weights, *, dev=None):
     if modes.ndim != 3 or weights.ndim != 1 or weights.shape[0] != modes.shape[2]:
         raise ValueError(f"shapes {modes.shape} and {weights.shape} do not contract")
     dev = torch.device(dev) if dev is not None else device.default_device()
+    tdt = torch.float32 if modes.dtype == np.float32 else torch.float64
     with torch.cuda.device(dev):
-        out = device.pod_tensordot(torch.from_numpy(modes).to(dev), torch.from_numpy(weights).to(dev))
+        # the caller passes the same modes at every time step
+        # (python/feelmri/Motion.py:254, :290): resident by content
+        out = device.pod_tensordot(resident.to_device(modes, tdt, dev), torch.from_numpy(weights).to(dev))
         return out.cpu().numpy()
