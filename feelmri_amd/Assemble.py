@@ -2,7 +2,7 @@
 
 ``MassAssemble(elems, nodes, finite_element, quadrature_rule)`` returns the
 consistent mass matrix as scipy CSC, like the reference
-(cpp/feelmri/Assemble.h:21-39, Assemble.cpp:116-155).  Only P1 tetrahedra
+(cpp/feelmri/Assemble.h:21-39, Assemble.cpp:78-116).  Only P1 tetrahedra
 are supported: their element matrices |T|/20 (1 + delta_ij) are exact for
 the order-2 rule the reference uses, so ``finite_element`` and
 ``quadrature_rule`` are checked (4 dofs) and otherwise unused.  The element

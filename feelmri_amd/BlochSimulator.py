@@ -3,15 +3,15 @@
 ``solve_mri(r0, T1, T2, delta_B, M0, gamma, rf_all, G_all, dt, regime_idx,
 Mxy_initial, Mz_initial, pod_trajectory)`` keeps the reference's signature
 and return value (cpp/feelmri/BlochSimulator.h:16-80,
-BlochSimulator.cpp:92-357): the full magnetisation history
+BlochSimulator.cpp:12-276): the full magnetisation history
 ``(Mxy (N, T) complex64, Mz (N, T) float32)`` with column 0 the initial
 state.  The reference binds only the float32 instantiation; here the
 integration runs in FP64 by default (``precision="f64"``) or FP32
 (``precision="f32"``) on the GPU and the result keeps the reference's
 dtypes.  ``regime_idx`` is accepted and unused, as in the reference
-(BlochSimulator.cpp:333-347).  Motion: ``pod_trajectory`` is evaluated at
+(BlochSimulator.cpp:253-267).  Motion: ``pod_trajectory`` is evaluated at
 the end of every step, ti = dt[1] + ... + dt[i+1] (BlochSimulator.cpp:
-161-166), through the mode/coefficient form of ``feelmri_amd.Motion``;
+81-89), through the mode/coefficient form of ``feelmri_amd.Motion``;
 generic callables are sampled per step on the host.
 """
 from __future__ import annotations
@@ -72,7 +72,7 @@ def solve_mri(r0, T1, T2, delta_B, M0, gamma, rf_all, G_all, dt, regime_idx, Mxy
         if mxy0.numel() != N or mz0.numel() != N:
             raise ValueError("initial magnetisation length differs from the node count")
         if pod_trajectory is not None:
-            pod_trajectory(0.0)  # BlochSimulator.cpp:124 evaluates once at t = 0
+            pod_trajectory(0.0)  # BlochSimulator.cpp:44 evaluates once at t = 0
         generic = pod_trajectory is not None and not Motion.is_supported(pod_trajectory)
         if not generic:
             ev = device.EventTable(rf, G, dtv, device=dev)

@@ -1,7 +1,7 @@
 // Per-node Bloch integrator (Cayley-Klein rotation, then T1/T2 relaxation).
 //
-// reference: cpp/feelmri/BlochSimulator.cpp:231-357 (static nodes) and
-// :92-228 (moving nodes), solve_mri<T>.
+// reference: cpp/feelmri/BlochSimulator.cpp:151-276 (static nodes) and
+// :12-148 (moving nodes), solve_mri<T>.
 //
 // One lane per node; node attributes come from an SoA table (coalesced).
 // The event table is walked in chunks of 256 events: each lane of the
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                     }
                 }
                 const Event<T> ev = sev[e];
-                // BlochSimulator.cpp:287-294: e1/e2 only when dt changes.  dt
+                // BlochSimulator.cpp:207-214: e1/e2 only when dt changes.  dt
                 // is the same for every lane (LDS broadcast), so test it on
                 // the scalar unit: a real branch, not two exp per event
                 // under a lane mask.

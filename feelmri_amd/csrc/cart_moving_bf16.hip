@@ -5,8 +5,8 @@
 // With POD motion, positions x_n(t) follow t, and on a Cartesian stack t is a
 // function of the readout index i only, so every sample of readout i sees the
 // same positions x_n(t_i) (the reference re-evaluates them whenever t changes
-// between consecutive samples, MRI.cpp:192-217).  For fixed i the sum over
-// nodes of MRI.cpp:279-381 is the product
+// between consecutive samples, MRI.cpp:109-134).  For fixed i the sum over
+// nodes of MRI.cpp:138-152 is the product
 //
 //   S[i, j, (k,v)] = sum_n Ey_i[j,n] B_i[n,(k,v)]
 //   Ey_i[j,n]      = exp(-2 pi i G_j . x_n(t_i))
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
                 y = packed[N + n] + dy;
                 z = packed[2 * N + n] + dz;
                 const double phi = packed[3 * N + n], r2 = packed[4 * N + n];
-                // c = b exp(-t/T2) exp(i phi t) exp(-2 pi i F.x)   (MRI.cpp:347-368)
+                // c = b exp(-t/T2) exp(i phi t) exp(-2 pi i F.x)   (MRI.cpp:124-146)
                 const double u = fma(Fx, x, fma(Fy, y, Fz * z)) - phi * ti * (1.0 / kTwoPi);
                 float ec, es;
                 cis_neg_f32(u, ec, es);

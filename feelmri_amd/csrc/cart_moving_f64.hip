@@ -152,7 +152,7 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_f64(
                 y = packed[N + n] + dy;
                 z = packed[2 * N + n] + dz;
                 const double phi = packed[3 * N + n], r2 = packed[4 * N + n];
-                // c = b exp(-t/T2) exp(i phi t) exp(-2 pi i F.x)   (MRI.cpp:347-368)
+                // c = b exp(-t/T2) exp(i phi t) exp(-2 pi i F.x)   (MRI.cpp:124-146)
                 const double u = fma(Fx, x, fma(Fy, y, Fz * z)) - phi * ti * (1.0 / kTwoPi);
                 double ec, es;
                 cis_neg_f64(u, ec, es);

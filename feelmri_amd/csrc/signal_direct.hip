@@ -2,7 +2,7 @@
 //
 //   S[s, v] = sum_n exp(i(-2 pi_f k_s . r_n + phi_n t_s)) exp(-t_s / T2_n) b[n, v]
 //
-// reference: cpp/feelmri/MRI.cpp:279-381 (static nodes), :121-242 (moving).
+// reference: cpp/feelmri/MRI.cpp:196-297 (static nodes), :38-159 (moving).
 //
 // Work decomposition.  Samples of equal t are grouped (the reference's t
 // cache) and cut into slots of up to 256 samples; a workgroup owns a tile of
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                             for (int v = 0; v < 2 * VB; ++v) nd.c[j][v] = nd.c[0][v];
                             continue;
                         }
-                        // c = b exp(-t/T2) exp(i phi t)   (MRI.cpp:347-354); the
+                        // c = b exp(-t/T2) exp(i phi t)   (MRI.cpp:264-271); the
                         // mixed form reduces phi t / 2 pi in FP64 and takes the
                         // FP32 hardware sine, cosine and exp2
                         double fr, fi;
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
 // (FP64 phase and sincospi for the anchor and the step, FP64 products and
 // accumulators).  Taken only when every run is linear to 1e-14 of the
 // largest |k'| and |t|; 15 FP64 complex products then leave the result
-// ~1e-15 from the per-sample sum of MRI.cpp:343-371.
+// ~1e-15 from the per-sample sum of MRI.cpp:260-288.
 template <int VB>
 struct alignas(16) RunNode64 {
     double x, y, z, dwc, r2, pad;

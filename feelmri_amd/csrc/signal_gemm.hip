@@ -4,7 +4,7 @@
 // indices, k(i,j,k) = F(i) + G(j) + H(k) per axis, and t depends on the
 // readout index i only (Cartesian stacks with one line per shot,
 // python/feelmri/KSpaceTraj.py:183-220), the direct sum of
-// cpp/feelmri/MRI.cpp:279-381 factorises:
+// cpp/feelmri/MRI.cpp:196-297 factorises:
 //
 //   S[i, (j,k,v)] = sum_n E[i,n] A[n,(j,k,v)]
 //   E[i,n]        = exp(-t_i/T2_n) exp(i phi_n t_i) exp(-2 pi i F_i . x_n)

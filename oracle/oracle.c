@@ -24,7 +24,7 @@
 #endif
 
 /* cpp/feelmri/MRI.h:13 declares `const float PI = M_PI;`, so every
- * two_pi = 2.0 * PI in MRI.cpp (:156, :314) is 2*(double)(float)pi even in
+ * two_pi = 2.0 * PI in MRI.cpp (:73, :231) is 2*(double)(float)pi even in
  * Signal<double>. */
 static const float ORC_PI_F = (float)M_PI;
 
@@ -40,14 +40,14 @@ int orc_max_threads(void)
 }
 
 /* ------------------------------------------------------------------------
- * Signal, static nodes.  cpp/feelmri/MRI.cpp:279-381 (Signal<T>(..., py::none)).
+ * Signal, static nodes.  cpp/feelmri/MRI.cpp:196-297 (Signal<T>(..., py::none)).
  *
  *   S[s, v] = sum_n exp(i(-2pi_f (kx x_n + ky y_n + kz z_n) + phi_dB0_n t_s))
  *                   * exp(-t_s / T2_n) * MMxy[n, v]
  *
- * MMxy = M * Mxy is formed by the caller (MRI.cpp:326).  Samples s run in the
+ * MMxy = M * Mxy is formed by the caller (MRI.cpp:243).  Samples s run in the
  * reference's (i, j, k) row-major order; the per-node decay and off-resonance
- * terms are recomputed only when t changes (MRI.cpp:343-359), exactly as in
+ * terms are recomputed only when t changes (MRI.cpp:260-276), exactly as in
  * the reference.  With OpenMP each thread owns a contiguous block of samples,
  * which leaves every per-sample sum in the same node order as the serial loop.
  *
@@ -80,7 +80,7 @@ void orc_signal(int64_t n_nodes, int64_t n_samples, int64_t n_enc,
         int have_t = 0;
         for (int64_t s = s0; s < s1; ++s) {
             if (!have_t || t_old != t[s]) {
-                /* MRI.cpp:347-357 */
+                /* MRI.cpp:264-274 */
                 for (int64_t n = 0; n < n_nodes; ++n) {
                     phi_off[n] = phi_dB0[n] * t[s];
                     decay[n] = exp(-t[s] * (1.0 / T2[n]));
@@ -88,14 +88,14 @@ void orc_signal(int64_t n_nodes, int64_t n_samples, int64_t n_enc,
                 t_old = t[s];
                 have_t = 1;
             }
-            /* MRI.cpp:362 */
+            /* MRI.cpp:279 */
             const double kx_ = two_pi * kx[s], ky_ = two_pi * ky[s], kz_ = two_pi * kz[s];
             for (int64_t v = 0; v < n_enc; ++v) acc[2 * v] = acc[2 * v + 1] = 0.0;
             for (int64_t n = 0; n < n_nodes; ++n) {
-                /* MRI.cpp:365-371 */
+                /* MRI.cpp:282-288 */
                 const double phase = -kx_ * r[3 * n] - ky_ * r[3 * n + 1] - kz_ * r[3 * n + 2] + phi_off[n];
                 const double c = cos(phase), sn = sin(phase);
-                /* MRI.cpp:374: fourier^T * (MMxy .* decay) */
+                /* MRI.cpp:291: fourier^T * (MMxy .* decay) */
                 for (int64_t v = 0; v < n_enc; ++v) {
                     const double br = mmxy[2 * (n * n_enc + v)] * decay[n];
                     const double bi = mmxy[2 * (n * n_enc + v) + 1] * decay[n];
@@ -115,18 +115,18 @@ void orc_signal(int64_t n_nodes, int64_t n_samples, int64_t n_enc,
 }
 
 /* ------------------------------------------------------------------------
- * Bloch integrator.  cpp/feelmri/BlochSimulator.cpp:231-357 (static nodes,
- * solve_mri<T>(..., py::none)) and :92-228 (moving nodes), instantiated at
+ * Bloch integrator.  cpp/feelmri/BlochSimulator.cpp:151-276 (static nodes,
+ * solve_mri<T>(..., py::none)) and :12-148 (moving nodes), instantiated at
  * T = double (the reference binds only T = float, BlochSimulator.h:50-80).
  *
  * Step i -> i+1 uses rf[i+1], G[i+1,:], dt[i+1]; column 0 is the initial
- * condition.  e1/e2 are refreshed only when dt changes (:287-294); |B| is
- * floored at 1e-12 (:318-319); rotation first, then relaxation toward the
- * scalar M0 (:350-351).  regime_idx is unused by the reference (:333-347).
+ * condition.  e1/e2 are refreshed only when dt changes (:207-214); |B| is
+ * floored at 1e-12 (:238-239); rotation first, then relaxation toward the
+ * scalar M0 (:270-271).  regime_idx is unused by the reference (:253-267).
  *
  * disp: NULL for static nodes, else (n_time-1, n_pos, 3) row-major: the
  * displacement pod_trajectory(ti) evaluated at the end of step i
- * (ti = dt[1] + ... + dt[i+1], BlochSimulator.cpp:162-166).
+ * (ti = dt[1] + ... + dt[i+1], BlochSimulator.cpp:81-89).
  *
  * history != 0: mxy_out is (n_pos, n_time) complex and mz_out (n_pos,
  * n_time), column-major like the reference's Eigen matrices
@@ -219,14 +219,14 @@ void orc_bloch(int64_t n_pos, int64_t n_time,
 }
 
 /* ------------------------------------------------------------------------
- * P1 tetrahedron mass matrix.  cpp/feelmri/Assemble.cpp:54-155
+ * P1 tetrahedron mass matrix.  cpp/feelmri/Assemble.cpp:15-116
  * (IsoMapping, LocalMassAssemble, MassAssemble) for the P1 Lagrange element
  * on the reference tetrahedron with an order-2 quadrature rule (the call in
  * python/feelmri/Phantom.py:379-392 uses quadrature_order=2).
  *
  * P1 basis: S = (1-r-s-t, r, s, t); dS/dr = (-1,1,0,0), dS/ds = (-1,0,1,0),
- * dS/dt = (-1,0,0,1).  J(a,b) = sum_i dS_a(i) x_i(b) (Assemble.cpp:75-84),
- * Me = sum_q S S^T |det J| w_q (Assemble.cpp:108-112).
+ * dS/dt = (-1,0,0,1).  J(a,b) = sum_i dS_a(i) x_i(b) (Assemble.cpp:36-45),
+ * Me = sum_q S S^T |det J| w_q (Assemble.cpp:69-73).
  *
  * The quadrature is the symmetric 4-point rule exact for degree 2
  * (points (a,b,b), (b,a,b), (b,b,a), (b,b,b) with b = (5-sqrt5)/20,
@@ -234,7 +234,7 @@ void orc_bloch(int64_t n_pos, int64_t n_time,
  * (basix's default included) yields the same element matrix.
  *
  * vals: (n_elems, 4, 4) element matrices, row-major, in element order -- the
- * triplets the reference emits at Assemble.cpp:148-150.
+ * triplets the reference emits at Assemble.cpp:109-111.
  * ---------------------------------------------------------------------- */
 void orc_mass_p1_element_matrices(int64_t n_elems, const int32_t *elems,
                                   const double *nodes, double *vals)
@@ -266,7 +266,7 @@ void orc_mass_p1_element_matrices(int64_t n_elems, const int32_t *elems,
 
 /* Lumped weights: w_i = sum_j M_ij (python/feelmri/Phantom.py:395-398),
  * accumulated from the element matrices in element order, the order in
- * which Eigen's setFromTriplets sums duplicates (Assemble.cpp:153). */
+ * which Eigen's setFromTriplets sums duplicates (Assemble.cpp:114). */
 void orc_mass_p1_lumped(int64_t n_elems, int64_t n_nodes, const int32_t *elems,
                         const double *nodes, double *w)
 {
@@ -297,7 +297,7 @@ void orc_pod_tensordot(int64_t P, int64_t C, int64_t M, const double *modes,
 }
 
 /* Sparse (CSR) complex SpMV y = M x, x (n, n_enc) complex: the M * Mxy
- * product of MRI.cpp:326 for a general (consistent) mass matrix. */
+ * product of MRI.cpp:243 for a general (consistent) mass matrix. */
 void orc_csr_spmv_c128(int64_t n_rows, int64_t n_enc, const int64_t *indptr,
                        const int64_t *indices, const double *data,
                        const double *x, double *y)

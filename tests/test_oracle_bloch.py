@@ -3,7 +3,7 @@
 Closed forms: on-resonance hard pulse, free precession with T2 decay and
 off-resonance, T1 recovery, the Ernst/FLASH steady state under the
 reference's ideal spoiling (python/feelmri/Bloch.py:363-368), rotation norm
-preservation, and the step semantics of BlochSimulator.cpp:284-351
+preservation, and the step semantics of BlochSimulator.cpp:204-271
 (column 0 = initial state; rf[0], G[0], dt[0] unused).
 """
 import numpy as np

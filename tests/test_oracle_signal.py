@@ -3,7 +3,7 @@
 Each check is fixed by mathematics independent of the oracle's loop:
 closed forms (one node, Dirichlet kernel of a lattice), the shift theorem,
 linearity/additivity, and the reference's float-pi constant
-(cpp/feelmri/MRI.h:13) and +phi t sign (MRI.cpp:365-368).
+(cpp/feelmri/MRI.h:13) and +phi t sign (MRI.cpp:282-285).
 """
 import numpy as np
 import pytest
