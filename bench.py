@@ -325,7 +325,9 @@ def serving_run(dev, n_requests: int, rate: float, max_active: int = 8):
     out.update({"rate_per_s": rate, "max_active": max_active, "resident_setup_s": prep,
                 "mean_latency_s": float(lat.mean()) if lat.size else None,
                 "p99_latency_by_form_s": {k: float(np.percentile(v, 99)) for k, v in by.items()},
-                "stream": "Poisson arrivals; N log-uniform 10k-2M; "
+                "stream": "Poisson arrivals; N log-uniform 10k-2M, served exactly (the first N nodes of a resident "
+                          "box phantom, tissue drawn per request); steady states of one admission pass batched "
+                          "into one graph replay; "
                           "Cartesian 2-D 64^2-256^2, 3-D 64^2-256^2 x 32, radial 64-512 spokes x 256-512; "
                           "FP64 or mixed"})
     return out

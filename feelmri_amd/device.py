@@ -411,12 +411,23 @@ class P1Colouring:
         _lib.call("fm_mass_p1_incidence", E, self.n_nodes, vp(el), vp(perm), vp(chunk_offsets), vp(degree),
                   vp(incident), incident.size, ctypes.byref(n_slots))
         self.n_slots = int(n_slots.value)
+        # elements ordered by their largest node id: a node prefix's sub-mesh
+        # is an element prefix (see lumped_mass)
+        self._emax = el.max(axis=1) if E else np.zeros(0, dtype=np.int32)
+        self.prefix_ordered = bool(np.all(np.diff(self._emax) >= 0))
         self.elems = torch.from_numpy(el).to(device)
         self.perm = torch.from_numpy(perm).to(device)
         self.chunk_offsets = torch.from_numpy(chunk_offsets).to(device)
         self.degree = torch.from_numpy(degree).to(device)
         self.incident = torch.from_numpy(incident).to(device)
         self.device = device
+
+    def n_elems_below(self, n: int) -> int:
+        """Elements whose nodes all lie below node ``n`` (the leading ones
+        when ``prefix_ordered``; all elements for n = n_nodes)."""
+        if n >= self.n_nodes:
+            return self.n_elems
+        return int(np.searchsorted(self._emax, n, side="left"))
 
 
 def _mass_args(colouring: P1Colouring, nodes: torch.Tensor, out):
@@ -429,16 +440,36 @@ def _mass_args(colouring: P1Colouring, nodes: torch.Tensor, out):
     return nodes, out, dt_code
 
 
-def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None) -> torch.Tensor:
+def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None,
+                n_nodes: int | None = None) -> torch.Tensor:
     """w_i = sum_{e ni i} |T_e| / 4 (float64, (N,)): each element writes
     |det J_e| once, then each node adds its elements' determinants in colour
-    order (element order for a mesh without a colouring)."""
-    nodes, out, dt_code = _mass_args(colouring, nodes, out)
-    q = torch.empty(max(colouring.n_elems, 1), dtype=torch.float64, device=nodes.device)
+    order (element order for a mesh without a colouring).
+
+    ``n_nodes`` below the mesh's node count weights the sub-mesh of the first
+    ``n_nodes`` nodes: the elements whose nodes all lie below it, which are
+    the leading elements when the mesh's elements are ordered by their
+    largest node id (``colouring.prefix_ordered``).  ``nodes`` then needs only
+    those rows."""
+    n = colouring.n_nodes if n_nodes is None else int(n_nodes)
+    if n != colouring.n_nodes:
+        if not colouring.prefix_ordered:
+            raise ValueError("a node prefix needs elements ordered by their largest node id")
+        if not 0 <= n <= colouring.n_nodes:
+            raise ValueError(f"n_nodes {n} outside [0, {colouring.n_nodes}]")
+    E = colouring.n_elems_below(n)
+    nodes = nodes.contiguous()
+    if nodes.dtype not in (torch.float32, torch.float64) or nodes.ndim != 2 or nodes.shape[1] != 3 \
+            or nodes.shape[0] < n:
+        raise ValueError(f"nodes must be float32/float64 of shape (>= {n}, 3)")
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=nodes.device)
+    dt_code = _lib.DTYPE_F64 if nodes.dtype == torch.float64 else _lib.DTYPE_F32
+    q = torch.empty(max(E, 1), dtype=torch.float64, device=nodes.device)
     with torch.cuda.device(nodes.device):
-        _lib.call("fm_mass_p1_lumped_gather", colouring.n_elems, colouring.n_nodes, ptr(colouring.elems),
-                  dt_code, ptr(nodes), ptr(colouring.chunk_offsets), ptr(colouring.degree),
-                  ptr(colouring.incident), colouring.n_slots, ptr(q), ptr(out), stream_handle(stream))
+        _lib.call("fm_mass_p1_lumped_gather", E, n, ptr(colouring.elems), dt_code, ptr(nodes),
+                  ptr(colouring.chunk_offsets), ptr(colouring.degree), ptr(colouring.incident), colouring.n_slots,
+                  ptr(q), ptr(out), stream_handle(stream))
     if isinstance(stream, torch.cuda.Stream):
         q.record_stream(stream)  # the scratch is freed here but used on that stream
     return out

@@ -1,31 +1,39 @@
 """Serving loop: continuous batching of synthetic scan requests on one GPU.
 
-A request is one synthetic scan: a phantom of N nodes, a spoiled-GRE Bloch
-solve to steady state, lumped-mass weighting and the k-space signal of its
-trajectory in FP64 or mixed precision.  The loop
+A request is one synthetic scan: a phantom of exactly N nodes, a spoiled-GRE
+Bloch solve to steady state, lumped-mass weighting and the k-space signal of
+its trajectory in FP64 or mixed precision.  The loop
 
 * admits requests in arrival order while their HBM footprint fits the
   budget (288 GB by default) and a stream slot is free; a request larger
   than the whole budget is rejected;
-* keeps phantoms (mesh, tissue, lumped weights; 9 size classes) and
-  trajectory plans resident across requests, and each admitted request's
-  packed nodes and output resident until it completes;
-* runs each request's spoiled-GRE steady state as the replay of a HIP graph
-  captured once per phantom class and stream slot (``SteadyStateGraph``);
-* cuts each request's signal into chunks of readout rows and interleaves the
-  chunks of all active requests over their own HIP streams, so a request
-  that arrives while others run starts at the next chunk boundary and its
-  work overlaps theirs on the device;
+* serves the requested node count exactly: every phantom is the first N
+  nodes of one resident box mesh (``NodePool``, grid order) with the
+  elements among them, weighted on the device from the pool's incidence
+  table, and its tissue is drawn on the device from the request's seed;
+* batches the steady-state Bloch solves of all requests admitted in one
+  scheduling pass into one launch: their node tables are concatenated and
+  the whole TR loop runs as one chained integrator launch (state in
+  registers, stores and spoiling in the kernel), replayed from a HIP graph
+  captured once per capacity class (``SteadyStateGraphs``: powers of two of
+  nodes, the batch padded up to the class);
+* keeps trajectory plans resident across requests and each admitted
+  request's packed nodes and output resident until it completes;
+* cuts each request's signal into chunks of readout rows and interleaves
+  the chunks of all active requests over their own HIP streams, so a
+  request that arrives while others run starts at the next chunk boundary
+  and its work overlaps theirs on the device;
 * records latency from arrival to the result on the host, and aggregate
   node*k-sample/s.
 
-Chunk launch shapes depend only on the chunk's plan and the request's
-sizes, and every reduction runs in a fixed order, so a request's result is
-bitwise the same alone and in any mix.
+Every per-node computation is independent of the other nodes in its launch,
+chunk launch shapes depend only on the chunk's plan and the request's sizes,
+and every reduction runs in a fixed order, so a request's result is bitwise
+the same alone and in any mix.
 
-The loop is written against a small backend interface (``start``,
-``step``, ``done``, ``finish``, ``footprint``); ``GpuBackend`` is the one
-that runs on the device.
+The loop is written against a small backend interface (``start`` or
+``start_batch``, ``step``, ``done``, ``finish``, ``footprint``);
+``GpuBackend`` is the one that runs on the device.
 """
 from __future__ import annotations
 
@@ -100,6 +108,7 @@ class ServeReport:
     wall: float = 0.0
     peak_bytes: int = 0
     max_active: int = 0
+    batches: list = field(default_factory=list)   # requests per admission batch
 
     def latencies(self) -> np.ndarray:
         return np.array([r.latency for r in self.results.values() if r.error is None])
@@ -116,11 +125,16 @@ class ServeReport:
             "wall_s": self.wall,
             "peak_admitted_bytes": self.peak_bytes,
             "max_concurrent": self.max_active,
+            "admission_batches": len(self.batches),
+            "max_batch": max(self.batches) if self.batches else 0,
         }
 
 
 class Scheduler:
-    """Arrival-ordered admission against a byte budget, round-robin chunk issue."""
+    """Arrival-ordered admission against a byte budget, round-robin chunk issue.
+
+    All requests admitted in one pass start together (``backend.start_batch``
+    when the backend has it), so their set-up can share launches."""
 
     def __init__(self, backend, budget_bytes: int = 288 * 1024**3, max_active: int = 8,
                  ahead: int = 2, realtime: bool = True):
@@ -141,7 +155,8 @@ class Scheduler:
         while pending or active:
             now = clock() if clock else virtual
             # admission: strictly in arrival order (no overtaking)
-            while pending and pending[0].arrival <= now and len(active) < self.max_active:
+            admitted = []
+            while pending and pending[0].arrival <= now and len(active) + len(admitted) < self.max_active:
                 req = pending[0]
                 fp = self.backend.footprint(req)
                 if fp > self.budget:
@@ -153,17 +168,24 @@ class Scheduler:
                 if used + fp > self.budget:
                     break  # wait for memory to free up
                 pending.pop(0)
-                busy = {getattr(a, "slot", -1) for a in active}
+                busy = {getattr(a, "slot", -1) for a in active} | {s for _, s, _ in admitted}
                 slot = next(i for i in range(self.max_active + 1) if i not in busy)
-                st = self.backend.start(req, slot=slot)
-                st.slot = slot
-                st.t_admit = now
-                st.fp = fp
-                active.append(st)
+                admitted.append((req, slot, fp))
                 used += fp
+            if admitted:
+                if hasattr(self.backend, "start_batch"):
+                    states = self.backend.start_batch([(r, s) for r, s, _ in admitted])
+                else:
+                    states = [self.backend.start(r, slot=s) for r, s, _ in admitted]
+                for st, (req, slot, fp) in zip(states, admitted):
+                    st.slot = slot
+                    st.t_admit = now
+                    st.fp = fp
+                    active.append(st)
+                rep.batches.append(len(admitted))
                 rep.peak_bytes = max(rep.peak_bytes, used)
                 rep.max_active = max(rep.max_active, len(active))
-            progressed = False
+            progressed = bool(admitted)
             for st in active:
                 progressed |= self.backend.step(st, self.ahead)
             for st in list(active):
@@ -196,119 +218,131 @@ class Scheduler:
 class _State:
     req: ScanRequest
     stream: object
-    phantom: object
     plans: list
     rows: list                    # (i0, i1) readout rows per chunk
     out: object = None
     ws: object = None
     nodes: object = None
+    tissue: dict = None
+    bloch: object = None
+    w: object = None
+    mxy: object = None            # the request's steady-state Mxy (M0 applied), (N, 1) complex128
     next_chunk: int = 0
     events: list = field(default_factory=list)
     host: object = None
     copy_event: object = None
     t_admit: float = 0.0
     fp: int = 0
-    work: int = 0                 # node*k-samples actually computed (phantom class size)
+    work: int = 0                 # node*k-samples computed
     slot: int = -1
 
 
-# Cube sides of the resident phantom classes: k^3 nodes from 10.6 k to 2.25 M
-# in steps of 1.25 in k (about 2x in nodes).
-PHANTOM_CLASSES = (22, 27, 34, 43, 54, 67, 84, 105, 131)
+class NodePool:
+    """The resident box phantom every request takes its nodes from.
 
+    A box grid of at least ``n_max`` nodes (C order, the last axis fastest)
+    whose elements are ordered by their largest node id: the first N nodes
+    and the leading elements -- those whose nodes all lie below N -- form a
+    mesh of their own (a slab of the box), weighted by the lumped-mass gather
+    from the pool's incidence table (``lumped_mass(n_nodes=N)``)."""
 
-class PhantomStore:
-    """Resident phantoms by size class: mesh, tissue, lumped weights, Bloch nodes.
-
-    A request's node count is served by the class whose k^3 is nearest in
-    log scale (``classes=None``: a class per cube side, i.e. the nearest
-    k^3)."""
-
-    def __init__(self, dev, classes=PHANTOM_CLASSES):
-        self.dev = dev
-        self.cache = {}
-        self.classes = tuple(classes) if classes else None
-
-    def key(self, n_nodes: int) -> int:
-        # cube side of a box mesh with about n_nodes nodes
-        k = max(4, int(round(n_nodes ** (1.0 / 3.0))))
-        if not self.classes:
-            return k
-        return min(self.classes, key=lambda c: abs(math.log(c) - math.log(k)))
-
-    def n_nodes(self, n_nodes: int) -> int:
-        return self.key(n_nodes) ** 3
-
-    def get(self, n_nodes: int):
+    def __init__(self, dev, n_max: int = 2_000_000):
         import torch
 
         from . import device, phantoms
 
-        k = self.key(n_nodes)
-        if k not in self.cache:
-            m = phantoms.box_mesh((-0.1, -0.1, -0.1), (0.1, 0.1, 0.1), (k, k, k))
-            tis = phantoms.draw_tissue(m.n_nodes, seed=2000 + k, r2s_range=(10.0, 50.0))
-            col = device.P1Colouring(m.elems, m.n_nodes, device=self.dev)
-            nodes_d = torch.from_numpy(m.nodes).to(self.dev)
-            w = device.lumped_mass(col, nodes_d)
-            bn = device.BlochNodes.from_arrays(m.nodes, tis.T1, tis.T2, tis.delta_B_mT, device=self.dev)
-            self.cache[k] = dict(mesh=m, tissue=tis, nodes=nodes_d, w=w, bloch=bn,
-                                 phi=torch.from_numpy(tis.delta_omega).to(self.dev),
-                                 T2s=torch.from_numpy(tis.T2star).to(self.dev),
-                                 M0=torch.from_numpy(tis.M0).to(self.dev))
-        return self.cache[k]
+        k = max(4, int(math.ceil(round(n_max ** (1.0 / 3.0), 9))))
+        while k ** 3 < n_max:
+            k += 1
+        m = phantoms.box_mesh((-0.1, -0.1, -0.1), (0.1, 0.1, 0.1), (k, k, k))
+        order = np.argsort(m.elems.max(axis=1), kind="stable")
+        self.elems = np.ascontiguousarray(m.elems[order])
+        self.n_max = m.n_nodes
+        self.colouring = device.P1Colouring(self.elems, m.n_nodes, device=dev)
+        assert self.colouring.prefix_ordered
+        self.nodes = torch.from_numpy(m.nodes).to(dev)          # (n_max, 3) float64
+        self.dev = dev
 
-
-class SteadyStateGraph:
-    """Spoiled-GRE steady state of one resident phantom, captured as a HIP graph.
-
-    The TR loop is recorded once per phantom class and stream slot, then
-    replayed per request as one graph launch:
-    - per TR one imaging block and one delay block;
-    - all blocks run as one chained integrator launch, with the state in
-      registers and the store and spoiling in the kernel.
-
-    The graph owns its state and store buffers. The initial state
-    (Mxy = 0, Mz = M0 = 1) is written by fill kernels inside the graph, so
-    every replay starts from the same state. The stored Mxy of the last TR
-    is ``store_xy[0]``, valid once the replay has run on the slot's stream.
-    """
-
-    def __init__(self, ph: dict, blocks: list, dev, stream):
-        import torch
-
-        from . import device, sequences
-
-        N = ph["mesh"].n_nodes
-        self.chain = device.BlockChain([(b.rf, b.G, b.dt) for b in blocks], [0 if b.store else -1 for b in blocks],
-                                       [0 if b.empty else 1 for b in blocks], device=dev)
-        self.blocks = blocks
-        self.nodes = ph["bloch"]
-        self.mxy = torch.empty(N, dtype=torch.complex128, device=dev)
-        self.mz = torch.empty(N, dtype=torch.float64, device=dev)
-        self.store_xy = torch.empty((1, N), dtype=torch.complex64, device=dev)
-        self.store_z = torch.empty((1, N), dtype=torch.float32, device=dev)
-        self.gamma = sequences.GAMMA
-        with torch.cuda.stream(stream):
-            self._body()           # eager run: loads the kernels before capture
-        stream.synchronize()
-        self.graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(self.graph, stream=stream):
-            self._body()
-
-    def _body(self):
+    def weights(self, n: int, stream=None):
         from . import device
 
-        self.mxy.zero_()
-        self.mz.fill_(1.0)
-        self.store_xy.zero_()
-        self.store_z.zero_()
-        # the TR loop as one chained integrator launch (state in registers)
-        device.bloch_chain(self.nodes, self.chain, 1.0, self.gamma, self.mxy, self.mz, self.store_xy,
-                           self.store_z)
+        return device.lumped_mass(self.colouring, self.nodes, n_nodes=n, stream=stream)
 
-    def replay(self):
-        self.graph.replay()
+
+def draw_tissue_device(n: int, seed: int, dev) -> dict:
+    """Per-node tissue on the device from a seed (the serving stream's draw:
+    M0~U[0.8,1], T1~U[500,1500] ms, T2~U[40,100] ms, dB0~N(0,20 Hz),
+    R2*~U[10,50] 1/s), with the derived off-resonance in mT and rad/ms and
+    T2* in ms.  The same (n, seed) gives the same values in any mix."""
+    import torch
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    u = torch.rand((4, n), generator=g, device=dev, dtype=torch.float64)
+    df = 20.0 * torch.randn(n, generator=g, device=dev, dtype=torch.float64)
+    T2 = 40.0 + 60.0 * u[2]
+    R2s = 10.0 + 40.0 * u[3]
+    return {"M0": 0.8 + 0.2 * u[0], "T1": 500.0 + 1000.0 * u[1], "T2": T2,
+            "dB": df / 42.58e6 * 1e3, "phi": 2.0 * math.pi * df / 1000.0,
+            "T2s": 1.0 / (1.0 / T2 + R2s / 1000.0)}
+
+
+class SteadyStateGraphs:
+    """The spoiled-GRE TR loop as HIP graphs, one per capacity class.
+
+    A class holds node tables for 2^k nodes.  A batch of requests is copied
+    into the class's node table (the padding beyond the batch keeps earlier,
+    finite node data and its results are ignored), and one replay runs the
+    initial-state fills and the whole block chain as one chained integrator
+    launch.  The stored Mxy of the last TR is ``store_xy[0]``."""
+
+    MIN_LOG2 = 14
+
+    def __init__(self, blocks: list, dev, stream):
+        from . import device, sequences
+
+        self.blocks = blocks
+        self.chain = device.BlockChain([(b.rf, b.G, b.dt) for b in blocks], [0 if b.store else -1 for b in blocks],
+                                       [0 if b.empty else 1 for b in blocks], device=dev)
+        self.gamma = sequences.GAMMA
+        self.dev = dev
+        self.stream = stream
+        self.classes = {}
+
+    def capacity(self, n: int) -> int:
+        return 1 << max(self.MIN_LOG2, int(math.ceil(math.log2(max(n, 1)))))
+
+    def get(self, n: int) -> dict:
+        import torch
+
+        from . import device
+
+        cap = self.capacity(n)
+        c = self.classes.get(cap)
+        if c is not None:
+            return c
+        c = {"cap": cap,
+             "packed": torch.zeros((6, cap), dtype=torch.float64, device=self.dev),
+             "mxy": torch.empty(cap, dtype=torch.complex128, device=self.dev),
+             "mz": torch.empty(cap, dtype=torch.float64, device=self.dev),
+             "store_xy": torch.empty((1, cap), dtype=torch.complex64, device=self.dev),
+             "store_z": torch.empty((1, cap), dtype=torch.float32, device=self.dev)}
+        nodes = device.BlochNodes(c["packed"], cap)
+
+        def body():
+            c["mxy"].zero_()
+            c["mz"].fill_(1.0)
+            device.bloch_chain(nodes, self.chain, 1.0, self.gamma, c["mxy"], c["mz"], c["store_xy"], c["store_z"])
+
+        with torch.cuda.stream(self.stream):
+            body()                      # eager run: loads the kernels before capture
+        self.stream.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            body()
+        c["graph"] = g
+        self.classes[cap] = c
+        return c
 
 
 def request_trajectory(req: ScanRequest):
@@ -321,21 +355,30 @@ def request_trajectory(req: ScanRequest):
 
 
 class GpuBackend:
-    """Runs requests on the device; phantoms and plans stay resident."""
+    """Runs requests on the device; the node pool, trajectory plans and the
+    steady-state graphs stay resident."""
 
-    def __init__(self, dev=None, chunk_rows: int = 32, n_dummy: int = 8):
+    def __init__(self, dev=None, chunk_rows: int = 32, n_dummy: int = 8, n_max: int = 2_000_000):
         import torch
 
         from . import device
 
         self.dev = torch.device(dev) if dev is not None else device.default_device()
-        self.store = PhantomStore(self.dev)
+        self.n_max = n_max
+        self._pool = None
         self.plans = {}
         self.chunk_rows = chunk_rows
         self.n_dummy = n_dummy
         self.streams = []
-        self.graphs = {}          # (phantom class, slot) -> SteadyStateGraph
+        self._graphs = None
         self._blocks = None
+        self.batch_stream = None
+
+    @property
+    def pool(self) -> NodePool:
+        if self._pool is None:
+            self._pool = NodePool(self.dev, self.n_max)
+        return self._pool
 
     def _stream(self, slot: int):
         import torch
@@ -360,17 +403,7 @@ class GpuBackend:
             self.plans[key] = (plans, rows)
         return self.plans[key]
 
-    def prepare(self, requests, slots: int = 0) -> None:
-        """Make the phantoms and plans of ``requests`` resident (service
-        start-up), and capture the steady-state graphs of their phantom
-        classes for stream slots 0 .. slots-1."""
-        for req in requests:
-            self.store.get(req.n_nodes)
-            self._chunks(req)
-            for slot in range(slots):
-                self._graph(req.n_nodes, slot)
-
-    def _steady_blocks(self):
+    def steady_blocks(self):
         from . import sequences
 
         if self._blocks is None:
@@ -378,45 +411,94 @@ class GpuBackend:
                                                  rf_dur=0.2)
         return self._blocks
 
-    def _graph(self, n_nodes: int, slot: int) -> SteadyStateGraph:
-        key = (self.store.key(n_nodes), slot)
-        g = self.graphs.get(key)
-        if g is None:
-            g = SteadyStateGraph(self.store.get(n_nodes), self._steady_blocks(), self.dev, self._stream(slot))
-            self.graphs[key] = g
-        return g
-
-    def footprint(self, req: ScanRequest) -> int:
-        N = self.store.n_nodes(req.n_nodes)
-        S = req.n_samples
-        plans, _ = self._chunks(req)
-        ws = max(p.workspace_bytes(req.precision, N, 1) for p in plans)
-        return N * (24 + 7 * 8) + ws + S * 16
-
-    def start(self, req: ScanRequest, slot: int):
+    @property
+    def graphs(self) -> SteadyStateGraphs:
         import torch
 
-        from . import device, sequences
+        if self._graphs is None:
+            self.batch_stream = torch.cuda.Stream(device=self.dev)      # default priority
+            self._graphs = SteadyStateGraphs(self.steady_blocks(), self.dev, self.batch_stream)
+        return self._graphs
 
-        ph = self.store.get(req.n_nodes)
-        stream = self._stream(slot)
-        plans, rows = self._chunks(req)
-        st = _State(req=req, stream=stream, phantom=ph, plans=plans, rows=rows)
-        N = ph["mesh"].n_nodes
-        st.work = N * req.n_samples
-        graph = self._graph(req.n_nodes, slot)
-        with torch.cuda.stream(stream):
-            # spoiled-GRE steady state: the phantom class's captured TR loop,
-            # replayed on this slot's stream, then copied out of the graph's
-            # buffers before the slot's next replay (same stream, in order)
-            graph.replay()
-            m = (graph.store_xy[0].to(torch.complex128) * ph["M0"]).reshape(-1, 1)
-            st.nodes = device.SignalNodes.from_arrays(ph["nodes"], ph["phi"], ph["T2s"], m, weights=ph["w"],
-                                                      device=self.dev, stream=stream)
-            st.out = torch.empty((req.n_samples, 1), dtype=torch.complex128, device=self.dev)
-            need = max(p.workspace_bytes(req.precision, N, 1) for p in plans)
-            st.ws = torch.empty(max(need, 1), dtype=torch.uint8, device=self.dev)
-        return st
+    def prepare(self, requests, slots: int = 0, max_batch: int = 8) -> None:
+        """Service start-up: the node pool, the plans of ``requests``' trajectories
+        and the steady-state graphs of every capacity class a batch of up to
+        ``max_batch`` of the largest request can need."""
+        if any(r.n_nodes > self.pool.n_max for r in requests):
+            raise ValueError("a request exceeds the node pool")
+        for req in requests:
+            self._chunks(req)
+        top = max((r.n_nodes for r in requests), default=1)
+        cap = self.graphs.capacity(1)
+        while cap <= self.graphs.capacity(max_batch * top):
+            self.graphs.get(cap)
+            cap *= 2
+        for slot in range(slots):
+            self._stream(slot)
+
+    def footprint(self, req: ScanRequest) -> int:
+        N, S = req.n_nodes, req.n_samples
+        plans, _ = self._chunks(req)
+        ws = max(p.workspace_bytes(req.precision, N, 1) for p in plans)
+        # tissue (6 rows), Bloch table (6), signal table (7), weights, output
+        return N * 8 * (6 + 6 + 7 + 1) + ws + S * 16
+
+    def start(self, req: ScanRequest, slot: int):
+        return self.start_batch([(req, slot)])[0]
+
+    def start_batch(self, admitted):
+        """Set up the admitted requests together: tissue, weights and Bloch
+        node tables on the batch stream, one steady-state replay for all of
+        them, then each request's signal nodes on its own slot stream."""
+        import torch
+
+        from . import device
+
+        pool = self.pool
+        graphs = self.graphs
+        bs = self.batch_stream
+        bs.wait_stream(torch.cuda.current_stream(self.dev))
+        states, offs, off = [], [], 0
+        with torch.cuda.stream(bs):
+            for req, slot in admitted:
+                if req.n_nodes > pool.n_max:
+                    raise ValueError(f"request {req.rid}: {req.n_nodes} nodes exceed the pool ({pool.n_max})")
+                plans, rows = self._chunks(req)
+                st = _State(req=req, stream=self._stream(slot), plans=plans, rows=rows)
+                N = req.n_nodes
+                st.work = N * req.n_samples
+                st.tissue = draw_tissue_device(N, req.seed, self.dev)
+                st.w = pool.weights(N, stream=bs)
+                st.bloch = device.BlochNodes.from_arrays(pool.nodes[:N], st.tissue["T1"], st.tissue["T2"],
+                                                         st.tissue["dB"], device=self.dev, stream=bs)
+                states.append(st)
+                offs.append(off)
+                off += N
+            cls = graphs.get(off)
+            for st, o in zip(states, offs):
+                cls["packed"][:, o:o + st.req.n_nodes].copy_(st.bloch.packed)
+            cls["graph"].replay()
+            store = cls["store_xy"][0]
+            for st, o in zip(states, offs):
+                N = st.req.n_nodes
+                st.mxy = (store[o:o + N].to(torch.complex128) * st.tissue["M0"]).reshape(-1, 1)
+            done = torch.cuda.Event()
+            done.record(bs)
+        for st in states:
+            req, N = st.req, st.req.n_nodes
+            s = st.stream
+            s.wait_event(done)
+            with torch.cuda.stream(s):
+                st.nodes = device.SignalNodes.from_arrays(pool.nodes[:N], st.tissue["phi"], st.tissue["T2s"],
+                                                          st.mxy, weights=st.w, device=self.dev, stream=s)
+                st.out = torch.empty((req.n_samples, 1), dtype=torch.complex128, device=self.dev)
+                need = max(p.workspace_bytes(req.precision, N, 1) for p in st.plans)
+                st.ws = torch.empty(max(need, 1), dtype=torch.uint8, device=self.dev)
+            # tensors made on the batch stream and used on this one
+            for t in (st.w, st.mxy, st.tissue["phi"], st.tissue["T2s"]):
+                t.record_stream(s)
+            st.tissue = st.bloch = None
+        return states
 
     def step(self, st: _State, ahead: int) -> bool:
         import torch
@@ -447,5 +529,5 @@ class GpuBackend:
 
     def finish(self, st: _State):
         res = st.host.numpy().reshape(*st.req.shape, 1).copy()
-        st.nodes = st.out = st.ws = st.host = None
+        st.nodes = st.out = st.ws = st.host = st.w = st.mxy = None
         return res

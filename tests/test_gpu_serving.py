@@ -59,10 +59,12 @@ def test_c4_sizes_alone_equals_mix_and_oracle(gpu, oracle):
     """c4 at its stated sizes: a 1.1M-node FP64 256^2 x 32 request, a mixed
     512-sample radial one and stream draws run concurrently; each k-space is
     bitwise the one the request gets alone, and the FP64 request matches the
-    FP64 oracle chain (lumped mass, spoiled-GRE steady state stored as
-    complex64 as BlochSolver keeps it, signal) at 16 sampled k-points."""
+    FP64 oracle chain on its exact phantom (the pool's first N nodes and the
+    elements among them, its seeded tissue; lumped mass, spoiled-GRE steady
+    state stored as complex64 as BlochSolver keeps it, signal) at 16 sampled
+    k-points."""
     from feelmri_amd import sequences
-    from feelmri_amd.serving import request_trajectory
+    from feelmri_amd.serving import draw_tissue_device, request_trajectory
 
     reqs = _c4_mix()
     be = GpuBackend(dev=gpu)
@@ -77,18 +79,22 @@ def test_c4_sizes_alone_equals_mix_and_oracle(gpu, oracle):
         assert np.array_equal(a, b), f"request {req.rid} ({req.n_nodes} nodes, {req.traj} {req.shape}, " \
                                      f"{req.precision}) differs alone vs in the mix"
     big = reqs[0]
-    ph = be.store.get(big.n_nodes)
-    m, tis = ph["mesh"], ph["tissue"]
-    w = oracle.mass_p1_lumped(m.elems, m.nodes)
-    blocks = [b.as_dict() for b in be._steady_blocks()]
-    mxy, _, _, _ = oracle.bloch_solver_solve(blocks, m.nodes, tis.T1, tis.T2, tis.delta_B_mT, 1.0,
-                                             sequences.GAMMA, np.zeros(m.n_nodes), np.ones(m.n_nodes))
-    mxy = mxy[:, -1].astype(np.complex64).astype(np.complex128) * tis.M0
+    N = big.n_nodes
+    pool = be.pool
+    nodes = pool.nodes[:N].cpu().numpy()
+    elems = pool.elems[:pool.colouring.n_elems_below(N)]
+    assert elems.max() < N
+    tis = {k: v.cpu().numpy() for k, v in draw_tissue_device(N, big.seed, gpu).items()}
+    w = oracle.mass_p1_lumped(elems, nodes)
+    blocks = [b.as_dict() for b in be.steady_blocks()]
+    mxy, _, _, _ = oracle.bloch_solver_solve(blocks, nodes, tis["T1"], tis["T2"], tis["dB"], 1.0,
+                                             sequences.GAMMA, np.zeros(N), np.ones(N))
+    mxy = mxy[:, -1].astype(np.complex64).astype(np.complex128) * tis["M0"]
     kloc, t = request_trajectory(big)
     rng = np.random.default_rng(5)
     idx = np.sort(rng.choice(t.size, 16, replace=False))
-    ref = oracle.signal_samples(*(np.asarray(k).ravel()[idx] for k in kloc), t.ravel()[idx], m.nodes,
-                                tis.delta_omega, tis.T2star, w[:, None] * mxy[:, None], n_threads=16)
+    ref = oracle.signal_samples(*(np.asarray(k).ravel()[idx] for k in kloc), t.ravel()[idx], nodes,
+                                tis["phi"], tis["T2s"], w[:, None] * mxy[:, None], n_threads=16)
     got = mix.results[big.rid].kspace.reshape(-1, 1)[idx]
     err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert err < 1e-8, err
@@ -104,37 +110,43 @@ def test_budget_queues_requests(gpu):
     assert all(r.error is None for r in rep.results.values())
 
 
-def test_steady_state_graph_replays_match_eager(gpu):
-    """The captured TR loop gives the eager loop's result bit for bit on every
-    replay, with eager work on the device between the replays."""
+def test_batched_steady_state_replays_match_per_request_blocks(gpu):
+    """One graph replay of the batched TR loop gives every request of the
+    batch, bit for bit, the steady state of its own block-by-block eager
+    solve (an integrator launch and a store/spoil launch per block), on every
+    replay, in batches of different composition and capacity class, with
+    other device work in between."""
     import torch
 
     from feelmri_amd import device, sequences
-    from feelmri_amd.serving import GpuBackend, SteadyStateGraph
+    from feelmri_amd.serving import draw_tissue_device
 
-    be = GpuBackend(dev=gpu, n_dummy=4)
-    ph = be.store.get(12_000)
-    blocks = be._steady_blocks()
-    N = ph["mesh"].n_nodes
-    mxy = torch.zeros(N, dtype=torch.complex128, device=gpu)
-    mz = torch.ones(N, dtype=torch.float64, device=gpu)
-    sx = torch.zeros((1, N), dtype=torch.complex64, device=gpu)
-    sz = torch.zeros((1, N), dtype=torch.float32, device=gpu)
-    for b in blocks:
-        ev = device.EventTable(b.rf, b.G, b.dt, device=gpu)
-        device.bloch(ph["bloch"], ev, 1.0, sequences.GAMMA, mxy, mz, mxy_out=mxy, mz_out=mz)
-        device.bloch_store_spoil(mxy, mz, sx, sz, col=0 if b.store else -1, spoil=not b.empty)
-    want_xy, want_z = sx[0].cpu().numpy(), sz[0].cpu().numpy()
-    assert np.abs(want_xy).max() > 0.05          # a real steady state, not zeros
-
-    s = torch.cuda.Stream(device=gpu)
-    g = SteadyStateGraph(ph, blocks, gpu, s)
-    for rep in range(4):
-        with torch.cuda.stream(s):
-            g.replay()
-        s.synchronize()
-        assert np.array_equal(g.store_xy[0].cpu().numpy(), want_xy), f"replay {rep}"
-        assert np.array_equal(g.store_z[0].cpu().numpy(), want_z), f"replay {rep}"
-        # eager device work between replays, touching the graph's buffers too
-        g.mxy.fill_(3.0)
-        torch.cuda.synchronize()
+    be = GpuBackend(dev=gpu, n_dummy=4, n_max=60_000)
+    pool = be.pool
+    blocks = be.steady_blocks()
+    reqs = [ScanRequest(0, 0.0, 12_345, "cart2d", (16, 16, 1), "f64", 7),
+            ScanRequest(1, 0.0, 50_000, "cart2d", (16, 16, 1), "mixed", 8),
+            ScanRequest(2, 0.0, 1_000, "cart2d", (16, 16, 1), "f64", 9)]
+    want = []
+    for r in reqs:
+        tis = draw_tissue_device(r.n_nodes, r.seed, gpu)
+        bn = device.BlochNodes.from_arrays(pool.nodes[:r.n_nodes], tis["T1"], tis["T2"], tis["dB"], device=gpu)
+        mxy = torch.zeros(r.n_nodes, dtype=torch.complex128, device=gpu)
+        mz = torch.ones(r.n_nodes, dtype=torch.float64, device=gpu)
+        sx = torch.zeros((1, r.n_nodes), dtype=torch.complex64, device=gpu)
+        sz = torch.zeros((1, r.n_nodes), dtype=torch.float32, device=gpu)
+        for b in blocks:
+            ev = device.EventTable(b.rf, b.G, b.dt, device=gpu)
+            device.bloch(bn, ev, 1.0, sequences.GAMMA, mxy, mz, mxy_out=mxy, mz_out=mz)
+            device.bloch_store_spoil(mxy, mz, sx, sz, col=0 if b.store else -1, spoil=not b.empty)
+        want.append((sx[0].to(torch.complex128) * tis["M0"]).cpu().numpy())
+    assert np.abs(want[1]).max() > 0.05            # a real steady state, not zeros
+    for rep in range(3):
+        for batch in ([0, 1, 2], [1], [2, 0]):
+            states = be.start_batch([(reqs[i], slot) for slot, i in enumerate(batch)])
+            torch.cuda.synchronize()
+            for i, st in zip(batch, states):
+                got = st.mxy.reshape(-1).cpu().numpy()
+                assert np.array_equal(got, want[i]), f"replay {rep}, batch {batch}, request {i}"
+            torch.ones(1 << 20, device=gpu).mul_(3.0)   # other device work between replays
+            torch.cuda.synchronize()
