@@ -43,6 +43,7 @@ _SIGNATURES = {
     "fm_signal_workspace_size": [_p, _int, _i64, _int, _int, ctypes.POINTER(_sz)],
     "fm_signal_nodes_pack": [_i64, _int, _int, _p, _p, _p, _p, _p, _p, _p],
     "fm_signal_execute": [_p, _int, _i64, _int, _p, _p, _int, _p, _p, _sz, _p, _p],
+    "fm_signal_execute_noalloc": [_p, _int, _i64, _int, _p, _p, _int, _p, _p, _p],
     "fm_signal_plan_set_timing": [_p, _int],
     "fm_signal_plan_kernel_ms": [_p, ctypes.POINTER(_dbl), ctypes.POINTER(_int)],
     "fm_csr_spmv_c128": [_i64, _int, _p, _p, _p, _p, _p, _p],

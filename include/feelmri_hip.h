@@ -129,6 +129,22 @@ int fm_signal_execute(const fm_signal_plan *plan, int precision, int64_t n_nodes
                       const double *motion_coef, void *workspace, size_t workspace_bytes,
                       double *out, void *stream);
 
+/* Size of the library's static signal workspace (bytes per device). */
+#define FM_SIGNAL_SCRATCH_BYTES (128u << 20)
+
+/* fm_signal_execute with no workspace argument and no allocation: the
+ * workspace is a file-scope device array of FM_SIGNAL_SCRATCH_BYTES that each
+ * device holds until the process ends (its address is looked up on the
+ * device's first call).  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Evaluations whose
+ * fm_signal_workspace_size exceeds the array return FM_ERR_WORKSPACE.  All
+ * calls on one device share the array, so calls that overlap on one device
+ * (other streams, other host threads) race; serialise them. */
+int fm_signal_execute_noalloc(const fm_signal_plan *plan, int precision, int64_t n_nodes, int n_enc,
+                              const double *packed, const float *modes, int n_modes,
+                              const double *motion_coef, double *out, void *stream);
+
 /* Measurement support (no reference counterpart): when enabled, each
  * fm_signal_execute records a pair of HIP events on its stream around the
  * dominant kernel (the GEMM, or the direct-sum launches), for up to
