@@ -56,7 +56,7 @@ _SIGNATURES = {
     "fm_mass_p1_color": [_i64, _i64, _p, _p, _p, _int, ctypes.POINTER(_int)],
     "fm_mass_p1_lumped": [_i64, _i64, _p, _int, _p, _p, _p, _int, _p, _p],
     "fm_mass_p1_incidence": [_i64, _i64, _p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)],
-    "fm_mass_p1_lumped_gather": [_i64, _i64, _p, _int, _p, _p, _p, _p, _i64, _p, _p, _p],
+    "fm_mass_p1_lumped_gather": [_i64, _i64, _p, _p, _int, _p, _p, _p, _p, _i64, _p, _p, _p],
     "fm_mass_p1_element_matrices": [_i64, _p, _int, _p, _p, _p],
     "fm_pod_tensordot": [_int, _i64, _i64, _i64, _p, _p, _p, _p],
     "fm_pod_positions": [_i64, _p, _p, _int, _p, _p, _p],

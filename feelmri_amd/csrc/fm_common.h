@@ -44,7 +44,7 @@ void set_error(const char *fmt, ...);
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
-inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // (double)(float)pi / pi: the reference forms 2*pi from a float constant
 // (cpp/feelmri/MRI.h:13), so every k.x phase carries this factor.

@@ -14,17 +14,20 @@
 //
 // The gather form does the same sums in two launches over a host-built
 // incidence table in sliced-ELL layout: node n's j-th incident element in
-// colour order is stored (as an element id) in slot chunk_offsets[n/64] +
-// 64 j + n%64.  Launch 1, per element in element order, writes |det J_e| =
-// 6 |T_e| once into a per-element array (coalesced connectivity reads and
-// stores).  Launch 2, per node, reads its deg(n) element ids -- a wave reads
-// 64 consecutive ids per step -- and adds the elements' determinants.  A
-// node has at most one element per colour, so it performs the scatter's
-// additions in the scatter's order; both forms add with an explicit
-// fma(|det J_e|, 1/24, w), so they give the same bits.  The determinants a
-// node chunk gathers belong to elements that share its nodes, which sit in a
-// few short runs of element ids on a mesh numbered along its grid, so the
-// gathers hit lines that neighbouring nodes fetch too.
+// colour order is stored, as its position k in the colour order, in slot
+// chunk_offsets[n/64] + 64 j + n%64.  Launch 1 walks the elements in their
+// own order (coalesced connectivity, coordinates re-read from L2) and writes
+// |det J| = 6 |T| once per element into dets[pos[e]], pos the inverse of the
+// colour permutation.  Launch 2, per node, reads its deg(n) positions -- a
+// wave reads 64 consecutive slots per step -- and adds their determinants.
+// The j-th elements of neighbouring nodes mostly share a colour, and within
+// a colour positions follow element order, so the determinants gathered
+// for a chunk of nodes lie close together.  Both launches deal contiguous
+// ranges of elements / nodes onto each XCD, so that the lines of one range
+// (coordinates, determinant writes of one colour, determinant reads) stay in
+// that XCD's L2.  A node has at most one element per colour, so it performs
+// the scatter's additions in the scatter's order; both forms add with an
+// explicit fma(|det J_e|, 1/24, w), so they give the same bits.
 #include <algorithm>
 #include <vector>
 
@@ -68,33 +71,46 @@ __global__ void k_lumped_colour(int64_t begin, int64_t end, const int32_t *__res
     for (int i = 0; i < 4; ++i) w[ids[i]] = fma(det, 1.0 / 24.0, w[ids[i]]);  // += |T|/4
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_abs_dets(int64_t E, const int32_t *__restrict__ elems,
-                                                  const T *__restrict__ nodes, double *__restrict__ dets)
+// Workgroup b of a launch of ceil(n / 8) * 8 workgroups takes range
+// (b % 8) * ceil(n / 8) + b / 8 of n ranges: workgroups are dealt onto the
+// 8 XCDs round-robin, so each XCD runs one contiguous eighth of the ranges.
+__device__ inline int64_t xcd_range(int64_t n_ranges)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    int32_t ids[4];
-    dets[e] = tet_abs_det<T>(elems + 4 * e, nodes, ids);
+    const int64_t per = (n_ranges + 7) / 8;
+    return (int64_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
 }
 
-// Elements with ids >= n_elems are skipped: a mesh whose elements are
-// ordered by their largest node id gives, for its first n_nodes nodes, the
-// sub-mesh of its first n_elems elements without a table of its own.
+inline unsigned xcd_grid(int64_t n_ranges) { return (unsigned)(((n_ranges + 7) / 8) * 8); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_abs_dets(int64_t E, const int32_t *__restrict__ elems,
+                                                  const int32_t *__restrict__ pos, const T *__restrict__ nodes,
+                                                  double *__restrict__ dets)
+{
+    const int64_t e = xcd_range(cdiv(E, 256)) * 256 + threadIdx.x;
+    if (e >= E) return;
+    int32_t ids[4];
+    dets[pos ? (int64_t)pos[e] : e] = tet_abs_det<T>(elems + 4 * e, nodes, ids);
+}
+
+// Positions >= n_elems are skipped: with the identity order on a mesh whose
+// elements are ordered by their largest node id, the first n_nodes nodes and
+// the first n_elems elements form a sub-mesh weighted without a table of its
+// own.
 __global__ __launch_bounds__(256) void k_lumped_gather(int64_t n_nodes, int64_t n_elems,
                                                        const int64_t *__restrict__ chunk_offsets,
                                                        const int32_t *__restrict__ degree,
                                                        const int32_t *__restrict__ incident,
                                                        const double *__restrict__ dets, double *__restrict__ w)
 {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = xcd_range(cdiv(n_nodes, 256)) * 256 + threadIdx.x;
     if (n >= n_nodes) return;
     const int32_t *ids = incident + chunk_offsets[n / 64] + n % 64;
     const int d = degree[n];
     double acc = 0.0;
     for (int j = 0; j < d; ++j) {
-        const int32_t e = ids[64 * j];
-        if (e < n_elems) acc = fma(dets[e], 1.0 / 24.0, acc);  // += |T|/4
+        const int32_t k = ids[64 * j];
+        if (k < n_elems) acc = fma(dets[k], 1.0 / 24.0, acc);  // += |T|/4
     }
     w[n] = acc;
 }
@@ -241,16 +257,16 @@ extern "C" int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int3
         seen[(size_t)e] = 1;
         for (int i = 0; i < 4; ++i) {
             const int64_t n = elems[4 * e + i];
-            incident[chunk_offsets[n / 64] + 64 * (int64_t)next[(size_t)n]++ + n % 64] = (int32_t)e;
+            incident[chunk_offsets[n / 64] + 64 * (int64_t)next[(size_t)n]++ + n % 64] = (int32_t)k;
         }
     }
     return FM_OK;
 }
 
-extern "C" int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const int32_t *elems, int dtype,
-                                        const void *nodes, const int64_t *chunk_offsets, const int32_t *degree,
-                                        const int32_t *incident, int64_t n_slots, double *det_scratch, double *w,
-                                        void *stream)
+extern "C" int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const int32_t *elems, const int32_t *pos,
+                                        int dtype, const void *nodes, const int64_t *chunk_offsets,
+                                        const int32_t *degree, const int32_t *incident, int64_t n_slots,
+                                        double *det_scratch, double *w, void *stream)
 {
     FM_CHECK_ARG(n_elems >= 0 && n_nodes >= 0 && n_slots >= 0, "negative size");
     FM_CHECK_ARG(dtype == FM_DTYPE_F32 || dtype == FM_DTYPE_F64, "unknown dtype %d", dtype);
@@ -261,19 +277,19 @@ extern "C" int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const 
                  (long long)n_slots);
     if (n_nodes == 0) return FM_OK;
     FM_CHECK_ARG(w && chunk_offsets && degree, "NULL array");
+    FM_CHECK_ARG(n_elems <= 0x7fffffffLL && n_nodes <= 0x7fffffffLL * 32, "mesh too large for one launch");
     hipStream_t st = as_stream(stream);
     if (n_elems > 0) {
         FM_CHECK_ARG(elems && nodes && incident && det_scratch, "NULL array");
-        FM_CHECK_ARG(n_elems <= 0x7fffffffLL * 256, "too many elements for one launch");
-        const unsigned eblocks = (unsigned)cdiv(n_elems, 256);
+        const unsigned eblocks = xcd_grid(cdiv(n_elems, 256));
         if (dtype == FM_DTYPE_F64)
-            k_abs_dets<double><<<eblocks, 256, 0, st>>>(n_elems, elems, (const double *)nodes, det_scratch);
+            k_abs_dets<double><<<eblocks, 256, 0, st>>>(n_elems, elems, pos, (const double *)nodes, det_scratch);
         else
-            k_abs_dets<float><<<eblocks, 256, 0, st>>>(n_elems, elems, (const float *)nodes, det_scratch);
+            k_abs_dets<float><<<eblocks, 256, 0, st>>>(n_elems, elems, pos, (const float *)nodes, det_scratch);
         FM_LAUNCH_CHECK("k_abs_dets");
     }
-    k_lumped_gather<<<(unsigned)cdiv(n_nodes, 256), 256, 0, st>>>(n_nodes, n_elems, chunk_offsets, degree,
-                                                                  incident, det_scratch, w);
+    k_lumped_gather<<<xcd_grid(cdiv(n_nodes, 256)), 256, 0, st>>>(n_nodes, n_elems, chunk_offsets, degree,
+                                                                   incident, det_scratch, w);
     FM_LAUNCH_CHECK("k_lumped_gather");
     return FM_OK;
 }

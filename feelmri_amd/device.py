@@ -372,14 +372,21 @@ class P1Colouring:
     """Host greedy colouring of a tet mesh and each node's incident elements
     in colour order, resident on the device.
 
-    A mesh whose greedy colouring needs more than 64 colours (a node with
-    many incident elements, e.g. an unstructured Delaunay mesh) keeps the
-    elements in their own order instead (``method == "element-order"``):
-    the gather then adds each node's contributions in ascending element
-    order -- deterministic, but no longer bitwise equal to a coloured
-    scatter, which is unavailable for such a mesh (``n_colours == 0``)."""
+    The gather writes each element's determinant at its position in the
+    colour order (``pos``), so that the incidence lists, which hold
+    positions, gather neighbouring entries for neighbouring nodes.  ``order="element"`` keeps the elements in their own order
+    (identity permutation): a node then adds its contributions in ascending
+    element order, and on a mesh whose elements are ordered by their largest
+    node id (``prefix_ordered``) a node prefix is weighted from the same
+    tables.  A mesh whose greedy colouring needs more than 64 colours (a node
+    with many incident elements, e.g. an unstructured Delaunay mesh) falls
+    back to element order (``method == "element-order"``,
+    ``n_colours == 0``): deterministic, but not bitwise equal to a coloured
+    scatter, which is unavailable for such a mesh."""
 
-    def __init__(self, elems, n_nodes: int, device=None):
+    def __init__(self, elems, n_nodes: int, device=None, order: str = "colour"):
+        if order not in ("colour", "element"):
+            raise ValueError(f"order must be 'colour' or 'element', got {order!r}")
         device = torch.device(device) if device is not None else default_device()
         el = np.ascontiguousarray(np.asarray(elems), dtype=np.int32)
         if el.ndim != 2 or el.shape[1] != 4:
@@ -392,31 +399,43 @@ class P1Colouring:
                                           perm.ctypes.data_as(ctypes.c_void_p),
                                           offs.ctypes.data_as(ctypes.c_void_p), 64, ctypes.byref(nc))
         if rc == _lib.FM_ERR_UNSUPPORTED:
-            perm = np.arange(E, dtype=np.int32)
-            self.method, self.n_colours = "element-order", 0
+            self.n_colours = 0
             self.offsets = np.zeros(1, dtype=np.int64)
         else:
             _lib.check(rc, "fm_mass_p1_color")
-            self.method, self.n_colours = "colour-order", nc.value
+            self.n_colours = nc.value
             self.offsets = np.ascontiguousarray(offs[: self.n_colours + 1])
+        if self.n_colours == 0 or order == "element":
+            gather_perm = np.arange(E, dtype=np.int32)
+            self.method = "element-order"
+        else:
+            gather_perm = perm
+            self.method = "colour-order"
         self.n_elems, self.n_nodes = E, int(n_nodes)
         chunk_offsets = np.empty((self.n_nodes + 63) // 64 + 1, dtype=np.int64)
         degree = np.empty(self.n_nodes, dtype=np.int32)
         n_slots = ctypes.c_int64()
         vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
         # size query, then the table
-        _lib.call("fm_mass_p1_incidence", E, self.n_nodes, vp(el), vp(perm), vp(chunk_offsets), vp(degree), None,
-                  0, ctypes.byref(n_slots))
+        _lib.call("fm_mass_p1_incidence", E, self.n_nodes, vp(el), vp(gather_perm), vp(chunk_offsets), vp(degree),
+                  None, 0, ctypes.byref(n_slots))
         incident = np.empty(max(n_slots.value, 1), dtype=np.int32)
-        _lib.call("fm_mass_p1_incidence", E, self.n_nodes, vp(el), vp(perm), vp(chunk_offsets), vp(degree),
+        _lib.call("fm_mass_p1_incidence", E, self.n_nodes, vp(el), vp(gather_perm), vp(chunk_offsets), vp(degree),
                   vp(incident), incident.size, ctypes.byref(n_slots))
         self.n_slots = int(n_slots.value)
-        # elements ordered by their largest node id: a node prefix's sub-mesh
-        # is an element prefix (see lumped_mass)
+        # elements ordered by their largest node id, kept in that order: a
+        # node prefix's sub-mesh is an element prefix (see lumped_mass)
         self._emax = el.max(axis=1) if E else np.zeros(0, dtype=np.int32)
-        self.prefix_ordered = bool(np.all(np.diff(self._emax) >= 0))
+        self.prefix_ordered = self.method == "element-order" and bool(np.all(np.diff(self._emax) >= 0))
         self.elems = torch.from_numpy(el).to(device)
-        self.perm = torch.from_numpy(perm).to(device)
+        if self.method == "element-order":
+            self.pos = None
+        else:  # each element's position in the colour order
+            pos = np.empty(E, dtype=np.int32)
+            pos[perm] = np.arange(E, dtype=np.int32)
+            self.pos = torch.from_numpy(pos).to(device)
+        # the colouring itself serves the coloured scatter whatever the order
+        self.perm = torch.from_numpy(perm if self.n_colours else gather_perm).to(device)
         self.chunk_offsets = torch.from_numpy(chunk_offsets).to(device)
         self.degree = torch.from_numpy(degree).to(device)
         self.incident = torch.from_numpy(incident).to(device)
@@ -444,17 +463,19 @@ def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=No
                 n_nodes: int | None = None) -> torch.Tensor:
     """w_i = sum_{e ni i} |T_e| / 4 (float64, (N,)): each element writes
     |det J_e| once, then each node adds its elements' determinants in colour
-    order (element order for a mesh without a colouring).
+    order (element order for ``order="element"`` or a mesh without a
+    colouring).
 
     ``n_nodes`` below the mesh's node count weights the sub-mesh of the first
     ``n_nodes`` nodes: the elements whose nodes all lie below it, which are
     the leading elements when the mesh's elements are ordered by their
-    largest node id (``colouring.prefix_ordered``).  ``nodes`` then needs only
-    those rows."""
+    largest node id and kept in element order (``colouring.prefix_ordered``).
+    ``nodes`` then needs only those rows."""
     n = colouring.n_nodes if n_nodes is None else int(n_nodes)
     if n != colouring.n_nodes:
         if not colouring.prefix_ordered:
-            raise ValueError("a node prefix needs elements ordered by their largest node id")
+            raise ValueError("a node prefix needs elements ordered by their largest node id, kept in element "
+                             "order (P1Colouring(..., order='element'))")
         if not 0 <= n <= colouring.n_nodes:
             raise ValueError(f"n_nodes {n} outside [0, {colouring.n_nodes}]")
     E = colouring.n_elems_below(n)
@@ -467,7 +488,8 @@ def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=No
     dt_code = _lib.DTYPE_F64 if nodes.dtype == torch.float64 else _lib.DTYPE_F32
     q = torch.empty(max(E, 1), dtype=torch.float64, device=nodes.device)
     with torch.cuda.device(nodes.device):
-        _lib.call("fm_mass_p1_lumped_gather", E, n, ptr(colouring.elems), dt_code, ptr(nodes),
+        _lib.call("fm_mass_p1_lumped_gather", E, n, ptr(colouring.elems),
+                  ptr(colouring.pos) if colouring.pos is not None else None, dt_code, ptr(nodes),
                   ptr(colouring.chunk_offsets), ptr(colouring.degree), ptr(colouring.incident), colouring.n_slots,
                   ptr(q), ptr(out), stream_handle(stream))
     if isinstance(stream, torch.cuda.Stream):
@@ -477,7 +499,7 @@ def lumped_mass(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=No
 
 def lumped_mass_coloured(colouring: P1Colouring, nodes: torch.Tensor, out=None, stream=None) -> torch.Tensor:
     """The same weights by the coloured scatter: one launch per colour."""
-    if colouring.method != "colour-order":
+    if colouring.n_colours == 0:
         raise ValueError("this mesh has no colouring of at most 64 colours: use lumped_mass (the gather)")
     nodes, out, dt_code = _mass_args(colouring, nodes, out)
     with torch.cuda.device(nodes.device):

@@ -258,7 +258,7 @@ class NodePool:
         order = np.argsort(m.elems.max(axis=1), kind="stable")
         self.elems = np.ascontiguousarray(m.elems[order])
         self.n_max = m.n_nodes
-        self.colouring = device.P1Colouring(self.elems, m.n_nodes, device=dev)
+        self.colouring = device.P1Colouring(self.elems, m.n_nodes, device=dev, order="element")
         assert self.colouring.prefix_ordered
         self.nodes = torch.from_numpy(m.nodes).to(dev)          # (n_max, 3) float64
         self.dev = dev

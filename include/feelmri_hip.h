@@ -237,30 +237,32 @@ int fm_mass_p1_lumped(int64_t n_elems, int64_t n_nodes, const int32_t *elems, in
                       int n_colors, double *w, void *stream);
 
 /* Host: each node's incident elements in the order of perm (colour order,
- * or element order for perm = identity), as element ids in the slots of a
- * sliced-ELL table: node n's j-th element is stored in slot
- * chunk_offsets[n / 64] + 64 j + n % 64; padding slots hold -1.  Outputs
- * (host): chunk_offsets (ceil(n_nodes / 64) + 1), degree (n_nodes), n_slots
- * (the table length, chunk_offsets' last entry) and, unless incident is
- * NULL (a size query), incident (max_slots >= n_slots int32 entries).
- * elems and perm as for fm_mass_p1_color (host). */
+ * or element order for perm = identity), as their positions k in perm
+ * (perm[k] = e) in the slots of a sliced-ELL table: node n's j-th element is
+ * stored in slot chunk_offsets[n / 64] + 64 j + n % 64; padding slots hold
+ * -1.  Outputs (host): chunk_offsets (ceil(n_nodes / 64) + 1), degree
+ * (n_nodes), n_slots (the table length, chunk_offsets' last entry) and,
+ * unless incident is NULL (a size query), incident (max_slots >= n_slots
+ * int32 entries).  elems and perm as for fm_mass_p1_color (host). */
 int fm_mass_p1_incidence(int64_t n_elems, int64_t n_nodes, const int32_t *elems, const int32_t *perm,
                          int64_t *chunk_offsets, int32_t *degree, int32_t *incident, int64_t max_slots,
                          int64_t *n_slots);
 
-/* Lumped weights w_i = sum_{e ni i} |T_e| / 4 in two launches: each element
- * writes |det J_e| = 6 |T_e| into det_scratch[e], then node i adds the
+/* Lumped weights w_i = sum_{e ni i} |T_e| / 4 in two launches: element e
+ * writes |det J| = 6 |T| into det_scratch[pos[e]], then node i adds the
  * determinants of its degree[i] incident elements -- the additions of
  * fm_mass_p1_lumped, in the same order and with the same rounding, so the
- * result is bitwise the same when perm is a colouring.  chunk_offsets,
- * degree, incident from fm_mass_p1_incidence (copied to the device);
- * n_slots = the slot count it returned; det_scratch (n_elems float64,
- * overwritten); elems, nodes, w: all device.  The tables may belong to a
- * larger mesh whose elements are ordered by their largest node id: incident
- * ids >= n_elems are skipped, so its first n_nodes nodes and first n_elems
- * elements (those whose nodes all lie below n_nodes) are weighted as a mesh
- * of their own. */
-int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const int32_t *elems, int dtype,
+ * result is bitwise the same when perm is a colouring.  elems: the
+ * connectivity (device); pos: the inverse of the perm the incidence table
+ * was built with (pos[perm[k]] = k, device int32; NULL for perm = identity);
+ * chunk_offsets, degree, incident from fm_mass_p1_incidence (copied to the
+ * device); n_slots = the slot count it returned; det_scratch (n_elems
+ * float64, overwritten); nodes, w: device.  With perm = identity the tables
+ * may belong to a larger mesh whose elements are ordered by their largest
+ * node id: positions >= n_elems are skipped, so its first n_nodes nodes and
+ * first n_elems elements (those whose nodes all lie below n_nodes) are
+ * weighted as a mesh of their own. */
+int fm_mass_p1_lumped_gather(int64_t n_elems, int64_t n_nodes, const int32_t *elems, const int32_t *pos, int dtype,
                              const void *nodes, const int64_t *chunk_offsets, const int32_t *degree,
                              const int32_t *incident, int64_t n_slots, double *det_scratch,
                              double *w, void *stream);

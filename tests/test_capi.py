@@ -113,9 +113,9 @@ def test_colouring_is_a_valid_partition():
 
 def test_incidence_lists_each_nodes_elements_in_colour_order():
     """The host incidence table behind the gather lumped mass: node n's slots
-    hold exactly the elements containing n, in the order they appear in the
-    colour permutation (brute force on a small mesh); padding slots hold -1;
-    the size query returns the same slot count."""
+    hold the colour-order positions k (perm[k] = e) of exactly the elements
+    containing n, ascending (brute force on a small mesh); padding slots hold
+    -1; the size query returns the same slot count."""
     import numpy as np
 
     from feelmri_amd import _lib, phantoms
@@ -138,7 +138,7 @@ def test_incidence_lists_each_nodes_elements_in_colour_order():
     args = lambda p: (E, N, vp(el), vp(p), vp(chunk_offsets), vp(degree), vp(inc), inc.size, ctypes.byref(n_slots))
     _lib.call("fm_mass_p1_incidence", *args(perm))
     for n in range(N):
-        want = [int(e) for e in perm if n in el[e]]
+        want = [k for k, e in enumerate(perm) if n in el[e]]
         assert degree[n] == len(want)
         c = n // 64
         width = (chunk_offsets[c + 1] - chunk_offsets[c]) // 64

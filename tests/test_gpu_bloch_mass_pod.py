@@ -344,6 +344,38 @@ def test_lumped_mass_unstructured_without_colouring(gpu):
         device.lumped_mass_coloured(col, torch.from_numpy(pts).to(gpu))
 
 
+def test_lumped_gather_element_order_and_node_prefix(gpu):
+    """order="element" on a colourable mesh: each node adds its elements in
+    ascending element order -- the oracle's weights to 1e-13; with the
+    elements sorted by their largest node id, every node prefix weighs the
+    sub-mesh of the elements below it, as the oracle does on that sub-mesh."""
+    import torch
+
+    import oracle
+    from feelmri_amd import device
+
+    m = phantoms.ellipsoid_mesh(n=14)
+    el = np.ascontiguousarray(m.elems[np.argsort(m.elems.max(axis=1), kind="stable")])
+    col = device.P1Colouring(el, m.n_nodes, device=gpu, order="element")
+    assert col.method == "element-order" and col.n_colours > 0 and col.prefix_ordered
+    nd = torch.from_numpy(m.nodes).to(gpu)
+    w = device.lumped_mass(col, nd).cpu().numpy()
+    ref = oracle.mass_p1_lumped(el, m.nodes)
+    assert np.max(np.abs(w - ref) / ref) <= 1e-13
+    # the coloured scatter stays available and agrees to rounding
+    c = device.lumped_mass_coloured(col, nd).cpu().numpy()
+    assert np.max(np.abs(c - ref) / ref) <= 1e-13
+    for n in (1, 97, m.n_nodes // 3, m.n_nodes - 1):
+        E = col.n_elems_below(n)
+        assert el[:E].max(initial=-1) < n and (E == el.shape[0] or el[E].max() >= n)
+        wp = device.lumped_mass(col, nd[:n].contiguous(), n_nodes=n).cpu().numpy()
+        rp = oracle.mass_p1_lumped(el[:E], m.nodes[:n]) if E else np.zeros(n)
+        rp = np.pad(rp, (0, n - rp.size))
+        assert np.allclose(wp, rp, rtol=1e-13, atol=0.0), n
+    with pytest.raises(ValueError, match="element order"):
+        device.lumped_mass(device.P1Colouring(el, m.n_nodes, device=gpu), nd, n_nodes=5)
+
+
 def test_lumped_gather_without_elements(gpu):
     import torch
 
