@@ -437,20 +437,31 @@ def run_flow(args, dev) -> dict:
     value = work / (ms * 1e-3)
     log(f"{args.form}: {ms:.1f} ms/step ({n_ph} phases), {value:.3e} node*k/s, dominant kernel {k_ms:.2f} ms/phase")
     forms = {args.form: {"ms_per_step": ms, "node_ksamples_per_s": value, "kernel_ms_per_phase": k_ms}}
-    err64 = None
+    err64 = err64_img = None
     out0 = outs[0].cpu().numpy()
     o64_0 = None
     if not args.no_f64 and args.form != "f64":
         o64, ms64, k64 = time_it("f64", 1, 0)
         forms["f64"] = {"ms_per_step": ms64, "node_ksamples_per_s": work / (ms64 * 1e-3), "kernel_ms_per_phase": k64}
         num = den = 0.0
+        inum = iden = 0.0
+        from feelmri_amd import recon
+        kshape = tuple(np.asarray(wl.kloc[0]).shape)
+        geo = recon.CartesianGeometry(kshape)          # one readout sample per voxel (no oversampling)
         for a, b in zip(outs, o64):
             num += float(torch.linalg.vector_norm(a - b)) ** 2
             den += float(torch.linalg.vector_norm(b)) ** 2
+            # CartesianRecon image of all 4 encodings, mixed vs FP64
+            ia = recon.CartesianRecon(a.cpu().numpy().reshape(*kshape, 4), geo, device=dev)
+            ib = recon.CartesianRecon(b.cpu().numpy().reshape(*kshape, 4), geo, device=dev)
+            inum += float(np.linalg.norm(ia - ib)) ** 2
+            iden += float(np.linalg.norm(ib)) ** 2
         err64 = (num / den) ** 0.5
+        err64_img = (inum / iden) ** 0.5
         o64_0 = o64[0].cpu().numpy()
         del o64
-        log(f"f64: {ms64:.1f} ms/step; {args.form} vs f64 relL2 {err64:.2e} over all phases; speed-up {ms64 / ms:.2f}x")
+        log(f"f64: {ms64:.1f} ms/step; {args.form} vs f64 relL2 {err64:.2e} over all phases (image "
+            f"{err64_img:.2e}); speed-up {ms64 / ms:.2f}x")
     # FP64 oracle on sampled k-points of phase 0: positions at each sample's t
     rng = np.random.default_rng(7)
     idx = np.sort(rng.choice(wl.n_samples, size=12, replace=False))
@@ -514,7 +525,8 @@ def run_flow(args, dev) -> dict:
         "config": {"workload": wl.info["desc"], "config": "c3", "nodes": m.n_nodes, "elements": m.n_elems,
                    "k_samples": wl.n_samples, "encodings": 4, "phases": n_ph, "method": plan.method,
                    "form": args.form, "arithmetic": _arithmetic(plan.method, args.form)},
-        "relL2_vs_f64_gpu": err64, "relL2_vs_oracle_sampled": err_or, "relL2_vs_cpu_baseline_readouts": err_cpu,
+        "relL2_vs_f64_gpu": err64, "image_relL2_vs_f64_gpu": err64_img, "relL2_vs_oracle_sampled": err_or,
+        "relL2_vs_cpu_baseline_readouts": err_cpu,
         "forms": forms, "roofline": roof, "cpu_baseline": cpu,
         "setup_ms": {"lumped_mass": scan_ms[0], "bloch_moving_acquisition": scan_ms[1],
                      "bloch_blocks": len(blocks)},
@@ -554,10 +566,16 @@ def measure_static(args, dev, wl, form: str, with_cpu: bool) -> dict:
             # CartesianRecon's Tukey-windowed image (Recon.py:13-90, before its resize)
             err_img_tukey = image_rel_l2(out_h.reshape(shape), out64_h.reshape(shape), "Tukey")
         elif wl.name.startswith("c2"):
-            from feelmri_amd.recon import adjoint_image_rel_l2
+            import numpy as np
 
-            # 2-D radial / spiral: direct adjoint DFT with ramp density compensation, 128^2 over the FOV
-            err_img = adjoint_image_rel_l2(wl.kloc[0], wl.kloc[1], out_h[:, 0], out64_h[:, 0], 0.24, n=128)
+            from feelmri_amd.recon import nufft_image_rel_l2
+
+            # 2-D radial / spiral: gridding-NUFFT adjoint with ramp density compensation on the
+            # smallest even grid over the 0.24 m FOV that holds the trajectory's band
+            kmax = float(np.max(np.hypot(wl.kloc[0], wl.kloc[1])))
+            n_img = 2 * int(np.ceil(kmax * 0.24)) + 2
+            err_img = nufft_image_rel_l2(wl.kloc[0], wl.kloc[1], out_h[:, 0], out64_h[:, 0], 0.24, n=n_img,
+                                         device=dev)
         log(f"f64: {ms64:.3f} ms/step; {form} vs f64 relL2 {err_f64:.2e} (image {err_img}); "
             f"speed-up {ms64 / ms_ev:.2f}x")
     w_h = w.cpu().numpy()
