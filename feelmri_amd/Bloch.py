@@ -8,12 +8,15 @@ Mxy is zeroed after every non-empty block -- the reference's artificial
 spoiling -- while Mz is always carried (:363-368).
 
 The state (Mxy complex128, Mz float64) stays in HBM between ``solve``
-calls (the reference keeps it in ``initial_Mxy`` / ``initial_Mz``).  Static
-nodes run a whole block range as one chained launch: the state stays in
-registers across blocks and the store/spoil steps happen in the kernel.
-Moving nodes take one integrator launch plus one store/spoil launch per
-block, with their motion tables built per block on the host.  Event tables
-are uploaded once and cached.
+calls (the reference keeps it in ``initial_Mxy`` / ``initial_Mz``).  A whole
+block range runs as one chained launch: the state stays in registers across
+blocks and the store/spoil steps happen in the kernel.  Moving nodes chain
+too: the motion modes are uploaded once and the per-step coefficients of
+every block (each at its block's time shift) form one table.  The event
+tables come from the host (uploaded once and cached) or, with
+``events="device"``, are built on the device from the blocks'
+specifications (``sequences.make_block``), bitwise the same tables, so that
+a captured graph rebuilds and integrates the whole chain with no host work.
 """
 from __future__ import annotations
 
@@ -21,7 +24,7 @@ import numpy as np
 import torch
 
 from . import Motion, device
-from .BlochSimulator import bloch_motion_tables, step_times
+from .BlochSimulator import step_times
 from .sequences import GAMMA, Block
 
 
@@ -51,6 +54,9 @@ class BlochSolver:
             self.mz = device.as_device(mz0, torch.float64, self.dev)
         self._events: dict[int, device.EventTable] = {}
         self._chains: dict[tuple[int, int], device.BlockChain] = {}
+        self._dev_events: dict[tuple[int, int], device.DeviceEvents] = {}
+        self._coefs: dict[tuple[int, int], torch.Tensor] = {}
+        self._modes = None
 
     def add_block(self, block: Block) -> None:
         self.blocks.append(block)
@@ -77,14 +83,74 @@ class BlochSolver:
             self._chains[key] = ch
         return ch
 
-    def solve_device(self, start: int = 0, end: int | None = None, stream=None, chain: bool = True):
+    def _device_events(self, start: int, end: int) -> device.DeviceEvents:
+        key = (start, end)
+        de = self._dev_events.get(key)
+        if de is None:
+            from .sequences import block_descriptions
+
+            de = device.DeviceEvents(*block_descriptions(self.blocks[start:end]), device=self.dev)
+            de.build()
+            de.check()   # once: every block fits (synchronises)
+            self._dev_events[key] = de
+        return de
+
+    def _motion_modes(self):
+        if self._modes is None:
+            modes = Motion.motion_modes(self.pod_trajectory)
+            if modes.shape[2] == 0:  # translation only: one zero mode
+                modes = np.zeros((self.n_nodes, 3, 1), dtype=np.float32)
+            if modes.shape[0] != self.n_nodes:
+                raise ValueError(f"motion modes have {modes.shape[0]} nodes, expected {self.n_nodes}")
+            self._modes = device.modes_soa(torch.from_numpy(np.ascontiguousarray(modes)).to(self.dev))
+        return self._modes
+
+    def _block_coefs(self, i: int) -> np.ndarray:
+        """Motion coefficients at the steps of block i, at its time shift."""
+        blk = self.blocks[i]
+        self.pod_trajectory.update_timeshift(blk.t0)
+        coef = Motion.motion_coefficients(self.pod_trajectory, step_times(blk.dt))
+        if Motion.n_modes(self.pod_trajectory) == 0:
+            coef = np.concatenate([np.zeros((coef.shape[0], 1)), coef], axis=1)
+        return coef
+
+    def _chain_coefs(self, start: int, end: int, min_rows: int = 0) -> torch.Tensor:
+        """One coefficient row per step of the blocks' concatenated tables:
+        block b's steps take rows off_b .. off_b + n_b - 2; the row of the
+        step that starts block b > 0 is not read (zeros); at least
+        ``min_rows`` rows (zero padded)."""
+        key = (start, end, min_rows)
+        c = self._coefs.get(key)
+        if c is None:
+            lens = [self.blocks[i].n_events for i in range(start, end)]
+            rows = []
+            for k, i in enumerate(range(start, end)):
+                if k:
+                    rows.append(None)
+                rows.append(self._block_coefs(i))
+            width = next(r.shape[1] for r in rows if r is not None)
+            table = np.zeros((max(sum(lens) - 1, min_rows, 1), width), dtype=np.float64)
+            r0 = 0
+            for r in rows:
+                if r is None:
+                    r0 += 1
+                    continue
+                table[r0:r0 + r.shape[0]] = r
+                r0 += r.shape[0]
+            c = torch.from_numpy(table).to(self.dev)
+            self._coefs[key] = c
+        return c
+
+    def solve_device(self, start: int = 0, end: int | None = None, stream=None, chain: bool = True,
+                     events: str = "host"):
         """Run blocks [start, end); returns device (Mxy (N, n_store) complex64,
         Mz (N, n_store) float32) of the storing blocks.
 
-        Static nodes run the whole range as one chained launch (the state
-        stays in registers across blocks; equal, bit for bit, to one
-        integrator launch and one store/spoil launch per block, which moving
-        nodes and ``chain=False`` take)."""
+        The range runs as one chained launch (the state stays in registers
+        across blocks; equal, bit for bit, to one integrator launch and one
+        store/spoil launch per block, which ``chain=False`` takes).
+        ``events="device"`` builds the event tables on the device from the
+        blocks' specifications first."""
         nb = len(self.blocks)
         if start < 0:
             start += nb
@@ -95,13 +161,37 @@ class BlochSolver:
         # column-major (N, n_store) stores, as (n_store, N) row-major buffers
         mxy_store = torch.zeros((max(n_store, 1), N), dtype=torch.complex64, device=self.dev)
         mz_store = torch.zeros((max(n_store, 1), N), dtype=torch.float32, device=self.dev)
-        self._run(start, end, mxy_store, mz_store, stream=stream, chain=chain)
+        self._run(start, end, mxy_store, mz_store, stream=stream, chain=chain, events=events)
         return mxy_store[:n_store].T, mz_store[:n_store].T
 
-    def _run(self, start: int, end: int, mxy_store, mz_store, stream=None, chain: bool = True):
+    def _run(self, start: int, end: int, mxy_store, mz_store, stream=None, chain: bool = True,
+             events: str = "host"):
         if end <= start:
             return
-        if chain and self.pod_trajectory is None:
+        if events not in ("host", "device"):
+            raise ValueError("events must be 'host' or 'device'")
+        if events == "device" and not chain:
+            raise ValueError("device-built event tables run as a chain")
+        moving = self.pod_trajectory is not None
+        n_store = sum(1 for i in range(start, end) if self.blocks[i].store)
+        if chain and (moving or events == "device"):
+            with torch.cuda.device(self.dev):
+                if events == "device":
+                    de = self._device_events(start, end)
+                    motion = (self._motion_modes(), self._chain_coefs(start, end, de.capacity - 1)) if moving \
+                        else None
+                    de.build(stream=stream)
+                    tabs = (de.rf, de.G, de.dt, de.marks, de.n_events, de.capacity)
+                else:
+                    motion = (self._motion_modes(), self._chain_coefs(start, end)) if moving else None
+                    ch = self._chain(start, end)
+                    ev = ch.events
+                    tabs = (ev.rf, ev.G, ev.dt, ch.marks, ch.n_events, ev.n_time)
+                device.bloch_chain_marked(self.nodes, *tabs, self.M0, self.gamma, self.mxy, self.mz,
+                                          mxy_store, mz_store, n_store, precision=self.precision, motion=motion,
+                                          stream=stream)
+            return
+        if chain:
             with torch.cuda.device(self.dev):
                 device.bloch_chain(self.nodes, self._chain(start, end), self.M0, self.gamma, self.mxy, self.mz,
                                    mxy_store, mz_store, precision=self.precision, stream=stream)
@@ -115,8 +205,8 @@ class BlochSolver:
                 ev = self._event_table(i)
                 motion = None
                 if self.pod_trajectory is not None:
-                    self.pod_trajectory.update_timeshift(blk.t0)
-                    motion = bloch_motion_tables(self.pod_trajectory, N, step_times(blk.dt), self.dev)
+                    coef = torch.from_numpy(np.ascontiguousarray(self._block_coefs(i))).to(self.dev)
+                    motion = (self._motion_modes(), coef)
                 device.bloch(self.nodes, ev, self.M0, self.gamma, self.mxy, self.mz,
                              precision=self.precision, motion=motion, history=False,
                              mxy_out=self.mxy, mz_out=self.mz, stream=stream)
@@ -131,34 +221,33 @@ class BlochSolver:
         mxy, mz = self.solve_device(start, end)
         return mxy.cpu().numpy(), mz.cpu().numpy()
 
-    def capture(self, stream=None) -> "BlochSolveGraph":
-        """Record the whole block chain as one HIP graph (static nodes).
+    def capture(self, stream=None, events: str = "host") -> "BlochSolveGraph":
+        """Record the whole block chain as one HIP graph.
 
         Each replay integrates every block from the state then held in
         ``self.mxy`` / ``self.mz`` and leaves the carried state there, as
-        ``solve_device()`` does, with one graph launch.
+        ``solve_device()`` does, with one graph launch; with
+        ``events="device"`` the replay also rebuilds the event tables from
+        the device block descriptions.
         """
-        return BlochSolveGraph(self, stream)
+        return BlochSolveGraph(self, stream, events=events)
 
 
 class BlochSolveGraph:
     """A ``BlochSolver``'s block chain captured once and replayed.
 
-    Event tables are uploaded and the kernels loaded (one eager run, after
-    which the carried state is restored) before the capture. The store
-    buffers belong to the graph; every replay rewrites all stored columns.
-    Moving nodes are not covered: their per-block tables come from the host.
+    Event tables (or, with ``events="device"``, their descriptions), motion
+    modes and coefficients are uploaded and the kernels loaded (one eager
+    run, after which the carried state is restored) before the capture.  The
+    store buffers belong to the graph; every replay rewrites all stored
+    columns.
     """
 
-    def __init__(self, solver: BlochSolver, stream=None):
-        if solver.pod_trajectory is not None:
-            raise ValueError("graph capture covers static nodes; moving nodes build per-block motion tables "
-                             "on the host")
+    def __init__(self, solver: BlochSolver, stream=None, events: str = "host"):
         self.solver = solver
+        self.events = events
         N = solver.n_nodes
-        nb = len(solver.blocks)
         self.n_store = sum(1 for b in solver.blocks if b.store)
-        solver._chain(0, nb)   # uploaded before the capture
         with torch.cuda.device(solver.dev):
             self.stream = stream if stream is not None else torch.cuda.Stream(device=solver.dev)
             self.mxy_store = torch.zeros((max(self.n_store, 1), N), dtype=torch.complex64, device=solver.dev)
@@ -176,7 +265,8 @@ class BlochSolveGraph:
 
     def _body(self):
         s = self.solver
-        s._run(0, len(s.blocks), self.mxy_store, self.mz_store)
+        s._run(0, len(s.blocks), self.mxy_store, self.mz_store, stream=torch.cuda.current_stream(s.dev),
+               events=self.events)
 
     def replay(self):
         """Replay on the current stream's device; returns device (Mxy (N, n_store)

@@ -22,6 +22,7 @@ FM_OK, FM_ERR_ARG, FM_ERR_HIP, FM_ERR_UNSUPPORTED, FM_ERR_WORKSPACE = range(5)
 PREC_F64, PREC_MIXED, PREC_F32 = 0, 1, 2
 DTYPE_F32, DTYPE_F64 = 0, 1
 METHOD_AUTO, METHOD_DIRECT, METHOD_CARTESIAN = 0, 1, 2
+EVENTS_MAX_CANDIDATES, EVENTS_MAX_PARTS = 8192, 16
 
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -53,6 +54,10 @@ _SIGNATURES = {
     "fm_bloch_store_spoil": [_i64, _p, _p, _p, _p, _i64, _int, _p],
     "fm_bloch_chain": [_int, _i64, _p, _dbl, _dbl, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _p, _p,
                        _p],
+    "fm_bloch_chain_marked": [_int, _i64, _p, _dbl, _dbl, _p, _i64, _p, _p, _p, _p, _p, _int, _p, _p, _p, _p, _p,
+                              _i64, _p, _p, _p],
+    "fm_events_bound": [_i64, _p, _p, _p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "fm_events_build": [_i64, _p, _p, _p, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
     "fm_mass_p1_color": [_i64, _i64, _p, _p, _p, _int, ctypes.POINTER(_int)],
     "fm_mass_p1_lumped": [_i64, _i64, _p, _int, _p, _p, _p, _int, _p, _p],
     "fm_mass_p1_incidence": [_i64, _i64, _p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)],

@@ -117,7 +117,7 @@ __device__ inline float one_minus_exp<float>(float x, float) { return (float)(-e
 template <typename T>
 __device__ inline T relax_mz(T mz, T e1, T om1, T M0);
 template <>
-__device__ inline double relax_mz<double>(double mz, double e1, double om1, double M0) { return mz * e1 + om1 * M0; }
+__device__ inline double relax_mz<double>(double mz, double e1, double om1, double M0) { return fma(mz, e1, om1 * M0); }
 template <>
 __device__ inline float relax_mz<float>(float mz, float, float om1, float M0) { return mz - om1 * (mz - M0); }
 
@@ -132,7 +132,7 @@ __device__ inline T mz_coef(T a2, T b2);
 template <>
 __device__ inline double mz_coef<double>(double a2, double b2) { return a2 - b2; }
 template <>
-__device__ inline float mz_coef<float>(float, float b2) { return 1.0f - 2.0f * b2; }
+__device__ inline float mz_coef<float>(float, float b2) { return fmaf(-2.0f, b2, 1.0f); }
 
 template <typename T>
 __device__ inline T dev_sqrt(T x);
@@ -188,21 +188,35 @@ struct Cplx2<float> {
     using type = float2;
 };
 
-template <typename T, bool MOTION, bool HISTORY, bool CHAIN = false>
+// Chained blocks: kChainHost takes the block boundaries from a ChainDesc
+// passed by value, kChainMarks from per-event marks in device memory
+// (marks[e] = 1 + code of the block ending before event e, 0 inside a
+// block; marks[n_time] = 1 + code of the last block) with the event count
+// read from device memory as well.
+constexpr int kChainNone = 0, kChainHost = 1, kChainMarks = 2;
+
+template <typename T, bool MOTION, bool HISTORY, int CHAIN = kChainNone>
 __global__ __launch_bounds__(kBlochThreads) void k_bloch(
     const double *__restrict__ packed, int64_t N, int64_t n_time, double M0d, double gammad,
     const double *__restrict__ rf, const double *__restrict__ G, const double *__restrict__ dt,
     const double *mxy_in, const double *mz_in,  // may alias mxy_fin / mz_fin
     const float *__restrict__ modes, int R, const double *__restrict__ mcoef,
     typename Cplx2<T>::type *__restrict__ mxy_out, T *__restrict__ mz_out,
-    double2 *mxy_fin, double *mz_fin, const std::conditional_t<CHAIN, ChainDesc, NoChain> chain = {},
-    float2 *__restrict__ store_xy = nullptr, float *__restrict__ store_z = nullptr)
+    double2 *mxy_fin, double *mz_fin, const std::conditional_t<CHAIN == kChainHost, ChainDesc, NoChain> chain = {},
+    float2 *__restrict__ store_xy = nullptr, float *__restrict__ store_z = nullptr,
+    const int32_t *__restrict__ marks = nullptr, const int64_t *__restrict__ n_events = nullptr,
+    int64_t n_store = 0)
 {
     using C2 = typename Cplx2<T>::type;
-    static_assert(!CHAIN || (!MOTION && !HISTORY), "chained blocks: static nodes, final state");
+    static_assert(CHAIN == kChainNone || !HISTORY, "chained blocks keep the final state only");
     __shared__ Event<T> sev[kBlochChunk];
     __shared__ T scoef[MOTION ? kBlochChunk * (kBlochMaxModes + 3) : 1];
-    __shared__ int32_t smark[CHAIN ? kBlochChunk : 1];  // 0: a step; else 1 + block-end code
+    __shared__ int32_t smark[CHAIN != kChainNone ? kBlochChunk : 1];  // 0: a step; else 1 + block-end code
+    if constexpr (CHAIN == kChainMarks) {
+        // the event count is produced on the device; n_time is the capacity
+        const int64_t ne = *n_events;
+        n_time = ne < n_time ? ne : n_time;
+    }
 
     const int tid = threadIdx.x;
     const int64_t p = (int64_t)blockIdx.x * kBlochThreads + tid;
@@ -263,7 +277,10 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                 ev.rf_re = ev.rf_im = ev.rf2 = ev.kappa = ev.gx = ev.gy = ev.gz = ev.dt = 0;
             }
             sev[tid] = ev;
-            if constexpr (CHAIN) {
+            if constexpr (CHAIN == kChainMarks) {
+                smark[tid] = i < n_steps ? marks[i + 1] : 0;
+            }
+            if constexpr (CHAIN == kChainHost) {
                 // event i + 1 starts block b > 0: it ends block b - 1
                 int mk = 0;
                 if (i < n_steps) {
@@ -289,11 +306,11 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
         const int cnt = (n_steps - base < kBlochChunk) ? (int)(n_steps - base) : kBlochChunk;
         if (active) {
             for (int e = 0; e < cnt; ++e) {
-                if constexpr (CHAIN) {
+                if constexpr (CHAIN != kChainNone) {
                     const int mk = smark[e];
                     if (mk != 0) {  // the same for every lane: a block ends here
                         const int col = ((mk - 1) >> 1) - 1;
-                        if (col >= 0) {
+                        if (col >= 0 && (CHAIN == kChainHost || col < n_store)) {
                             store_xy[p + col * N] = make_float2((float)mxr, (float)mxi);
                             store_z[p + col * N] = (float)mz;
                         }
@@ -319,21 +336,21 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
 #pragma unroll
                     for (int r = 0; r < kBlochMaxModes; ++r) {
                         if (r < R) {
-                            dx += (T)md[r] * cf[r];
-                            dy += (T)md[kBlochMaxModes + r] * cf[r];
-                            dz += (T)md[2 * kBlochMaxModes + r] * cf[r];
+                            dx = fma((T)md[r], cf[r], dx);
+                            dy = fma((T)md[kBlochMaxModes + r], cf[r], dy);
+                            dz = fma((T)md[2 * kBlochMaxModes + r], cf[r], dz);
                         }
                     }
                     px += dx;
                     py += dy;
                     pz += dz;
                 }
-                const T Bz = px * ev.gx + py * ev.gy + pz * ev.gz + dB;
+                const T Bz = fma(pz, ev.gz, fma(py, ev.gy, px * ev.gx)) + dB;
                 if (uniform_zero(ev.rf2)) {
                     // free precession: Mxy exp(-i gamma dt Bz); kappa = -gamma dt / 2
                     T s, c;
                     dev_sincos<T>((ev.kappa + ev.kappa) * Bz, &s, &c);
-                    const T nmr = c * mxr - s * mxi, nmi = c * mxi + s * mxr;
+                    const T nmr = fma(c, mxr, -(s * mxi)), nmi = fma(c, mxi, s * mxr);
                     mxr = nmr * e2;
                     mxi = nmi * e2;
                     mz = relax_mz<T>(mz, e1, om1, M0);
@@ -347,7 +364,7 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                     }
                     continue;
                 }
-                T Bn = dev_sqrt<T>(Bz * Bz + ev.rf2);
+                T Bn = dev_sqrt<T>(fma(Bz, Bz, ev.rf2));
                 if (Bn < (T)1e-12) Bn = (T)1e-12;
                 const T inv = dev_rcp<T>(Bn);
                 const T nz = Bz * inv;
@@ -356,16 +373,19 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
                 // alpha = c - i nz s ; beta = -i (rf/|B|) s
                 const T ar = c, ai = -nz * s;
                 const T br = ev.rf_im * inv * s, bi = -ev.rf_re * inv * s;
-                const T a2 = ar * ar + ai * ai, b2 = br * br + bi * bi;
+                // every sum of two products is written as one fma and one
+                // product, so that each instantiation of this kernel rounds
+                // alike (a free contraction may fuse either product)
+                const T a2 = fma(ar, ar, ai * ai), b2 = fma(br, br, bi * bi);
                 const T car = ar, cai = -ai;
-                const T cab_r = car * br - cai * bi, cab_i = car * bi + cai * br;
-                const T ca2_r = car * car - cai * cai, ca2_i = (T)2 * car * cai;
-                const T bb_r = br * br - bi * bi, bb_i = (T)2 * br * bi;
+                const T cab_r = fma(car, br, -(cai * bi)), cab_i = fma(car, bi, cai * br);
+                const T ca2_r = fma(car, car, -(cai * cai)), ca2_i = ((T)2 * car) * cai;
+                const T bb_r = fma(br, br, -(bi * bi)), bb_i = ((T)2 * br) * bi;
                 const T cmr = mxr, cmi = -mxi;
-                const T nmr = (T)2 * cab_r * mz + (ca2_r * mxr - ca2_i * mxi) - (bb_r * cmr - bb_i * cmi);
-                const T nmi = (T)2 * cab_i * mz + (ca2_r * mxi + ca2_i * mxr) - (bb_r * cmi + bb_i * cmr);
-                const T ab_r = ar * br - ai * bi, ab_i = ar * bi + ai * br;
-                const T nmz = mz_coef<T>(a2, b2) * mz - (T)2 * (ab_r * cmr - ab_i * cmi);
+                const T nmr = fma((T)2 * cab_r, mz, fma(ca2_r, mxr, -(ca2_i * mxi)) - fma(bb_r, cmr, -(bb_i * cmi)));
+                const T nmi = fma((T)2 * cab_i, mz, fma(ca2_r, mxi, ca2_i * mxr) - fma(bb_r, cmi, bb_i * cmr));
+                const T ab_r = fma(ar, br, -(ai * bi)), ab_i = fma(ar, bi, ai * br);
+                const T nmz = fma(mz_coef<T>(a2, b2), mz, -((T)2 * fma(ab_r, cmr, -(ab_i * cmi))));
                 mxr = nmr * e2;
                 mxi = nmi * e2;
                 mz = relax_mz<T>(nmz, e1, om1, M0);
@@ -380,11 +400,13 @@ __global__ __launch_bounds__(kBlochThreads) void k_bloch(
             }
         }
     }
-    if constexpr (CHAIN) {
+    if constexpr (CHAIN != kChainNone) {
         if (active) {  // the last block's end
-            const int code = chain.code[chain.n - 1];
+            int code;
+            if constexpr (CHAIN == kChainHost) code = chain.code[chain.n - 1];
+            else code = marks[n_time] - 1;
             const int col = (code >> 1) - 1;
-            if (col >= 0) {
+            if (col >= 0 && (CHAIN == kChainHost || col < n_store)) {
                 store_xy[p + col * N] = make_float2((float)mxr, (float)mxi);
                 store_z[p + col * N] = (float)mz;
             }
@@ -461,9 +483,28 @@ static void launch_bloch_chain(int64_t N, int64_t n_time, const double *packed, 
                                float2 *store_xy, float *store_z, hipStream_t st)
 {
     const unsigned blocks = (unsigned)cdiv(N, kBlochThreads);
-    k_bloch<T, false, false, true><<<blocks, kBlochThreads, 0, st>>>(
+    k_bloch<T, false, false, kChainHost><<<blocks, kBlochThreads, 0, st>>>(
         packed, N, n_time, M0, gamma, rf, G, dt, mxy_in, mz_in, nullptr, 0, nullptr, nullptr, nullptr,
         reinterpret_cast<double2 *>(mxy_out), mz_out, cd, store_xy, store_z);
+}
+
+template <typename T>
+static void launch_bloch_marked(bool motion, int64_t N, int64_t capacity, const int64_t *n_events,
+                                const double *packed, double M0, double gamma, const double *rf, const double *G,
+                                const double *dt, const int32_t *marks, const float *modes, int R,
+                                const double *mcoef, const double *mxy_in, const double *mz_in, double *mxy_out,
+                                double *mz_out, float2 *store_xy, float *store_z, int64_t n_store, hipStream_t st)
+{
+    const unsigned blocks = (unsigned)cdiv(N, kBlochThreads);
+    double2 *mf = reinterpret_cast<double2 *>(mxy_out);
+    if (motion)
+        k_bloch<T, true, false, kChainMarks><<<blocks, kBlochThreads, 0, st>>>(
+            packed, N, capacity, M0, gamma, rf, G, dt, mxy_in, mz_in, modes, R, mcoef, nullptr, nullptr, mf, mz_out,
+            NoChain{}, store_xy, store_z, marks, n_events, n_store);
+    else
+        k_bloch<T, false, false, kChainMarks><<<blocks, kBlochThreads, 0, st>>>(
+            packed, N, capacity, M0, gamma, rf, G, dt, mxy_in, mz_in, nullptr, 0, nullptr, nullptr, nullptr, mf,
+            mz_out, NoChain{}, store_xy, store_z, marks, n_events, n_store);
 }
 
 }  // namespace fm
@@ -589,5 +630,35 @@ extern "C" int fm_bloch_chain(int precision, int64_t n_pos, const double *packed
         FM_LAUNCH_CHECK("k_bloch (chained blocks)");
         ev0 += o;
     }
+    return FM_OK;
+}
+
+extern "C" int fm_bloch_chain_marked(int precision, int64_t n_pos, const double *packed, double M0, double gamma,
+                                     const int64_t *n_events, int64_t capacity, const double *rf, const double *G,
+                                     const double *dt, const int32_t *marks, const float *modes, int n_modes,
+                                     const double *motion_coef, const double *mxy_in, const double *mz_in,
+                                     void *mxy_store, void *mz_store, int64_t n_store, double *mxy_out,
+                                     double *mz_out, void *stream)
+{
+    FM_CHECK_ARG(precision == FM_PREC_F64 || precision == FM_PREC_F32,
+                 "Bloch precision must be FM_PREC_F64 or FM_PREC_F32 (got %d)", precision);
+    FM_CHECK_ARG(n_pos >= 0 && capacity >= 1 && n_store >= 0, "negative size or capacity < 1");
+    FM_CHECK_ARG(n_modes >= 0 && n_modes <= kBlochMaxModes, "n_modes must be in [0, %d]", kBlochMaxModes);
+    if (n_pos == 0) return FM_OK;
+    FM_CHECK_ARG(packed && n_events && rf && G && dt && marks, "NULL array");
+    FM_CHECK_ARG(mxy_in && mz_in && mxy_out && mz_out, "NULL state");
+    FM_CHECK_ARG(n_store == 0 || (mxy_store && mz_store), "NULL store");
+    const bool motion = modes != nullptr && n_modes > 0;
+    FM_CHECK_ARG(!motion || motion_coef != nullptr, "motion_coef is NULL");
+    hipStream_t st = as_stream(stream);
+    float2 *sx = reinterpret_cast<float2 *>(mxy_store);
+    float *sz = reinterpret_cast<float *>(mz_store);
+    if (precision == FM_PREC_F64)
+        launch_bloch_marked<double>(motion, n_pos, capacity, n_events, packed, M0, gamma, rf, G, dt, marks, modes,
+                                    n_modes, motion_coef, mxy_in, mz_in, mxy_out, mz_out, sx, sz, n_store, st);
+    else
+        launch_bloch_marked<float>(motion, n_pos, capacity, n_events, packed, M0, gamma, rf, G, dt, marks, modes,
+                                   n_modes, motion_coef, mxy_in, mz_in, mxy_out, mz_out, sx, sz, n_store, st);
+    FM_LAUNCH_CHECK("k_bloch (marked chain)");
     return FM_OK;
 }

@@ -314,10 +314,131 @@ def bloch(nodes: BlochNodes, events: EventTable, M0: float, gamma: float, mxy0: 
     return mxy_out, mz_out
 
 
+# C layouts of fm_block_desc, fm_hard_pulse, fm_trapezoid
+BLOCK_DESC = np.dtype([("t0", "f8"), ("dur", "f8"), ("dt", "f8"), ("dt_rf", "f8"), ("dt_gr", "f8"),
+                       ("pulse0", "i4"), ("n_pulses", "i4"), ("trap0", "i4"), ("n_traps", "i4"),
+                       ("spoil", "i4"), ("store_col", "i4")], align=True)
+HARD_PULSE = np.dtype([("start", "f8"), ("dur", "f8"), ("amp_re", "f8"), ("amp_im", "f8")], align=True)
+TRAPEZOID = np.dtype([("start", "f8"), ("ramp", "f8"), ("flat", "f8"), ("amp", "f8"), ("axis", "i4"),
+                      ("reserved", "i4")], align=True)
+
+
+def marks_from_blocks(block_len, codes) -> np.ndarray:
+    """Per-event block-boundary marks of concatenated tables: 1 + code of the
+    block ending before event e when e starts block b > 0, 0 elsewhere, and
+    1 + code of the last block after the last event (code = spoil |
+    (store_col + 1) << 1)."""
+    block_len = np.asarray(block_len, dtype=np.int64)
+    codes = np.asarray(codes, dtype=np.int32)
+    off = np.concatenate([[0], np.cumsum(block_len)])
+    marks = np.zeros(int(off[-1]) + 1, dtype=np.int32)
+    marks[off[1:-1]] = 1 + codes[:-1]
+    marks[off[-1]] = 1 + codes[-1]
+    return marks
+
+
+class DeviceEvents:
+    """A block chain's event tables built on the device (``fm_events_build``)
+    from block descriptions: ``desc`` (BLOCK_DESC records), ``pulses``
+    (HARD_PULSE), ``traps`` (TRAPEZOID).  Holds the device descriptions and
+    output tables; ``build()`` (re)writes the tables on a stream with no host
+    round trip, so it can be recorded into a HIP graph, and reads nothing
+    back: ``n_events`` stays on the device."""
+
+    def __init__(self, desc, pulses, traps, device=None):
+        device = torch.device(device) if device is not None else default_device()
+        desc = np.ascontiguousarray(desc, dtype=BLOCK_DESC)
+        pulses = np.ascontiguousarray(pulses, dtype=HARD_PULSE)
+        traps = np.ascontiguousarray(traps, dtype=TRAPEZOID)
+        self.n_blocks = int(desc.size)
+        if self.n_blocks == 0:
+            raise ValueError("no block")
+        mx, tot = ctypes.c_int64(), ctypes.c_int64()
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a.size else None  # noqa: E731
+        _lib.call("fm_events_bound", self.n_blocks, vp(desc), vp(pulses), vp(traps), ctypes.byref(mx),
+                  ctypes.byref(tot))
+        self.max_candidates, self.capacity = int(mx.value), max(int(tot.value), 1)
+        if self.max_candidates > _lib.EVENTS_MAX_CANDIDATES:
+            raise ValueError(f"a block has {self.max_candidates} grid candidates; the builder sorts at most "
+                             f"{_lib.EVENTS_MAX_CANDIDATES} per block")
+        self.n_store = int(desc["store_col"].max()) + 1 if self.n_blocks else 0
+        u8 = lambda a: torch.from_numpy(a.view(np.uint8).copy()).to(device)  # noqa: E731
+        self.desc, self.pulses, self.traps = u8(desc), u8(pulses), u8(traps)
+        C = self.capacity
+        self.offsets = torch.zeros(self.n_blocks + 1, dtype=torch.int64, device=device)
+        self.times = torch.zeros(C, dtype=torch.float64, device=device)
+        self.rf = torch.zeros(C, dtype=torch.complex128, device=device)
+        self.G = torch.zeros((C, 3), dtype=torch.float64, device=device)
+        self.dt = torch.zeros(C, dtype=torch.float64, device=device)
+        self.marks = torch.zeros(C + 1, dtype=torch.int32, device=device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+        self.device = device
+
+    @property
+    def n_events(self) -> torch.Tensor:
+        """Total event count (device, (1,) int64 view of offsets)."""
+        return self.offsets[self.n_blocks:]
+
+    def build(self, stream=None) -> "DeviceEvents":
+        with torch.cuda.device(self.device):
+            _lib.call("fm_events_build", self.n_blocks, ptr(self.desc), ptr(self.pulses) if self.pulses.numel() else None,
+                      ptr(self.traps) if self.traps.numel() else None, self.max_candidates, self.capacity,
+                      ptr(self.offsets), ptr(self.times), ptr(torch.view_as_real(self.rf)), ptr(self.G), ptr(self.dt),
+                      ptr(self.marks), ptr(self.status), stream_handle(stream))
+        return self
+
+    def check(self) -> None:
+        """Read the builder's status (synchronises): raise when a block did not
+        fit or the events exceeded the capacity."""
+        st = int(self.status.item())
+        if st:
+            raise RuntimeError(f"fm_events_build status {st}: " + ", ".join(
+                m for b, m in ((1, "a block exceeded max_candidates"), (2, "events exceed the capacity")) if st & b))
+
+    def tables(self):
+        """Host copies (times, rf, G, dt, offsets, marks) of the built tables
+        (synchronises)."""
+        n = int(self.offsets[-1].item())
+        return (self.times[:n].cpu().numpy(), self.rf[:n].cpu().numpy(), self.G[:n].cpu().numpy(),
+                self.dt[:n].cpu().numpy(), self.offsets.cpu().numpy(), self.marks[:n + 1].cpu().numpy())
+
+
+def bloch_chain_marked(nodes: BlochNodes, rf, G, dt, marks, n_events, capacity: int, M0: float, gamma: float,
+                       mxy, mz, mxy_store=None, mz_store=None, n_store: int = 0, precision="f64", motion=None,
+                       stream=None):
+    """A block chain over concatenated tables with per-event marks and a
+    device event count (``fm_bloch_chain_marked``): static nodes, or moving
+    nodes with ``motion`` = (modes_soa, coef (capacity - 1, R+3) float64,
+    one row per step).  The state is updated in place."""
+    prec = _prec(precision)
+    N = nodes.n_pos
+    if mxy.dtype != torch.complex128 or mz.dtype != torch.float64 or mxy.numel() != N or mz.numel() != N:
+        raise ValueError("chain state must be complex128 / float64 of n_pos entries")
+    if n_store and (mxy_store is None or mz_store is None or mxy_store.dtype != torch.complex64
+                    or mz_store.dtype != torch.float32 or mxy_store.shape[0] < n_store
+                    or mxy_store.shape[1] != N or tuple(mz_store.shape) != tuple(mxy_store.shape)):
+        raise ValueError("stores must be (n_store, N) complex64 / float32")
+    if rf.numel() < capacity or G.shape[0] < capacity or dt.numel() < capacity or marks.numel() < capacity + 1:
+        raise ValueError("event tables shorter than the capacity")
+    modes = coef = None
+    R = 0
+    if motion is not None:
+        modes, coef = motion
+        R = int(modes.shape[0]) // 3
+        if coef.dtype != torch.float64 or coef.shape[1] != R + 3 or coef.shape[0] < capacity - 1:
+            raise ValueError("chain motion coefficients must be float64 of shape (>= capacity - 1, R+3)")
+    with torch.cuda.device(nodes.packed.device):
+        _lib.call("fm_bloch_chain_marked", prec, N, ptr(nodes.packed), float(M0), float(gamma), ptr(n_events),
+                  int(capacity), ptr(torch.view_as_real(rf)), ptr(G), ptr(dt), ptr(marks), ptr(modes), R, ptr(coef),
+                  ptr(torch.view_as_real(mxy)), ptr(mz), ptr(mxy_store), ptr(mz_store), int(n_store),
+                  ptr(torch.view_as_real(mxy)), ptr(mz), stream_handle(stream))
+
+
 class BlockChain:
     """Consecutive blocks for ``bloch_chain``: their event tables concatenated
     on the device, and per block (host) its event count, store column (-1:
-    none) and spoil flag."""
+    none) and spoil flag; also the per-event marks and event count
+    (device) that ``bloch_chain_marked`` reads."""
 
     def __init__(self, events: list, store_col, spoil, device=None):
         device = torch.device(device) if device is not None else default_device()
@@ -332,6 +453,10 @@ class BlockChain:
         if self.store_col.size != self.n_blocks or self.spoil.size != self.n_blocks:
             raise ValueError("one store column and one spoil flag per block")
         self.n_store = int(self.store_col.max()) + 1 if self.n_blocks else 0
+        if self.n_blocks:
+            codes = (self.spoil != 0).astype(np.int32) | ((self.store_col + 1) << 1)
+            self.marks = torch.from_numpy(marks_from_blocks(self.block_len, codes)).to(device)
+            self.n_events = torch.tensor([int(self.block_len.sum())], dtype=torch.int64, device=device)
 
 
 def bloch_chain(nodes: BlochNodes, chain: BlockChain, M0: float, gamma: float, mxy, mz, mxy_store, mz_store,

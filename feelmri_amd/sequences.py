@@ -109,8 +109,41 @@ def make_block(t0: float, dur: float, pulses=(), gradients=(), dt_rf: float = 0.
     G = np.zeros((times.size, 3), dtype=np.float64)
     for g in gradients:
         G[:, g.axis] += g(times)
+    spec = dict(t0=float(t0), dur=float(dur), dt=float(dt), dt_rf=float(dt_rf), dt_gr=float(dt_gr),
+                pulses=list(pulses), gradients=list(gradients))
     return Block(times=times, rf=rf, G=G, dt=dts, empty=empty, store=store, t0=ext0,
-                 info=dict(extent=(ext0, ext1)))
+                 info=dict(extent=(ext0, ext1), spec=spec))
+
+
+def block_descriptions(blocks):
+    """The blocks' specifications (``make_block``'s arguments, kept in
+    ``Block.info["spec"]``) as the records of the device event builder:
+    (desc, pulses, trapezoids) numpy structured arrays in the C layouts of
+    fm_block_desc, fm_hard_pulse and fm_trapezoid.  Store columns are
+    numbered over the storing blocks in order; a non-empty block spoils."""
+    from .device import BLOCK_DESC, HARD_PULSE, TRAPEZOID
+
+    desc = np.zeros(len(blocks), dtype=BLOCK_DESC)
+    pl, tl = [], []
+    col = 0
+    for b, blk in enumerate(blocks):
+        spec = blk.info.get("spec")
+        if spec is None:
+            raise ValueError(f"block {b} has no specification (build blocks with make_block)")
+        d = desc[b]
+        d["t0"], d["dur"], d["dt"], d["dt_rf"], d["dt_gr"] = (spec["t0"], spec["dur"], spec["dt"], spec["dt_rf"],
+                                                             spec["dt_gr"])
+        d["pulse0"], d["n_pulses"] = len(pl), len(spec["pulses"])
+        d["trap0"], d["n_traps"] = len(tl), len(spec["gradients"])
+        d["spoil"] = 0 if blk.empty else 1
+        d["store_col"] = col if blk.store else -1
+        col += int(blk.store)
+        for p in spec["pulses"]:
+            amp = complex(p.b1 * np.exp(1j * p.phase))   # the value HardPulse.__call__ forms
+            pl.append((p.start, p.dur, amp.real, amp.imag))
+        for g in spec["gradients"]:
+            tl.append((g.start, g.ramp, g.flat, g.amp, g.axis, 0))
+    return desc, np.array(pl, dtype=HARD_PULSE), np.array(tl, dtype=TRAPEZOID)
 
 
 def delay_block(t0: float, dur: float, dt: float = 1.0) -> Block:

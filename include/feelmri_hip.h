@@ -218,6 +218,83 @@ int fm_bloch_chain(int precision, int64_t n_pos, const double *packed, double M0
                    int64_t n_store, double *mxy_out, double *mz_out, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Sequence event tables built on the device -- replaces
+ * SequenceBlock._discretization and __call__ (python/feelmri/Bloch.py:105-129,
+ * :44-49) with dt = diff(times, prepend=0) (Bloch.py:340), for hard pulses
+ * and trapezoidal gradients; with fm_bloch_chain_marked a whole block chain
+ * (BlochSolver.solve, Bloch.py:298-380) runs from device descriptions with
+ * nothing read back to the host.
+ * ---------------------------------------------------------------------- */
+/* One block: the union of a dt grid over its extent, each pulse's start,
+ * end and dt_rf grid, each trapezoid's 4 knots and dt_gr grid, sorted and
+ * de-duplicated (grids as numpy's arange: start, start + step, then
+ * start + i * ((start + step) - start); dt_rf, dt_gr <= 0: no grid).  The
+ * extent runs from min(t0, pulse starts, trapezoid starts) to
+ * max(t0 + dur, pulse ends, trapezoid ends).  RF at time t: the sum, in
+ * order, of amp over the pulses with start < t <= start + dur; gradient on
+ * an axis: the sum, in order, of numpy's interp of each trapezoid of that
+ * axis over its knots (zero outside).  dt must be > 0. */
+typedef struct fm_block_desc {
+    double t0, dur, dt, dt_rf, dt_gr;   /* ms */
+    int32_t pulse0, n_pulses;           /* pulses [pulse0, pulse0 + n_pulses) */
+    int32_t trap0, n_traps;             /* trapezoids [trap0, trap0 + n_traps) */
+    int32_t spoil;                      /* != 0: Mxy zeroed after the block (a non-empty block) */
+    int32_t store_col;                  /* store column, -1: none */
+} fm_block_desc;
+
+typedef struct fm_hard_pulse {
+    double start, dur;                  /* ms: constant B1 over (start, start + dur] */
+    double amp_re, amp_im;              /* mT */
+} fm_hard_pulse;
+
+typedef struct fm_trapezoid {
+    double start, ramp, flat, amp;      /* ms, mT/m: knots start, +ramp, +flat, +ramp */
+    int32_t axis, reserved;             /* 0, 1, 2 */
+} fm_trapezoid;
+
+#define FM_EVENTS_MAX_CANDIDATES 8192   /* grid candidates per block */
+#define FM_EVENTS_MAX_PARTS 16          /* pulses and trapezoids per block, each */
+
+/* Host: per-block candidate counts (grid points before de-duplication):
+ * max_candidates receives the largest (the builder's per-block bound) and
+ * total_bound their sum (a bound on the event count).  Same arithmetic as
+ * the device builder.  Host arrays. */
+int fm_events_bound(int64_t n_blocks, const fm_block_desc *blocks, const fm_hard_pulse *pulses,
+                    const fm_trapezoid *traps, int64_t *max_candidates, int64_t *total_bound);
+
+/* Device: the blocks' event tables, concatenated.  blocks, pulses, traps:
+ * device arrays; max_candidates: a bound from fm_events_bound (at most
+ * FM_EVENTS_MAX_CANDIDATES); capacity: entries of times, dt (float64), rf
+ * (2 x float64), G (3 x float64) and, plus one, of marks (int32).
+ * offsets (device, n_blocks + 1 int64) receives each block's first event and
+ * the total; marks[e] is 0, or 1 + code of the block that ends before event
+ * e when e starts block b > 0, and marks[total] = 1 + code of the last block
+ * (code = spoil | (store_col + 1) << 1).  status (device int32) receives 0,
+ * or 1 when a block has more candidates than max_candidates, or 2 when the
+ * events exceed capacity (the tables are then incomplete; nothing is
+ * written out of bounds).  Three launches, nothing allocated: the call can
+ * be recorded into a HIP graph. */
+int fm_events_build(int64_t n_blocks, const fm_block_desc *blocks, const fm_hard_pulse *pulses,
+                    const fm_trapezoid *traps, int64_t max_candidates, int64_t capacity, int64_t *offsets,
+                    double *times, double *rf, double *G, double *dt, int32_t *marks, int32_t *status,
+                    void *stream);
+
+/* A block chain over concatenated event tables whose block boundaries are
+ * given per event (marks, as fm_events_build writes them) and whose event
+ * count is read on the device (*n_events, at most capacity): static nodes,
+ * or moving nodes with modes / n_modes / motion_coef as fm_bloch_execute,
+ * motion_coef holding one row per step of the concatenated table (step i
+ * uses event i + 1; rows of the steps that start a block are not read).
+ * Same results, bit for bit, as fm_bloch_execute (history == 0) and
+ * fm_bloch_store_spoil per block.  Store columns at or beyond n_store are
+ * not written.  One launch, any number of blocks, nothing allocated. */
+int fm_bloch_chain_marked(int precision, int64_t n_pos, const double *packed, double M0, double gamma,
+                          const int64_t *n_events, int64_t capacity, const double *rf, const double *G,
+                          const double *dt, const int32_t *marks, const float *modes, int n_modes,
+                          const double *motion_coef, const double *mxy_in, const double *mz_in, void *mxy_store,
+                          void *mz_store, int64_t n_store, double *mxy_out, double *mz_out, void *stream);
+
+/* ------------------------------------------------------------------------
  * P1 mass -- replaces Assemble.MassAssemble (cpp/feelmri/Assemble.h:21-39;
  * Assemble.cpp:78-116) followed by lumping (python/feelmri/
  * Phantom.py:379-399).  P1 tetrahedra only.
