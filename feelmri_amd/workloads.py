@@ -178,7 +178,7 @@ def c3(scale: float = 1.0, n_phases: int = 20, n_modes: int = 10, kspace=(96, 96
     wl = FlowWorkload("c3", m, tis, kloc, t, n_enc=4, n_phases=n_phases, period=period, venc=venc,
                       motion=motion, velocity=velocity, enc_blocks=enc_blocks,
                       info=dict(encoding=enc, desc=f"4-D flow, {m.n_nodes}-node curved tube, {n_modes} POD modes, 4 encodings, "
-                                     f"{'x'.join(map(str, kspace))} x {n_phases} phases, direct per-line sum"))
+                                     f"{'x'.join(map(str, kspace))} x {n_phases} phases, per-readout GEMM with moving nodes"))
     return wl
 
 
