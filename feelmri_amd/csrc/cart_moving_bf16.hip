@@ -84,7 +84,7 @@ __device__ inline bf16x8 mv_neg(bf16x8 v)
 // cosine (which take revolutions).
 __device__ inline void cis_neg_f32(double u, float &c, float &s)
 {
-    const float rf = (float)(u - __builtin_rint(u));
+    const float rf = turns_f32(u);
     c = __builtin_amdgcn_cosf(rf);
     s = -__builtin_amdgcn_sinf(rf);
 }

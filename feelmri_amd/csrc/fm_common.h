@@ -46,6 +46,13 @@ inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s);
 
 __host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// A phase u in cycles reduced for the FP32 hardware sine and cosine, which
+// take revolutions and want an argument in [0, 1]: one fract (u - floor(u),
+// exact) and, from FP64, one conversion.  The result can round to 1.0f, which
+// the periodic functions take as 0.
+__device__ inline float turns_f32(double u) { return (float)__builtin_amdgcn_fract(u); }
+__device__ inline float turns_f32(float u) { return __builtin_amdgcn_fractf(u); }
+
 // (double)(float)pi / pi: the reference forms 2*pi from a float constant
 // (cpp/feelmri/MRI.h:13), so every k.x phase carries this factor.
 constexpr double kPiFOverPi = static_cast<double>(static_cast<float>(3.14159265358979323846)) /

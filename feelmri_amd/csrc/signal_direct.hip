@@ -17,10 +17,11 @@
 // a fixed order, so results do not depend on scheduling.
 //
 // Phase handling.  u = k'.x is accumulated in FP64 in cycles (k' carries the
-// reference's float-pi factor) and range-reduced to r = u - rint(u) in
-// [-1/2, 1/2].  FP64 form: sincospi(2r).  Mixed form: r rounded to FP32 and
-// fed to the hardware v_sin_f32 / v_cos_f32, which take revolutions;
-// partial sums over a 256-node chunk are kept in FP32 and added into FP64.
+// reference's float-pi factor).  FP64 form: r = u - rint(u) in [-1/2, 1/2],
+// sincospi(2r).  Mixed form: fract(u) in [0, 1) (one instruction, exact),
+// rounded to FP32 and fed to the hardware v_sin_f32 / v_cos_f32, which take
+// revolutions; partial sums over a 256-node chunk are kept in FP32 and added
+// into FP64.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                         double fr, fi;
                         if constexpr (MIXED) {
                             const double u = phi * tg[j] * (1.0 / kTwoPi);
-                            const float r = (float)(u - __builtin_rint(u));
+                            const float r = turns_f32(u);
                             const float d = __builtin_amdgcn_exp2f((float)(-tg[j] * r2 * 1.4426950408889634));
                             fr = (double)(d * __builtin_amdgcn_cosf(r));
                             fi = (double)(d * __builtin_amdgcn_sinf(r));
@@ -270,10 +271,9 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                 const double u = PLANAR ? fma(kxs[0], nd.x[0], kys[0] * nd.y[0])
                                         : fma(kxs[0], nd.x[0], fma(kys[0], nd.y[0], kzs[0] * nd.z[0]));
                 float r[2];
-                r[0] = (float)(u - __builtin_rint(u));
-                r[1] = r[0] + (PLANAR ? fmaf(dkx, nd.xf[0], dky * nd.xf[1])
-                                      : fmaf(dkx, nd.xf[0], fmaf(dky, nd.xf[1], dkz * nd.xf[2])));
-                r[1] -= __builtin_rintf(r[1]);
+                r[0] = turns_f32(u);
+                r[1] = turns_f32(r[0] + (PLANAR ? fmaf(dkx, nd.xf[0], dky * nd.xf[1])
+                                                : fmaf(dkx, nd.xf[0], fmaf(dky, nd.xf[1], dkz * nd.xf[2]))));
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     const CT sn = __builtin_amdgcn_sinf(r[j]), cs = __builtin_amdgcn_cosf(r[j]);
@@ -297,13 +297,13 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
                 double u = PLANAR ? fma(kxs[j], nd.x[jp], kys[j] * nd.y[jp])
                                   : fma(kxs[j], nd.x[jp], fma(kys[j], nd.y[jp], kzs[j] * nd.z[jp]));
                 if (!SHARED_T) u = fma(-nd.dwc, ts[j], u);  // phase - phi t, in cycles
-                const double rr = u - __builtin_rint(u);
                 CT cs, sn;
                 if (MIXED) {
-                    const float rf = (float)rr;
+                    const float rf = turns_f32(u);
                     sn = __builtin_amdgcn_sinf(rf);
                     cs = __builtin_amdgcn_cosf(rf);
                 } else {
+                    const double rr = u - __builtin_rint(u);
                     double s_, c_;
                     sincospi(2.0 * rr, &s_, &c_);
                     sn = (CT)s_;
@@ -432,8 +432,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
                 for (int j = 0; j < nst; ++j) {
                     const double *st = &run_steps[((int64_t)blockIdx.x * kRunSteps + j) * 4];
                     const float sdkx = (float)st[0], sdky = (float)st[1], sdkz = (float)st[2], sdt = (float)st[3];
-                    float du = fmaf(sdkx, nd.xf, fmaf(sdky, nd.yf, fmaf(sdkz, nd.zf, -nd.dwcf * sdt)));
-                    du -= __builtin_rintf(du);
+                    const float du = turns_f32(fmaf(sdkx, nd.xf, fmaf(sdky, nd.yf, fmaf(sdkz, nd.zf, -nd.dwcf * sdt))));
                     const float ds = __builtin_amdgcn_exp2f(-sdt * nd.r2f * 1.44269504f);
                     sq32[j * kSqStride + tid] = make_float2(ds * __builtin_amdgcn_cosf(du), -ds * __builtin_amdgcn_sinf(du));
                 }
@@ -452,7 +451,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
             const RunNode<VB> &nd = snode[i];
             // anchor: FP64 phase in cycles, range-reduced
             const double u = fma(rd.k0x, nd.x, fma(rd.k0y, nd.y, fma(rd.k0z, nd.z, -nd.dwc * rd.t0)));
-            const float r = (float)(u - __builtin_rint(u));
+            const float r = turns_f32(u);
             const float d0 = __builtin_amdgcn_exp2f((float)(-rd.t0 * nd.r2) * 1.44269504f);
             float er = d0 * __builtin_amdgcn_cosf(r);
             float ei = -d0 * __builtin_amdgcn_sinf(r);
@@ -463,8 +462,7 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_runs(
                 qr = q.x;
                 qi = q.y;
             } else {
-                float du = fmaf(dkx, nd.xf, fmaf(dky, nd.yf, fmaf(dkz, nd.zf, -nd.dwcf * dtf)));
-                du -= __builtin_rintf(du);
+                const float du = turns_f32(fmaf(dkx, nd.xf, fmaf(dky, nd.yf, fmaf(dkz, nd.zf, -nd.dwcf * dtf))));
                 const float ds = __builtin_amdgcn_exp2f(-dtf * nd.r2f * 1.44269504f);
                 qr = ds * __builtin_amdgcn_cosf(du);
                 qi = -ds * __builtin_amdgcn_sinf(du);

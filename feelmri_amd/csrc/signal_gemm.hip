@@ -291,7 +291,7 @@ __device__ __forceinline__ void cart_E_bf16(int64_t t, const double *__restrict_
             float er = 0.0f, ei = 0.0f;
             if (row_ok && q < nvalid) {
                 const double u = fma(fx, x[q], fma(fy, y[q], fz * z[q])) - ph[q] * ti * (1.0 / kTwoPi);
-                const float rf = (float)(u - __builtin_rint(u));
+                const float rf = turns_f32(u);
                 const float d = __builtin_amdgcn_exp2f((float)(-ti * r2[q] * 1.4426950408889634));
                 er = d * __builtin_amdgcn_cosf(rf);
                 ei = -d * __builtin_amdgcn_sinf(rf);
