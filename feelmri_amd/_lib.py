@@ -38,7 +38,7 @@ _SIGNATURES = {
     "fm_signal_plan_destroy": [_p],
     "fm_signal_plan_analyse": [_i64, _i64, _i64, _p, _p, _p, _p, _int, ctypes.POINTER(_int),
                                ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64),
-                               ctypes.POINTER(_int), ctypes.POINTER(_int)],
+                               ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)],
     "fm_signal_plan_info": [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_int), _p],
     "fm_signal_plan_runs": [_p, ctypes.POINTER(_i64), ctypes.POINTER(_int), ctypes.POINTER(_int)],
     "fm_signal_workspace_size": [_p, _int, _i64, _int, _int, ctypes.POINTER(_sz)],

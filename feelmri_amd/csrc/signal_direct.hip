@@ -47,12 +47,13 @@ struct CoefT<false> {
 // (the slots of a tile may belong to different t-groups) or, in packed slots
 // of GP t-groups, one per (slot, group); for moving nodes one position per
 // slot.
-template <int VB, bool MIXED, bool SHARED_T, bool MOTION, int GP = 1>
+template <int VB, bool MIXED, bool SHARED_T, bool MOTION, int GP = 1, bool CONJ = false>
 struct alignas(16) StagedNode {
     using CT = typename CoefT<MIXED>::T;
     static constexpr int kPos = MOTION ? kDirectSPL : 1;
-    static constexpr int kCoef = SHARED_T ? kDirectSPL * GP : 1;
-    static constexpr bool kPair = MIXED && GP > 1;  // second slot's phase by FP32 offset
+    // conjugate-pair tiles: both slots hold the same GP t-groups
+    static constexpr int kCoef = SHARED_T ? (CONJ ? GP : kDirectSPL * GP) : 1;
+    static constexpr bool kPair = MIXED && GP > 1 && !CONJ;  // second slot's phase by FP32 offset
     double x[kPos], y[kPos], z[kPos];
     double dwc;                  // phi/(2 pi), cycles/ms   (per-sample-t kernel only)
     double r2;                   // 1/T2, 1/ms              (per-sample-t kernel only)
@@ -76,7 +77,14 @@ struct alignas(16) StagedNode {
 // first factor), the second phase is formed as the first one's FP64-reduced
 // phase plus (k'1 - k'0).x in FP32: an FP32 error of ~1e-6 cycles on top of
 // the FP64 range reduction.  PLANAR drops the kz.z term (every kz = 0).
-template <int VB, bool MIXED, bool SHARED_T, bool MOTION, int GP = 1, bool PLANAR = false>
+//
+// Conjugate-pair tiles (CONJ, mixed form): both slots hold the same GP
+// t-groups, and lane l's slot-1 sample has k'_b = -k'_a of its slot-0
+// sample (to 1e-12 of the largest |k'|, the plan checks; e.g. spiral
+// interleaves half a turn apart).  One phase and one sine/cosine then serve
+// both: exp(-2 pi i k'_b.x) = conj(exp(-2 pi i k'_a.x)), and the two share
+// the t-group's coefficient c, so the lane adds c e and c conj(e).
+template <int VB, bool MIXED, bool SHARED_T, bool MOTION, int GP = 1, bool PLANAR = false, bool CONJ = false>
 __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     const double *__restrict__ packed, int64_t N, int V, int v0,
     const double4 *__restrict__ ks, const int32_t *__restrict__ perm,
@@ -85,8 +93,9 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
     int64_t nodes_per_split, double2 *__restrict__ dst, int64_t S, int write_final, double pair_dk)
 {
     using CT = typename CoefT<MIXED>::T;
-    using Node = StagedNode<VB, MIXED, SHARED_T, MOTION, GP>;
+    using Node = StagedNode<VB, MIXED, SHARED_T, MOTION, GP, CONJ>;
     static_assert(GP == 1 || (SHARED_T && !MOTION), "packed t-groups: shared-t tiles of static nodes");
+    static_assert(!CONJ || (MIXED && GP > 1 && kDirectSPL == 2), "conjugate pairs: mixed packed slots");
     static_assert(!PLANAR || GP > 1, "planar phases: packed slots only");
     constexpr bool kPair = Node::kPair && kDirectSPL == 2;
     constexpr int kSlots = SHARED_T ? kDirectSPL : 1;
@@ -262,7 +271,25 @@ __global__ __launch_bounds__(kDirectThreads) void k_signal_direct(
 #pragma unroll
             for (int v = 0; v < VB; ++v) acc_re[j][v] = acc_im[j][v] = (CT)0;
 
-        if (pair_path) {
+        if constexpr (CONJ) {
+            // one phase per pair; slot 1 takes the conjugate factor
+#pragma unroll 2
+            for (int m = 0; m < cnt; ++m) {
+                const Node &nd = snode[m];
+                const double u = PLANAR ? fma(kxs[0], nd.x[0], kys[0] * nd.y[0])
+                                        : fma(kxs[0], nd.x[0], fma(kys[0], nd.y[0], kzs[0] * nd.z[0]));
+                const float r = turns_f32(u);
+                const CT sn = __builtin_amdgcn_sinf(r), cs = __builtin_amdgcn_cosf(r);
+#pragma unroll
+                for (int v = 0; v < VB; ++v) {
+                    const CT cr = nd.c[gq][2 * v], ci = nd.c[gq][2 * v + 1];
+                    acc_re[0][v] = fma(cs, cr, fma(sn, ci, acc_re[0][v]));
+                    acc_im[0][v] = fma(cs, ci, fma(-sn, cr, acc_im[0][v]));
+                    acc_re[1][v] = fma(cs, cr, fma(-sn, ci, acc_re[1][v]));
+                    acc_im[1][v] = fma(cs, ci, fma(sn, cr, acc_im[1][v]));
+                }
+            }
+        } else if (pair_path) {
             // first slot: FP64 phase, range-reduced; second: plus dk.x in FP32
             if constexpr (kPair) {
 #pragma unroll 2
@@ -712,9 +739,10 @@ struct DirectLaunch {
 };
 
 // Tilings of the direct path: 0 consecutive, 1 t-grouped, 2 readout runs,
-// 3 packed t-groups.
+// 3 packed t-groups, 4 conjugate pairs in packed t-groups (mixed form).
 static int64_t direct_tiles(const fm_signal_plan *p, int tiling)
 {
+    if (tiling == 4) return p->n_tiles_conj;
     if (tiling == 3) return p->n_tiles_packed;
     if (tiling == 2) return cdiv(p->n_runs, kDirectThreads);
     return tiling == 1 ? p->n_tiles_grouped : p->n_tiles_consec;
@@ -798,14 +826,38 @@ static void launch_packed(const fm_signal_plan *p, dim3 grid, const double *pack
 #undef FM_LP
 }
 
+static void launch_conj(const fm_signal_plan *p, dim3 grid, const double *packed, int64_t N, int V, int v0,
+                        int64_t nps, double2 *dst, int write_final, hipStream_t st)
+{
+#define FM_LQ(GPV, PL)                                                                                          \
+    k_signal_direct<1, true, true, false, GPV, PL, true><<<grid, kDirectThreads, 0, st>>>(                     \
+        packed, N, V, v0, p->d_ks, p->d_perm_conj, p->d_tiles_conj, p->d_tg_t, nullptr, 0, nullptr, nps, dst, \
+        p->n_samples, write_final, 0.0)
+    if (p->planar) {
+        switch (p->conj_pack) {
+        case 4: FM_LQ(4, true); break;
+        case 8: FM_LQ(8, true); break;
+        default: FM_LQ(16, true); break;
+        }
+    } else {
+        switch (p->conj_pack) {
+        case 4: FM_LQ(4, false); break;
+        case 8: FM_LQ(8, false); break;
+        default: FM_LQ(16, false); break;
+        }
+    }
+#undef FM_LQ
+}
+
 static size_t direct_workspace(const fm_signal_plan *p, int64_t N, int V)
 {
     if (N <= 0 || p->n_samples == 0) return 0;
     // the largest of the tilings' needs (motion forces the grouped one)
     size_t need = 0;
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < 5; ++g) {
         if (g == 2 && p->n_runs == 0) continue;
         if (g == 3 && p->group_pack <= 1) continue;
+        if (g == 4 && p->conj_pack <= 1) continue;
         const DirectLaunch L = direct_launch_shape(p, N, g);
         if (L.n_splits > 1)
             need = std::max(need, (size_t)L.n_splits * (size_t)p->n_samples * (size_t)V * sizeof(double2));
@@ -829,8 +881,10 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
     const bool use_runs = !motion && p->n_runs > 0 && (precision != FM_PREC_F64 || p->runs_exact);
     // static nodes with small equal-size t-groups: packed slots
     const bool use_packed = !motion && !use_runs && p->group_pack > 1;
+    // mixed form: conjugate pairs when every packed group pairs up
+    const bool use_conj = use_packed && precision != FM_PREC_F64 && p->conj_pack > 1;
     const bool grouped = motion || p->prefer_grouped;
-    const int tiling = use_runs ? 2 : (use_packed ? 3 : (grouped ? 1 : 0));
+    const int tiling = use_runs ? 2 : (use_conj ? 4 : (use_packed ? 3 : (grouped ? 1 : 0)));
     const DirectLaunch L = direct_launch_shape(p, N, tiling);
     const size_t need = L.n_splits > 1 ? (size_t)L.n_splits * (size_t)S * (size_t)V * sizeof(double2) : 0;
     if (need > ws_bytes) {
@@ -871,6 +925,8 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
         for (int v0 = 0; v0 < V; ++v0) {
             if (precision == FM_PREC_F64)
                 launch_packed<false>(p, grid, packed, N, V, v0, L.nodes_per_split, dst, write_final, st);
+            else if (use_conj)
+                launch_conj(p, grid, packed, N, V, v0, L.nodes_per_split, dst, write_final, st);
             else
                 launch_packed<true>(p, grid, packed, N, V, v0, L.nodes_per_split, dst, write_final, st);
             FM_LAUNCH_CHECK("k_signal_direct (packed t-groups)");
@@ -905,7 +961,8 @@ namespace fm {
 // Host tables of a plan before upload.
 struct PlanHost {
     std::vector<int32_t> perm;
-    std::vector<SignalTile> tiles_g, tiles_c, tiles_p;
+    std::vector<SignalTile> tiles_g, tiles_c, tiles_p, tiles_q;
+    std::vector<int32_t> perm_conj;
     std::vector<double4> ks;
     std::vector<RunDesc> runs;
     std::vector<double> run_steps;    // (tiles, kRunSteps, 4)
@@ -1013,6 +1070,71 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
             const double4 a = ks[(size_t)perm[(size_t)(s0.offset + l)]];
             const double4 q = ks[(size_t)perm[(size_t)(s1.offset + l)]];
             p->packed_dk = std::max(p->packed_dk, std::fabs(q.x - a.x) + std::fabs(q.y - a.y) + std::fabs(q.z - a.z));
+        }
+    }
+
+    // Conjugate pairs (mixed form): in every packed t-group pair each sample
+    // with one whose k' is its negative, to 1e-12 of the largest |k'| (an
+    // O(g0^2) search per group of g0 <= 128 samples; k' = 0 pairs with
+    // another k' = 0).  All groups or none.
+    p->conj_pack = 1;
+    p->n_tiles_conj = 0;
+    p->conj_dk = 0.0;
+    if (p->group_pack > 1 && S > 0) {
+        const int64_t g0 = kDirectThreads / p->group_pack;
+        double kmax = 0.0;
+        for (int64_t s = 0; s < S; ++s)
+            kmax = std::max({kmax, std::fabs(ks[(size_t)s].x), std::fabs(ks[(size_t)s].y), std::fabs(ks[(size_t)s].z)});
+        const double tol = 1e-12 * (kmax > 0.0 ? kmax : 1.0);
+        std::vector<int32_t> first((size_t)S / 2), second((size_t)S / 2);  // per group: g0/2 pairs
+        bool ok = g0 % 2 == 0;
+        double worst = 0.0;
+        std::vector<char> used((size_t)g0);
+        for (int64_t g = 0; g < p->n_tgroups && ok; ++g) {
+            const int32_t *mem = &perm[(size_t)(g * g0)];  // uniform groups: group g at g * g0
+            std::fill(used.begin(), used.end(), 0);
+            int64_t np = 0;
+            for (int64_t a = 0; a < g0 && ok; ++a) {
+                if (used[(size_t)a]) continue;
+                const double4 ka = ks[(size_t)mem[a]];
+                int64_t best = -1;
+                double bd = 0.0;
+                for (int64_t b = a + 1; b < g0; ++b) {
+                    if (used[(size_t)b]) continue;
+                    const double4 kb = ks[(size_t)mem[b]];
+                    const double d = std::fabs(ka.x + kb.x) + std::fabs(ka.y + kb.y) + std::fabs(ka.z + kb.z);
+                    if (d <= tol && (best < 0 || d < bd)) {
+                        best = b;
+                        bd = d;
+                    }
+                }
+                if (best < 0) {
+                    ok = false;
+                    break;
+                }
+                used[(size_t)a] = used[(size_t)best] = 1;
+                first[(size_t)(g * g0 / 2 + np)] = mem[a];
+                second[(size_t)(g * g0 / 2 + np)] = mem[best];
+                worst = std::max(worst, bd);
+                ++np;
+            }
+        }
+        if (ok) {
+            const int64_t half = g0 / 2, gpc = kDirectThreads / half;
+            std::vector<int32_t> &pc = h.perm_conj;
+            pc.clear();
+            for (int64_t g = 0; g < p->n_tgroups; g += gpc) {
+                const int64_t ng = std::min<int64_t>(gpc, p->n_tgroups - g);
+                const int32_t o0 = (int32_t)pc.size();
+                for (int64_t q = g * half; q < (g + ng) * half; ++q) pc.push_back(first[(size_t)q]);
+                const int32_t o1 = (int32_t)pc.size();
+                for (int64_t q = g * half; q < (g + ng) * half; ++q) pc.push_back(second[(size_t)q]);
+                h.tiles_q.push_back({(int32_t)g, o0, (int32_t)(ng * half), 0});
+                h.tiles_q.push_back({(int32_t)g, o1, (int32_t)(ng * half), 0});
+            }
+            p->conj_pack = (int)gpc;
+            p->n_tiles_conj = (int64_t)h.tiles_q.size() / kDirectSPL;
+            p->conj_dk = worst;
         }
     }
 
@@ -1167,6 +1289,12 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
             if ((e = hipMalloc(&p->d_tiles_packed, sizeof(SignalTile) * h.tiles_p.size())) != hipSuccess) return fail(e, "hipMalloc");
             if ((e = hipMemcpy(p->d_tiles_packed, h.tiles_p.data(), sizeof(SignalTile) * h.tiles_p.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         }
+        if (!h.tiles_q.empty()) {
+            if ((e = hipMalloc(&p->d_tiles_conj, sizeof(SignalTile) * h.tiles_q.size())) != hipSuccess) return fail(e, "hipMalloc");
+            if ((e = hipMemcpy(p->d_tiles_conj, h.tiles_q.data(), sizeof(SignalTile) * h.tiles_q.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+            if ((e = hipMalloc(&p->d_perm_conj, sizeof(int32_t) * h.perm_conj.size())) != hipSuccess) return fail(e, "hipMalloc");
+            if ((e = hipMemcpy(p->d_perm_conj, h.perm_conj.data(), sizeof(int32_t) * h.perm_conj.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        }
         if ((e = hipMemcpy(p->d_tg_t, p->tg_t.data(), sizeof(double) * (size_t)p->n_tgroups, hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         if (p->n_runs > 0) {
             if ((e = hipMalloc(&p->d_runs, sizeof(RunDesc) * runs.size())) != hipSuccess) return fail(e, "hipMalloc");
@@ -1193,7 +1321,8 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
 extern "C" int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n_kz, const double *kx,
                                       const double *ky, const double *kz, const double *t, int method,
                                       int *method_out, int64_t *n_tgroups, int64_t *n_runs,
-                                      int64_t *n_tiles_grouped, int *runs_exact, int *group_pack)
+                                      int64_t *n_tiles_grouped, int *runs_exact, int *group_pack,
+                                      int *conj_pack)
 {
     FM_CHECK_ARG(n_meas >= 0 && n_lines >= 0 && n_kz >= 0, "negative k-space dimension");
     FM_CHECK_ARG(method == FM_METHOD_AUTO || method == FM_METHOD_DIRECT || method == FM_METHOD_CARTESIAN,
@@ -1215,6 +1344,7 @@ extern "C" int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n
     if (n_tiles_grouped) *n_tiles_grouped = p.n_tiles_grouped;
     if (runs_exact) *runs_exact = p.runs_exact ? 1 : 0;
     if (group_pack) *group_pack = p.group_pack;
+    if (conj_pack) *conj_pack = p.conj_pack;
     return FM_OK;
 }
 
@@ -1276,6 +1406,8 @@ extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
     (void)hipFree(p->d_run_nsteps);
     (void)hipFree(p->d_tiles_consec);
     (void)hipFree(p->d_tiles_packed);
+    (void)hipFree(p->d_tiles_conj);
+    (void)hipFree(p->d_perm_conj);
     (void)hipFree(p->d_tg_t);
     cartesian_free(p);
     delete p;

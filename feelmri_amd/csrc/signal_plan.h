@@ -73,6 +73,15 @@ struct fm_signal_plan {
     // (cycles per unit length), and whether every sample has kz = 0
     double packed_dk = 0.0;
     int planar = 0;
+    // conjugate pairs (mixed form): within every packed t-group the samples
+    // pair up as k'_b = -k'_a (to 1e-12 of the largest |k'|); a conj tile
+    // holds one member of each pair in slot 0 and its partner in slot 1,
+    // lane for lane, conj_pack t-groups per tile (1: none)
+    int conj_pack = 1;
+    int64_t n_tiles_conj = 0;
+    double conj_dk = 0.0;  // largest |k'_a + k'_b|_1 over the pairs
+    int32_t *d_perm_conj = nullptr;
+    fm::SignalTile *d_tiles_conj = nullptr;
     // device tables
     double4 *d_ks = nullptr;      // per sample: kx', ky', kz' (cycles/m, pi_f-scaled), t
     int32_t *d_perm = nullptr;    // samples ordered by t-group (grouped tiles index it)

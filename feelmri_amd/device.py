@@ -56,19 +56,21 @@ def _prec(precision) -> int:
 def analyse_trajectory(kloc, t, method: str = "auto") -> dict:
     """Host-only analysis of a trajectory (no device needed), as
     ``SignalPlan`` performs it: the method it would choose, the number of
-    distinct t, of readout runs and of t-grouped tiles."""
+    distinct t, of readout runs and of t-grouped tiles, and the t-groups per
+    packed slot and per conjugate-pair tile (1: none)."""
     kx, ky, kz = (np.ascontiguousarray(np.asarray(k, dtype=np.float64)) for k in kloc)
     tt = np.ascontiguousarray(np.asarray(t, dtype=np.float64))
     if tt.ndim != 3 or any(k.shape != tt.shape for k in (kx, ky, kz)):
         raise ValueError("kloc arrays and t must share one 3-D shape (n_meas, n_lines, n_kz)")
     meth, ng, nr, nt = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
-    ex, gp = ctypes.c_int(), ctypes.c_int()
+    ex, gp, cp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
     _lib.call("fm_signal_plan_analyse", *tt.shape, kx.ctypes.data_as(ctypes.c_void_p),
               ky.ctypes.data_as(ctypes.c_void_p), kz.ctypes.data_as(ctypes.c_void_p),
               tt.ctypes.data_as(ctypes.c_void_p), METHODS[method], ctypes.byref(meth), ctypes.byref(ng),
-              ctypes.byref(nr), ctypes.byref(nt), ctypes.byref(ex), ctypes.byref(gp))
+              ctypes.byref(nr), ctypes.byref(nt), ctypes.byref(ex), ctypes.byref(gp), ctypes.byref(cp))
     return {"method": METHOD_NAMES[meth.value], "n_tgroups": ng.value, "n_runs": nr.value,
-            "n_tiles_grouped": nt.value, "runs_exact": bool(ex.value), "group_pack": gp.value}
+            "n_tiles_grouped": nt.value, "runs_exact": bool(ex.value), "group_pack": gp.value,
+            "conj_pack": cp.value}
 
 
 class SignalPlan:

@@ -82,12 +82,15 @@ int fm_signal_plan_destroy(fm_signal_plan *plan);
  * of 16-sample readout runs and of t-grouped tiles, whether the runs are
  * linear to 1e-14 (runs_exact: the FP64 form takes them too), and how many
  * equal-size t-groups a packed slot holds (group_pack, 1 when the groups do
- * not pack).  No counterpart in the reference (plan introspection; usable
- * without a GPU). */
+ * not pack), and how many a conjugate-pair tile holds (conj_pack, 1 when
+ * some packed group's samples do not pair up as k and -k: the mixed form of
+ * static nodes then takes the packed slots).  Out-pointers may be NULL.  No
+ * counterpart in the reference (plan introspection; usable without a
+ * GPU). */
 int fm_signal_plan_analyse(int64_t n_meas, int64_t n_lines, int64_t n_kz, const double *kx,
                            const double *ky, const double *kz, const double *t, int method,
                            int *method_out, int64_t *n_tgroups, int64_t *n_runs,
-                           int64_t *n_tiles_grouped, int *runs_exact, int *group_pack);
+                           int64_t *n_tiles_grouped, int *runs_exact, int *group_pack, int *conj_pack);
 /* n_tgroups: number of distinct t; method: the method chosen
  * (FM_METHOD_DIRECT or FM_METHOD_CARTESIAN).  tgroup_t (host, may be
  * NULL) receives the n_tgroups distinct times in the order the motion

@@ -147,26 +147,49 @@ def test_spiral_packed_t_groups(gpu, n_il, v):
     _check(kloc, t, n=3001, v=v, seed=n_il)
 
 
+def _irregular_interleaves(n_il=32, seed=3):
+    """Spiral interleaves at irregular angles: equal-size t-groups that pack,
+    with no sample at -k of another."""
+    kloc, t = trajectories.spiral(0.24, 64, 90, 1, t_first=0.2)
+    base = kloc[0][:, 0, 0] + 1j * kloc[1][:, 0, 0]
+    ang = 2 * np.pi * np.sort(np.random.default_rng(seed).uniform(size=n_il))
+    K = base[:, None] * np.exp(1j * ang)[None, :]
+    kx, ky = np.ascontiguousarray(K.real[..., None]), np.ascontiguousarray(K.imag[..., None])
+    return (kx, ky, np.zeros_like(kx)), np.ascontiguousarray(np.broadcast_to(t[:, :1, :], kx.shape))
+
+
+@pytest.mark.parametrize("kind", ["conj", "pair"])
 @pytest.mark.parametrize("far", [False, True])
-def test_spiral_packed_pair_phase(gpu, far):
-    """In packed slots the mixed form takes the second sample of a lane (same
-    interleave, 8 readout indices on) as the first one's FP64-reduced phase
-    plus dk.x in FP32, for 256-node chunks whose nodes all have
-    |dk|_1 max|x| <= 4 cycles.  With `far`, nodes 700..1299 sit 25x further
-    out (up to 1.5 m), so their chunks take the FP64 phase for both samples.
-    Either way the error stays far inside the mixed bar: <= 2e-6."""
+def test_spiral_packed_mixed_paths(gpu, kind, far):
+    """The mixed form's two packed paths.  ``conj``: interleaves half a turn
+    apart pair up as k and -k at equal t, so one phase and one sine/cosine
+    serve both samples of a lane.  ``pair``: irregular interleave angles; the
+    second sample of a lane (same interleave, 8 readout indices on) is the
+    first one's FP64-reduced phase plus dk.x in FP32, for 256-node chunks
+    whose nodes all have |dk|_1 max|x| <= 4 cycles.  With ``far``, nodes
+    700..1299 sit 25x further out (up to 1.5 m): pair chunks take the FP64
+    phase for both samples.  Either way the error stays far inside the mixed
+    bar: <= 2e-6."""
     from feelmri_amd import MRI
+    from feelmri_amd.device import analyse_trajectory
     import oracle
 
-    kloc, t = trajectories.spiral(0.24, 64, 90, 32, t_first=0.2)
+    if kind == "conj":
+        kloc, t = trajectories.spiral(0.24, 64, 90, 32, t_first=0.2)
+    else:
+        kloc, t = _irregular_interleaves()
+    a = analyse_trajectory(kloc, t)
+    assert a["group_pack"] == 8 and a["conj_pack"] == (16 if kind == "conj" else 1)
     r0, phi, T2, mxy, w = _nodes(3001, 1, 21)
     if far:
         r0[700:1300] *= 25.0
     ref = oracle.Signal(0, sp.diags(w), kloc, t, r0, phi, T2, mxy, None)
-    got = MRI.Signal(0, sp.diags(w), kloc, t, r0, phi, T2, mxy, None, precision="mixed")
-    err = rel_l2(got, ref)
-    print("relL2 mixed", f"{err:.2e}")
-    assert err <= 2e-6
+    errs = {}
+    for prec in ("mixed", "f64"):
+        got = MRI.Signal(0, sp.diags(w), kloc, t, r0, phi, T2, mxy, None, precision=prec)
+        errs[prec] = rel_l2(got, ref)
+    print("relL2", {k: f"{e:.2e}" for k, e in errs.items()})
+    assert errs["mixed"] <= 2e-6 and errs["f64"] <= TOL_F64
 
 
 def test_random_times_per_sample(gpu):

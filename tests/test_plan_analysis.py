@@ -102,3 +102,37 @@ def test_packed_t_groups():
     t = np.asarray(t).copy()
     t[3, 0, 0] = 99.0
     assert analyse_trajectory(kloc, t)["group_pack"] == 1
+
+
+def _rotated_interleaves(angles, n_samples=90, res=64, fov=0.24, t_first=0.2):
+    """One spiral interleave rotated by each of ``angles``."""
+    kloc, t = trajectories.spiral(fov, res, n_samples, 1, t_first=t_first)
+    base = kloc[0][:, 0, 0] + 1j * kloc[1][:, 0, 0]
+    K = base[:, None] * np.exp(1j * np.asarray(angles))[None, :]
+    kx, ky = np.ascontiguousarray(K.real[..., None]), np.ascontiguousarray(K.imag[..., None])
+    tt = np.ascontiguousarray(np.broadcast_to(t[:, :1, :], kx.shape))
+    return (kx, ky, np.zeros_like(kx)), tt
+
+
+@pytest.mark.parametrize("n_il", [32, 64, 128])
+def test_conjugate_pairs_of_spiral_interleaves(n_il):
+    """Interleaves half a turn apart have k and -k at equal t: every packed
+    group pairs up, 512 / n_il groups per conjugate-pair tile."""
+    kloc, t = trajectories.spiral(0.24, 64, 90, n_il, t_first=0.2)
+    a = analyse_trajectory(kloc, t)
+    assert a["group_pack"] == 256 // n_il and a["conj_pack"] == 512 // n_il
+
+
+def test_no_conjugate_pairs_without_antipodal_interleaves():
+    """32 interleaves at irregular angles pack (equal groups) but do not pair."""
+    ang = 2 * np.pi * np.sort(np.random.default_rng(3).uniform(size=32))
+    kloc, t = _rotated_interleaves(ang)
+    a = analyse_trajectory(kloc, t)
+    assert a["group_pack"] == 8 and a["conj_pack"] == 1
+    # pairs present but one interleave nudged off by more than the 1e-12 tolerance
+    ang = 2 * np.pi * np.arange(32) / 32
+    kloc, t = _rotated_interleaves(ang)
+    assert analyse_trajectory(kloc, t)["conj_pack"] == 16
+    kx = kloc[0].copy()
+    kx[40, 5, 0] *= 1.0 + 1e-9
+    assert analyse_trajectory((kx, kloc[1], kloc[2]), t)["conj_pack"] == 1
