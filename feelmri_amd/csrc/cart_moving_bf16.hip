@@ -94,7 +94,12 @@ constexpr int kEzStride = 18;
 
 // Up to 3 column blocks the kernel fits 168 VGPRs (3 waves per SIMD);
 // 4 column blocks keep 8 accumulator tiles per wave and take 2.
-template <int W, int CB>
+// STD: a standard stack (lines along y, kz along z: the other components of
+// G and H are exactly 0) whose column tiles are all full.  A line or kz phase
+// is then one FP64 product, which rounds as the general three-term form
+// does (fma(0, x, fma(G_y, y, 0 z)) = G_y y), and no column needs masking:
+// the same bits with ~16 % fewer operand instructions per step.
+template <int W, int CB, bool STD = false>
 __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
     const double *__restrict__ packed, int64_t N, int V, const float *__restrict__ modes, int R,
     const double *__restrict__ mcoef, const int32_t *__restrict__ ro_group, const double *__restrict__ F,
@@ -181,7 +186,8 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
         float2 *dst = sEz + buf * ez_buf;
         for (int e = tid; e < nk * 16; e += kThreads) {
             const int kk = e / 16, m = s * 16 + e % 16;
-            const double u = fma(sH[3 * kk], sx[m], fma(sH[3 * kk + 1], sy[m], sH[3 * kk + 2] * sz[m]));
+            const double u = STD ? sH[3 * kk + 2] * sz[m]
+                                 : fma(sH[3 * kk], sx[m], fma(sH[3 * kk + 1], sy[m], sH[3 * kk + 2] * sz[m]));
             float c, sn;
             cis_neg_f32(u, c, sn);
             dst[kk * kEzStride + e % 16] = make_float2(c, sn);
@@ -248,8 +254,8 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
                             const int m = 8 * half + q;
                             const float2 a = ez[bkk[r] * kEzStride + m];
                             const float2 b = sc[bv[r]][s * 16 + m];
-                            br_[q] = bok[r] ? a.x * b.x - a.y * b.y : 0.0f;
-                            bi_[q] = bok[r] ? a.x * b.y + a.y * b.x : 0.0f;
+                            br_[q] = (STD || bok[r]) ? a.x * b.x - a.y * b.y : 0.0f;
+                            bi_[q] = (STD || bok[r]) ? a.x * b.y + a.y * b.x : 0.0f;
                         }
                         bf16x8 rh, rl, ih, il;
                         split8(br_, bi_, rh, rl, ih, il);
@@ -268,7 +274,7 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const int m = s * 16 + 8 * half + q;
-                    const double u = fma(Gx, sx[m], fma(Gy, sy[m], Gz * sz[m]));
+                    const double u = STD ? Gy * sy[m] : fma(Gx, sx[m], fma(Gy, sy[m], Gz * sz[m]));
                     cis_neg_f32(u, ar_[q], ai_[q]);
                 }
                 split8(ar_, ai_, arh, arl, aih, ail);
@@ -350,9 +356,14 @@ static void launch_moving_wc(const MovingShape &s, const CartesianTables &c, con
 {
     const int64_t grid = s.n_splits * s.M * s.n_rt * s.n_ct;
     const size_t dyn = 2 * (size_t)s.ez_rows * kEzStride * sizeof(float2);
-    k_cart_moving_bf16<W, CB><<<(unsigned)grid, 64 * W, dyn, st>>>(
-        packed, N, (int)s.V, modes, R, mcoef, c.d_ro_group, c.d_F, c.d_G, c.d_H, c.d_t, s.M, s.L, s.K, s.n_rt,
-        s.n_ct, s.n_splits, s.nodes_per_split, s.ez_rows, part);
+    const bool std_stack = c.axis_aligned && s.KV == s.n_ct * 32 * CB;
+#define FM_MV(STDV)                                                                                           \
+    k_cart_moving_bf16<W, CB, STDV><<<(unsigned)grid, 64 * W, dyn, st>>>(                                    \
+        packed, N, (int)s.V, modes, R, mcoef, c.d_ro_group, c.d_F, c.d_G, c.d_H, c.d_t, s.M, s.L, s.K, s.n_rt, \
+        s.n_ct, s.n_splits, s.nodes_per_split, s.ez_rows, part)
+    if (std_stack) FM_MV(true);
+    else FM_MV(false);
+#undef FM_MV
 }
 
 template <int W>

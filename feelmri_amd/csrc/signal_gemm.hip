@@ -94,6 +94,9 @@ bool cartesian_analyse(fm_signal_plan *p, const double *kx, const double *ky, co
     }
     for (auto *v : {&c.F, &c.G, &c.H})
         for (double &x : *v) x *= kPiFOverPi;
+    c.axis_aligned = true;
+    for (int64_t j = 0; j < L; ++j) c.axis_aligned &= c.G[(size_t)(3 * j)] == 0.0 && c.G[(size_t)(3 * j + 2)] == 0.0;
+    for (int64_t k = 0; k < K; ++k) c.axis_aligned &= c.H[(size_t)(3 * k)] == 0.0 && c.H[(size_t)(3 * k + 1)] == 0.0;
     c.ok = true;
     return true;
 }

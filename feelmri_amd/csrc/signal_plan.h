@@ -49,6 +49,9 @@ struct CartesianTables {
     std::vector<double> F, G, H;  // (n_ro,3), (n_ln,3), (n_kz,3)
     std::vector<double> t;        // (n_ro)
     std::vector<int32_t> ro_group;  // (n_ro) t-group of each readout index
+    // a standard stack: G varies along y only and H along z only (their
+    // other components are exactly 0), so a line or kz phase is one product
+    bool axis_aligned = false;
     double *d_F = nullptr, *d_G = nullptr, *d_H = nullptr, *d_t = nullptr;
     int32_t *d_ro_group = nullptr;
 };

@@ -77,6 +77,30 @@ def test_c1_full_size(gpu):
     _static_case(workloads.c1(), "cartesian", 11, gpu)
 
 
+def test_c1_full_size_fp32_direct_form(gpu):
+    """c1's FP32 form: the direct sum (FP32 readout runs) at full size against
+    the FP64 Cartesian GEMM over every sample; the FP64 direct form (FP64
+    readout runs) against the FP64 GEMM too."""
+    import torch
+
+    from feelmri_amd import device, workloads
+
+    wl = workloads.c1()
+    m, tis = wl.mesh, wl.tissue
+    mxy, w = _coefficients(m.n_nodes, 1, 14)
+    nodes = device.SignalNodes.from_arrays(m.nodes, tis.delta_omega, tis.T2star, mxy, weights=w, device=gpu)
+    gemm = device.SignalPlan(wl.kloc, wl.t, method="cartesian", device=gpu)
+    direct = device.SignalPlan(wl.kloc, wl.t, method="direct", device=gpu)
+    assert direct.method == "direct" and direct.n_runs > 0
+    ref = gemm.execute(nodes, precision="f64").cpu().numpy()
+    f32 = direct.execute(nodes, precision="mixed").cpu().numpy()
+    f64 = direct.execute(nodes, precision="f64").cpu().numpy()
+    torch.cuda.synchronize()
+    e32, e64 = rel_l2(f32, ref), rel_l2(f64, ref)
+    print(f"c1 direct FP32 vs FP64 GEMM {e32:.2e}, direct FP64 vs FP64 GEMM {e64:.2e}")
+    assert e32 <= 1e-4 and e64 <= 1e-10
+
+
 @pytest.mark.parametrize("kind", ["radial", "spiral"])
 def test_c2_full_size(gpu, kind):
     """973 272 nodes; radial 256 x 512 (readout runs), spiral 32 x 2048
