@@ -333,19 +333,24 @@ def serving_run(dev, n_requests: int, rate: float, max_active: int = 8):
     return out
 
 
-def _dtype(method: str, form: str) -> str:
+def _dtype(method: str, form: str, line_gemm: bool = False) -> str:
     """The type the dominant kernel computes in: bf16 MFMA inputs on the
-    Cartesian GEMMs, FP32 on the direct kernels, FP64 in the FP64 form."""
+    Cartesian and per-line GEMMs, FP32 on the direct kernels, FP64 in the
+    FP64 form."""
     if form == "f64":
         return "f64"
-    return "bf16" if method == "cartesian" else "f32"
+    return "bf16" if method == "cartesian" or line_gemm else "f32"
 
 
-def _arithmetic(method: str, form: str) -> str:
+def _arithmetic(method: str, form: str, line_gemm: bool = False) -> str:
     if form == "f64":
         return "FP64 throughout (FP64 MFMA on Cartesian plans)"
     if method == "cartesian":
         return "bf16 hi+lo operands, 3 products per real product, FP32 accumulation, FP64 split reduction"
+    if line_gemm:
+        return ("per-line GEMM S[Qp+q] = sum_n (c e Z^p)(z^q): FP64 phases range-reduced for FP32 sine/cosine, "
+                "FP32 operand chains, bf16 hi+lo operands with 3 products per real product, FP32 accumulation, "
+                "FP64 split reduction")
     return "FP64 phase and range reduction, FP32 sine/cosine and accumulation, FP64 chunk sums"
 
 
@@ -602,6 +607,11 @@ def measure_static(args, dev, wl, form: str, with_cpu: bool) -> dict:
         flops = 8.0 * work
         peak = PEAKS["bf16_mfma_tflops"] if form == "mixed" else PEAKS["fp64_mfma_tflops"]
         dom = "k_cart_gemm"
+    elif form == "mixed" and plan.line_gemm:
+        bound, unit = "mfma", "TFLOP/s"
+        flops = 8.0 * work
+        peak = PEAKS["bf16_mfma_tflops"]
+        dom = "k_signal_lines_bf16"
     elif form == "mixed" and plan.n_runs > 0:
         bound, unit = "valu", "TFLOP/s"
         flops = (8.0 + 8.0 * wl.n_enc) * wl.n_nodes * wl.n_samples
@@ -617,10 +627,10 @@ def measure_static(args, dev, wl, form: str, with_cpu: bool) -> dict:
     roof = {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": unit,
             "frac": achieved / peak, "traffic": load_traffic(dom, wl.name, form), "kernel_ms": ms_dom,
             "per_unit": ("8 flops per node*k-sample*encoding (complex MAC); the mixed GEMM executes 3 bf16 "
-                         "products (hi*hi + hi*lo + lo*hi) per real product, i.e. 24" if plan.method == "cartesian"
+                         "products (hi*hi + hi*lo + lo*hi) per real product, i.e. 24" if bound == "mfma"
                          else "8 flops (k.x 3 FMA + phi t 1 FMA) + 8 per encoding (complex MAC) and 2 "
                               "transcendentals per node*k-sample")}
-    if plan.method == "cartesian" and form == "mixed":
+    if bound == "mfma" and form == "mixed":
         roof["executed"] = 3.0 * achieved
     cpu = None
     err_cpu = {}
@@ -634,10 +644,10 @@ def measure_static(args, dev, wl, form: str, with_cpu: bool) -> dict:
             err_cpu["f64"] = rel_l2(sel(out64_h), ks)
         log(f"CPU: {cpu['value']:.3e} node*k/s on {cpu['cores']} ranks; GPU vs CPU: {err_cpu}")
     return {
-        "value": value, "ms_per_step": ms_ev, "dtype": _dtype(plan.method, form),
+        "value": value, "ms_per_step": ms_ev, "dtype": _dtype(plan.method, form, plan.line_gemm),
         "config": {"workload": wl.info["desc"], "config": wl.name, "nodes": wl.n_nodes,
                    "k_samples": wl.n_samples, "encodings": wl.n_enc, "method": plan.method, "form": form,
-                   "arithmetic": _arithmetic(plan.method, form)},
+                   "arithmetic": _arithmetic(plan.method, form, plan.line_gemm)},
         "relL2_vs_f64_gpu": err_f64, "image_relL2_vs_f64_gpu": err_img,
         "image_relL2_tukey_vs_f64_gpu": err_img_tukey, "relL2_vs_oracle_sampled": err_oracle,
         "relL2_vs_cpu_baseline_readouts": err_cpu, "forms": forms, "roofline": roof, "cpu_baseline": cpu,

@@ -41,6 +41,7 @@ _SIGNATURES = {
                                ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_int)],
     "fm_signal_plan_info": [_p, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_int), _p],
     "fm_signal_plan_runs": [_p, ctypes.POINTER(_i64), ctypes.POINTER(_int), ctypes.POINTER(_int)],
+    "fm_signal_plan_lines": [_p, ctypes.POINTER(_i64), ctypes.POINTER(_int)],
     "fm_signal_workspace_size": [_p, _int, _i64, _int, _int, ctypes.POINTER(_sz)],
     "fm_signal_nodes_pack": [_i64, _int, _int, _p, _p, _p, _p, _p, _p, _p],
     "fm_signal_execute": [_p, _int, _i64, _int, _p, _p, _int, _p, _p, _sz, _p, _p],

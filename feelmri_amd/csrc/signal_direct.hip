@@ -852,8 +852,9 @@ static void launch_conj(const fm_signal_plan *p, dim3 grid, const double *packed
 static size_t direct_workspace(const fm_signal_plan *p, int64_t N, int V)
 {
     if (N <= 0 || p->n_samples == 0) return 0;
-    // the largest of the tilings' needs (motion forces the grouped one)
-    size_t need = 0;
+    // the largest of the tilings' needs (motion forces the grouped one) and
+    // of the per-line GEMM's partial samples (one encoding at a time)
+    size_t need = lines_workspace(p, N);
     for (int g = 0; g < 5; ++g) {
         if (g == 2 && p->n_runs == 0) continue;
         if (g == 3 && p->group_pack <= 1) continue;
@@ -875,6 +876,8 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
         return zero_f64_async(out, 2 * S * V, st);
     }
     const bool motion = modes != nullptr && R > 0;
+    // mixed form, static nodes, whole linear lines: the per-line GEMM
+    if (!motion && precision != FM_PREC_F64 && lines_usable(p)) return lines_execute(p, N, V, packed, ws, ws_bytes, out, st);
     // positions depend on t, so moving nodes always take the t-grouped tiles;
     // static nodes take the readout runs when the plan has them (the FP64
     // form only when they are linear to 1e-14)
@@ -965,6 +968,7 @@ struct PlanHost {
     std::vector<int32_t> perm_conj;
     std::vector<double4> ks;
     std::vector<RunDesc> runs;
+    std::vector<RunDesc> lines;       // whole linear lines (per-line GEMM)
     std::vector<double> run_steps;    // (tiles, kRunSteps, 4)
     std::vector<int32_t> run_nsteps;  // (tiles)
 };
@@ -1221,6 +1225,36 @@ static int plan_analyse(fm_signal_plan *p, const double *kx, const double *ky, c
                 h.run_steps.clear();
                 h.run_nsteps.clear();
             }
+            // whole lines: every sample within 1e-12 of the line's first
+            // sample plus m times its slope (the per-line GEMM's model)
+            bool whole = true;
+            std::vector<RunDesc> &lines = h.lines;
+            lines.reserve((size_t)stride);
+            for (int64_t line = 0; line < stride && whole; ++line) {
+                const double4 f = ks[(size_t)line], l = ks[(size_t)((n_meas - 1) * stride + line)];
+                const double inv = 1.0 / (double)(n_meas - 1);
+                const double4 d = make_double4((l.x - f.x) * inv, (l.y - f.y) * inv, (l.z - f.z) * inv, (l.w - f.w) * inv);
+                for (int64_t m = 1; m < n_meas && whole; ++m) {
+                    const double4 c = ks[(size_t)(m * stride + line)];
+                    const double ek = std::max({std::fabs(c.x - f.x - m * d.x), std::fabs(c.y - f.y - m * d.y),
+                                                std::fabs(c.z - f.z - m * d.z)});
+                    whole = ek <= tolk && std::fabs(c.w - f.w - m * d.w) <= tolt;
+                }
+                RunDesc r{};
+                r.k0x = f.x;
+                r.k0y = f.y;
+                r.k0z = f.z;
+                r.t0 = f.w;
+                r.dkx = d.x;
+                r.dky = d.y;
+                r.dkz = d.z;
+                r.dt = d.w;
+                r.s0 = (int32_t)line;
+                r.len = (int32_t)n_meas;
+                lines.push_back(r);
+            }
+            if (whole) p->n_line_tabs = stride;
+            else lines.clear();
         } else {
             runs.clear();
         }
@@ -1299,6 +1333,10 @@ extern "C" int fm_signal_plan_create(fm_signal_plan **plan, int64_t n_meas, int6
         if (p->n_runs > 0) {
             if ((e = hipMalloc(&p->d_runs, sizeof(RunDesc) * runs.size())) != hipSuccess) return fail(e, "hipMalloc");
             if ((e = hipMemcpy(p->d_runs, runs.data(), sizeof(RunDesc) * runs.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
+        }
+        if (p->n_line_tabs > 0) {
+            if ((e = hipMalloc(&p->d_lines, sizeof(RunDesc) * h.lines.size())) != hipSuccess) return fail(e, "hipMalloc");
+            if ((e = hipMemcpy(p->d_lines, h.lines.data(), sizeof(RunDesc) * h.lines.size(), hipMemcpyHostToDevice)) != hipSuccess) return fail(e, "hipMemcpy");
         }
         if (p->run_steps > 0) {
             if ((e = hipMalloc(&p->d_run_steps, sizeof(double) * h.run_steps.size())) != hipSuccess) return fail(e, "hipMalloc");
@@ -1402,6 +1440,7 @@ extern "C" int fm_signal_plan_destroy(fm_signal_plan *p)
     (void)hipFree(p->d_perm);
     (void)hipFree(p->d_tiles_grouped);
     (void)hipFree(p->d_runs);
+    (void)hipFree(p->d_lines);
     (void)hipFree(p->d_run_steps);
     (void)hipFree(p->d_run_nsteps);
     (void)hipFree(p->d_tiles_consec);
@@ -1431,6 +1470,14 @@ extern "C" int fm_signal_plan_runs(const fm_signal_plan *p, int64_t *n_runs, int
     if (n_runs) *n_runs = p->n_runs;
     if (run_len) *run_len = kRunLen;
     if (runs_exact) *runs_exact = p->runs_exact ? 1 : 0;
+    return FM_OK;
+}
+
+extern "C" int fm_signal_plan_lines(const fm_signal_plan *p, int64_t *n_lines, int *line_gemm)
+{
+    FM_CHECK_ARG(p != nullptr, "plan is NULL");
+    if (n_lines) *n_lines = p->n_line_tabs;
+    if (line_gemm) *line_gemm = lines_usable(p) ? 1 : 0;
     return FM_OK;
 }
 

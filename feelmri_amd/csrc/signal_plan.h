@@ -33,6 +33,9 @@ struct RunDesc {
 };
 // distinct (dk, dt) steps a 256-run tile may share through LDS
 constexpr int kRunSteps = 8;
+// Lines linear in k and t over their whole length take the per-line GEMM
+// (mixed form, static nodes) from this many samples per line.
+constexpr int kLineMinLen = 128;
 
 struct SignalTile {
     int32_t group;   // t-group index (shared-t kernels) or -1
@@ -98,6 +101,11 @@ struct fm_signal_plan {
     // 1e-12 of run detection): the FP64 form may then take the runs too
     bool runs_exact = false;
     fm::RunDesc *d_runs = nullptr;
+    // whole lines linear in k and t (to 1e-12 of the largest |k'| / |t|
+    // from the line's first sample): one RunDesc per line, len = n_meas,
+    // s0 = the line's first sample; 0 when some line is not
+    int64_t n_line_tabs = 0;
+    fm::RunDesc *d_lines = nullptr;
     // per 256-run tile, its distinct steps (kRunSteps x (dkx, dky, dkz, dt))
     // and their count; run_steps = the largest count, 0 when some tile has
     // more than kRunSteps (each lane then forms its own step factor)
@@ -132,6 +140,12 @@ size_t cartesian_workspace(const fm_signal_plan *p, int precision, int64_t n_nod
 int cartesian_execute(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc,
                       const double *packed, void *ws, size_t ws_bytes, double *out,
                       hipStream_t stream);
+// Static nodes on linear lines, per-line GEMM, mixed form
+// (signal_lines_bf16.hip).
+bool lines_usable(const fm_signal_plan *p);
+size_t lines_workspace(const fm_signal_plan *p, int64_t n_nodes);
+int lines_execute(const fm_signal_plan *p, int64_t n_nodes, int n_enc, const double *packed, void *ws,
+                  size_t ws_bytes, double *out, hipStream_t stream);
 // Moving nodes, per-readout GEMM in either form (n_enc <= kMovMaxV).
 size_t cartesian_moving_workspace(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc);
 int cartesian_moving_execute(const fm_signal_plan *p, int precision, int64_t n_nodes, int n_enc,

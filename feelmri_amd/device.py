@@ -113,6 +113,11 @@ class SignalPlan:
         # form of static nodes takes the run kernel when n_runs > 0, the FP64
         # form its FP64 run kernel when the runs are also exact to 1e-14
         self.n_runs, self.run_len, self.runs_exact = nr.value, rl.value, bool(ex.value)
+        nl, lg = ctypes.c_int64(), ctypes.c_int()
+        _lib.call("fm_signal_plan_lines", self._h, ctypes.byref(nl), ctypes.byref(lg))
+        # whole linear lines: the mixed form of static nodes takes the
+        # per-line GEMM when line_gemm (it then precedes the run kernel)
+        self.n_line_tabs, self.line_gemm = nl.value, bool(lg.value)
 
     def sample_groups(self) -> np.ndarray:
         """Index into ``tgroup_t`` of every sample, in sample order."""

@@ -194,4 +194,4 @@ def c2(scale: float = 1.0, kind: str = "radial") -> Workload:
         kloc, t = trajectories.spiral(0.24, 256, ns, nl, density_exponent=2.0, t_first=0.5, t_readout=8.0)
     blocks = sequences.spoiled_gre(flip_deg=15, TR=5.0, n_dummy=16, n_lines=1, rf_dur=0.2)
     return Workload(f"c2-{kind}", m, tis, kloc, t, blocks,
-                    info=dict(desc=f"{kind} direct sum, {m.n_nodes}-node ellipsoid, {t.shape[0]}x{t.shape[1]}"))
+                    info=dict(desc=f"{kind} signal sum, {m.n_nodes}-node ellipsoid, {t.shape[0]}x{t.shape[1]}"))

@@ -48,7 +48,12 @@ def test_noalloc_equals_execute_twice(gpu):
     kloc, t = trajectories.radial(0.24, 128, 32, t_first=0.5)
     _, got, want = _run(gpu, kloc, t, 20_000, 2, "f64", 2)
     assert np.array_equal(got.cpu().numpy(), want.cpu().numpy())
-    print("no-alloc call 3: 3000 nodes, Cartesian 16 x 8 x 4, FP64", flush=True)
+    print("no-alloc call 3: 30000 nodes, radial 512 x 16, mixed (per-line GEMM)", flush=True)
+    kloc, t = trajectories.radial(0.24, 512, 16, t_first=0.3)
+    plan, got, want = _run(gpu, kloc, t, 30_000, 1, "mixed", 4)
+    assert plan.line_gemm
+    assert np.array_equal(got.cpu().numpy(), want.cpu().numpy())
+    print("no-alloc call 4: 3000 nodes, Cartesian 16 x 8 x 4, FP64", flush=True)
     kloc, t = trajectories.cartesian((0.24, 0.24, 0.24), (16, 8, 4), t_first=1.0)
     _, got, want = _run(gpu, kloc, t, 3_000, 1, "f64", 3)
     assert np.array_equal(got.cpu().numpy(), want.cpu().numpy())
