@@ -877,7 +877,7 @@ static int direct_execute(const fm_signal_plan *p, int precision, int64_t N, int
     }
     const bool motion = modes != nullptr && R > 0;
     // mixed form, static nodes, whole linear lines: the per-line GEMM
-    if (!motion && precision != FM_PREC_F64 && lines_usable(p)) return lines_execute(p, N, V, packed, ws, ws_bytes, out, st);
+    if (!motion && precision != FM_PREC_F64 && lines_usable(p, N)) return lines_execute(p, N, V, packed, ws, ws_bytes, out, st);
     // positions depend on t, so moving nodes always take the t-grouped tiles;
     // static nodes take the readout runs when the plan has them (the FP64
     // form only when they are linear to 1e-14)
@@ -1477,7 +1477,7 @@ extern "C" int fm_signal_plan_lines(const fm_signal_plan *p, int64_t *n_lines, i
 {
     FM_CHECK_ARG(p != nullptr, "plan is NULL");
     if (n_lines) *n_lines = p->n_line_tabs;
-    if (line_gemm) *line_gemm = lines_usable(p) ? 1 : 0;
+    if (line_gemm) *line_gemm = lines_usable(p, 0) ? 1 : 0;
     return FM_OK;
 }
 

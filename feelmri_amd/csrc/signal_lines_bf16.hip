@@ -38,7 +38,7 @@ typedef __attribute__((ext_vector_type(2))) float ln_f32x2;
 typedef __attribute__((ext_vector_type(2))) __bf16 ln_bf16x2;
 
 constexpr int kLineNB = 32;                  // nodes per operand build (two K steps of 16)
-constexpr int kLineStride = kLineNB + 4;     // LDS row stride, 32-bit words
+constexpr int kLineRow = 2 * kLineNB + 4;    // LDS words of an operand row: hi words, lo words, pad
 constexpr int64_t kLineNodesPerSplit = 16384;
 constexpr int kLineMaxRT = 4;                // row tiles of 16 (P <= 64)
 constexpr int kLineValuPerMfma = 12;         // scheduling: VALU slots between two products
@@ -78,8 +78,26 @@ __device__ inline bf16x8 ln_swap_neg(bf16x8 v)
     return __builtin_bit_cast(bf16x8, u);
 }
 
+// LDS operand of R rows (32-bit words): row r holds the bf16 hi words of its
+// 32 nodes, then the lo words (kLineRow words a row).  Even rows form one
+// block and odd rows a second block placed 32 banks away, since the two wave
+// halves write rows 2 j and 2 j + 1 at once (64 banks of 4 bytes); the
+// 16-byte fragment reads of 16 rows then also fall on distinct banks.
+template <int R>
+struct LinePlane {
+    static constexpr int kOdd = (R / 2) * kLineRow + (32 - ((R / 2) * kLineRow) % 64 + 64) % 64;
+    static constexpr int kWords = kOdd + (R / 2) * kLineRow;
+    __device__ static constexpr int row(int r) { return (r & 1) * kOdd + (r >> 1) * kLineRow; }
+};
+
+// Node data of one build, loaded a chunk ahead of its use.
+struct LineNodeRaw {
+    double x, y, z, phi, r2, br, bi;
+};
+
 // One wave per workgroup: line `line` of the (split, line) pair the
-// workgroup id maps to; P = 16 NRT rows, Q = 16 NCT columns.
+// workgroup id maps to; P = 16 NRT rows, Q = 16 NCT columns.  Node offsets
+// are 32-bit (N < kLineMaxNodes).
 template <int NRT, int NCT>
 __global__ __launch_bounds__(64) void k_signal_lines_bf16(const double *__restrict__ packed, int64_t N, int v,
                                                           const RunDesc *__restrict__ lines, int64_t n_lines,
@@ -87,8 +105,9 @@ __global__ __launch_bounds__(64) void k_signal_lines_bf16(const double *__restri
                                                           float2 *__restrict__ part)
 {
     constexpr int P = 16 * NRT, Q = 16 * NCT, PQ = P * Q;
-    __shared__ uint32_t sBh[Q * kLineStride], sBl[Q * kLineStride];
-    __shared__ uint32_t sAh[P * kLineStride], sAl[P * kLineStride];
+    using PB = LinePlane<Q>;
+    using PA = LinePlane<P>;
+    __shared__ uint32_t sB[PB::kWords], sA[PA::kWords];
 
     // workgroup -> (split, line); the lines of one split are contiguous and
     // dealt onto one XCD, which then reads the split's nodes from its L2
@@ -103,8 +122,8 @@ __global__ __launch_bounds__(64) void k_signal_lines_bf16(const double *__restri
     const int nl = lane & 31, half = lane >> 5;
     const int r16 = lane & 15, g16 = lane >> 4;
     const RunDesc ld = lines[line];
-    const int64_t n_begin = split * nodes_per_split;
-    const int64_t n_end = (n_begin + nodes_per_split < N) ? n_begin + nodes_per_split : N;
+    const uint32_t n_begin = (uint32_t)(split * nodes_per_split);
+    const uint32_t n_end = (uint32_t)std::min<int64_t>((int64_t)n_begin + nodes_per_split, N);
 
     // per half: the factor this lane forms (z or Z) and its chain start's
     // phase and time (c e or c e Z)
@@ -124,31 +143,37 @@ __global__ __launch_bounds__(64) void k_signal_lines_bf16(const double *__restri
                  *pr2 = packed + 4 * N, *pbr = packed + (5 + 2 * (int64_t)v) * N,
                  *pbi = packed + (6 + 2 * (int64_t)v) * N;
 
-    // operand build of the 32 nodes from `base` into the LDS planes; nodes
-    // past the split's end get c = 0 (so A = 0) and are never read back
-    auto build = [&](int64_t base) {
-        const int64_t n = base + nl;
-        const bool live = n < n_end;
-        const int64_t nn = live ? n : n_end - 1;
-        const double x = px[nn], y = py[nn], z = pz[nn], dwc = pphi[nn] * (1.0 / kTwoPi), r2 = pr2[nn];
-        const double brv = pbr[nn], biv = pbi[nn];  // loaded unconditionally (nn is in range)
-        const float cr = live ? (float)brv : 0.0f, ci = live ? (float)biv : 0.0f;
+    // this lane's node of the chunk at `base` (clamped into the split: the
+    // build zeroes c past its end)
+    auto load = [&](uint32_t base) {
+        const uint32_t n = base + nl;
+        const uint32_t nn = n < n_end ? n : n_end - 1;
+        return LineNodeRaw{px[nn], py[nn], pz[nn], pphi[nn], pr2[nn], pbr[nn], pbi[nn]};
+    };
+
+    // operand build of the chunk at `base` into the LDS planes; nodes past
+    // the split's end get c = 0 (so A = 0) and are never read back
+    auto build = [&](uint32_t base, const LineNodeRaw &nd) {
+        const bool live = base + nl < n_end;
+        const double dwc = nd.phi * (1.0 / kTwoPi);
+        const float cr = live ? (float)nd.br : 0.0f, ci = live ? (float)nd.bi : 0.0f;
         // phases in cycles (MRI.cpp:138-146, with k' = k pi_f / pi)
-        const double uz = fma(dkx, x, fma(dky, y, fma(dkz, z, -dwc * dt)));
-        const double ue = fma(ld.k0x, x, fma(ld.k0y, y, fma(ld.k0z, z, -dwc * ld.t0)));
-        const ln_f32x2 own = ln_cis_decay(sa * uz, sa * dt, r2);                    // z | Z
-        const ln_f32x2 eb = ln_cis_decay(half ? fma((double)Q, uz, ue) : ue, tb, r2);  // e | e Z
-        const ln_f32x2 oth = {__shfl_xor(own.x, 32), __shfl_xor(own.y, 32)};       // Z | z
+        const double uz = fma(dkx, nd.x, fma(dky, nd.y, fma(dkz, nd.z, -dwc * dt)));
+        const double ue = fma(ld.k0x, nd.x, fma(ld.k0y, nd.y, fma(ld.k0z, nd.z, -dwc * ld.t0)));
+        const ln_f32x2 own = ln_cis_decay(sa * uz, sa * dt, nd.r2);                    // z | Z
+        const ln_f32x2 eb = ln_cis_decay(half ? fma((double)Q, uz, ue) : ue, tb, nd.r2);  // e | e Z
+        const ln_f32x2 oth = {__shfl_xor(own.x, 32), __shfl_xor(own.y, 32)};          // Z | z
         const ln_f32x2 zf = half ? oth : own, Zf = half ? own : oth;
         const ln_f32x2 z2 = ln_cmul(zf, zf);
         // B: q = 2 j + half, from 1 (half 0) or z (half 1)
         ln_f32x2 b = half ? zf : ln_f32x2{1.0f, 0.0f};
+        uint32_t *wb = sB + half * PB::kOdd + nl;
 #pragma unroll
         for (int j = 0; j < Q / 2; ++j) {
             uint32_t hw, lw;
             ln_split(b.x, b.y, hw, lw);
-            sBh[(2 * j + half) * kLineStride + nl] = hw;
-            sBl[(2 * j + half) * kLineStride + nl] = lw;
+            wb[j * kLineRow] = hw;
+            wb[j * kLineRow + kLineNB] = lw;
             b = ln_cmul(b, z2);
         }
         // A' = conj(A): p = 2 j + half, from conj(c e) (half 0) or conj(c e Z)
@@ -156,41 +181,46 @@ __global__ __launch_bounds__(64) void k_signal_lines_bf16(const double *__restri
         const ln_f32x2 Zc = {Zf.x, -Zf.y};
         const ln_f32x2 Z2c = ln_cmul(Zc, Zc);
         ln_f32x2 a = ln_cmul(ln_f32x2{cr, -ci}, ln_f32x2{eb.x, -eb.y});
+        uint32_t *wa = sA + half * PA::kOdd + nl;
 #pragma unroll
         for (int j = 0; j < P / 2; ++j) {
             uint32_t hw, lw;
             ln_split(a.x, a.y, hw, lw);
-            sAh[(2 * j + half) * kLineStride + nl] = hw;
-            sAl[(2 * j + half) * kLineStride + nl] = lw;
+            wa[j * kLineRow] = hw;
+            wa[j * kLineRow + kLineNB] = lw;
             a = ln_cmul(a, Z2c);
         }
     };
 
     // Software pipeline: the fragments of chunk k go to registers, then the
-    // wave builds chunk k + 1 into the planes while the matrix cores run
-    // chunk k's products.
-    build(n_begin);
+    // wave builds chunk k + 1 into the planes (its node data loaded one chunk
+    // earlier) while the matrix cores run chunk k's products.
+    LineNodeRaw nxt = load(n_begin);
+    build(n_begin, nxt);
+    nxt = load(n_begin + kLineNB);
     __syncthreads();
-    for (int64_t base = n_begin; base < n_end; base += kLineNB) {
+    for (uint32_t base = n_begin; base < n_end; base += kLineNB) {
         bf16x8 bh[2][NCT], bl[2][NCT], ah[2][NRT], al[2][NRT];
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
             const int col = 16 * ks + 4 * g16;  // this lane's 4 nodes (words) of the K step
 #pragma unroll
             for (int b = 0; b < NCT; ++b) {
-                const int q = 16 * b + r16;
-                bh[ks][b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&sBh[q * kLineStride + col]));
-                bl[ks][b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&sBl[q * kLineStride + col]));
+                const uint32_t *rb = sB + PB::row(16 * b + r16) + col;
+                bh[ks][b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(rb));
+                bl[ks][b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(rb + kLineNB));
             }
 #pragma unroll
             for (int a = 0; a < NRT; ++a) {
-                const int p = 16 * a + r16;
-                ah[ks][a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&sAh[p * kLineStride + col]));
-                al[ks][a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&sAl[p * kLineStride + col]));
+                const uint32_t *ra = sA + PA::row(16 * a + r16) + col;
+                ah[ks][a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(ra));
+                al[ks][a] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(ra + kLineNB));
             }
         }
         __syncthreads();  // the planes are read before the next build rewrites them
-        build(base + kLineNB);
+        const LineNodeRaw cur = nxt;
+        nxt = load(base + 2 * kLineNB);
+        build(base + kLineNB, cur);
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
@@ -277,14 +307,15 @@ static LineShape line_shape(const fm_signal_plan *p, int64_t N)
     return s;
 }
 
-bool lines_usable(const fm_signal_plan *p)
+bool lines_usable(const fm_signal_plan *p, int64_t N)
 {
-    return p->n_line_tabs > 0 && p->n_meas >= kLineMinLen && p->n_meas <= 16 * kLineMaxRT * 32;
+    return p->n_line_tabs > 0 && p->n_meas >= kLineMinLen && p->n_meas <= 16 * kLineMaxRT * 32 &&
+           N < kLineMaxNodes;
 }
 
 size_t lines_workspace(const fm_signal_plan *p, int64_t N)
 {
-    if (!lines_usable(p) || N <= 0) return 0;
+    if (!lines_usable(p, N) || N <= 0) return 0;
     return line_shape(p, N).part_bytes;
 }
 
@@ -300,7 +331,7 @@ static void launch_lines_t(const LineShape &s, const fm_signal_plan *p, const do
 int lines_execute(const fm_signal_plan *p, int64_t N, int V, const double *packed, void *ws, size_t ws_bytes,
                   double *out, hipStream_t st)
 {
-    if (!lines_usable(p)) {
+    if (!lines_usable(p, N)) {
         set_error("plan has no linear lines for the line GEMM");
         return FM_ERR_UNSUPPORTED;
     }

@@ -36,6 +36,8 @@ constexpr int kRunSteps = 8;
 // Lines linear in k and t over their whole length take the per-line GEMM
 // (mixed form, static nodes) from this many samples per line.
 constexpr int kLineMinLen = 128;
+// ... and up to this many nodes (32-bit byte offsets into the node arrays).
+constexpr int64_t kLineMaxNodes = int64_t(1) << 28;
 
 struct SignalTile {
     int32_t group;   // t-group index (shared-t kernels) or -1
@@ -142,7 +144,7 @@ int cartesian_execute(const fm_signal_plan *p, int precision, int64_t n_nodes, i
                       hipStream_t stream);
 // Static nodes on linear lines, per-line GEMM, mixed form
 // (signal_lines_bf16.hip).
-bool lines_usable(const fm_signal_plan *p);
+bool lines_usable(const fm_signal_plan *p, int64_t n_nodes);
 size_t lines_workspace(const fm_signal_plan *p, int64_t n_nodes);
 int lines_execute(const fm_signal_plan *p, int64_t n_nodes, int n_enc, const double *packed, void *ws,
                   size_t ws_bytes, double *out, hipStream_t stream);

@@ -107,8 +107,9 @@ int fm_signal_plan_runs(const fm_signal_plan *plan, int64_t *n_runs, int *run_le
  * m dt from each line's first sample, to 1e-12 of the largest |k| / |t|):
  * n_lines = lines * kz when every line is, else 0.  line_gemm = 1 when the
  * mixed form of static nodes takes the per-line GEMM (n_meas from 128 to
- * 2048 samples): S[Q p + q] = sum_n (c e Z^p)[n] (z^q)[n] on the matrix
- * cores.  No counterpart in the reference (plan introspection). */
+ * 2048 samples, fewer than 2^28 nodes): S[Q p + q] = sum_n (c e Z^p)[n]
+ * (z^q)[n] on the matrix cores.  No counterpart in the reference (plan
+ * introspection). */
 int fm_signal_plan_lines(const fm_signal_plan *plan, int64_t *n_lines, int *line_gemm);
 /* Workspace (device bytes) fm_signal_execute needs for this plan, with
  * static nodes (moving = 0) or moving nodes (moving = 1: modes given). */
