@@ -7,7 +7,10 @@ c1  80^3 = 512 000-node box of side 0.2 m with two spherical inclusions
     (3 tissue classes), R2* ~ U[10, 50] 1/s; 3-D Cartesian 128 x 128 x 32,
     3 ms readout.
 c2  ~1 M-node ellipsoid, dB0 ~ N(0, 50 Hz); golden-angle radial 256 x 512
-    or variable-density spiral 32 x 2048.
+    or variable-density (p = 2) spiral stack 32 x 2048 under the default
+    scanner's gradient limits (res 256 over FOV 0.24 m: T_ro 1.82 ms; the
+    receiver bandwidth is set to 1024 / T_ro so that the readout holds 2048
+    samples at oversampling 2).
 c3  curved tube R = 2 cm, L = 20 cm (~2 M nodes), POD motion, 4 encodings,
     Cartesian 96 x 96 x 24 per phase.
 
@@ -191,7 +194,13 @@ def c2(scale: float = 1.0, kind: str = "radial") -> Workload:
         kloc, t = trajectories.radial(0.24, ns, nl, t_first=0.5, t_readout=2.56)
     else:
         ns, nl = max(2, int(round(2048 * scale))), max(2, int(round(32 * scale)))
-        kloc, t = trajectories.spiral(0.24, 256, ns, nl, density_exponent=2.0, t_first=0.5, t_readout=8.0)
+        fov, res = (0.24, 0.24, 0.24), (256, nl, 1)
+        # readout duration of the base spiral, then the bandwidth that gives ns samples
+        _, _, T_ms = trajectories.spiral_stack(fov, res, density_exponent=2.0, dtype=np.float64)
+        kloc, t, _ = trajectories.spiral_stack(fov, res, oversampling=2, t_start=0.5,
+                                               receiver_bw=(ns // 2) / (T_ms * 1e-3), density_exponent=2.0,
+                                               dtype=np.float64)
+        assert t.shape[0] == 2 * (ns // 2), t.shape
     blocks = sequences.spoiled_gre(flip_deg=15, TR=5.0, n_dummy=16, n_lines=1, rf_dur=0.2)
     return Workload(f"c2-{kind}", m, tis, kloc, t, blocks,
                     info=dict(desc=f"{kind} signal sum, {m.n_nodes}-node ellipsoid, {t.shape[0]}x{t.shape[1]}"))

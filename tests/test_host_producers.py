@@ -1,7 +1,10 @@
 """Pins for the host producers that feed the kernels: the event grid of a
 sequence block (python/feelmri/Bloch.py:105-129, :340), the Cartesian and
 golden-angle radial k-space extents (python/feelmri/KSpaceTraj.py:40-44,
-:183-220, :335-379) and the spiral's rotation of interleaves.
+:183-220, :335-379), the spiral's rotation of interleaves and the
+gradient-limited spiral stack (KSpaceTraj.py:386-562): its time law against
+closed forms and quadrature in the amplitude- and slew-limited regimes, the
+ADC grid, the end points, the turn count and the kz planes.
 
 Each is checked by what the definitions fix, not by retyping them: grid
 membership of every knot, the half-open dt grid, dt = diff(prepend 0), the
@@ -119,3 +122,73 @@ def test_spiral_interleaves_are_rotations():
     np.testing.assert_allclose(np.abs(K[-1, 0]), 0.5 * 256 / 0.24, rtol=1e-12)
     u = (len(K) // 2) / (len(K) - 1)
     np.testing.assert_allclose(np.abs(K[len(K) // 2, 0]) / np.abs(K[-1, 0]), u ** 2, rtol=1e-9)
+
+
+def _spiral_p1_law(n0, turns, kmax, g_max, slew, n=256):
+    """T of the p = 1 spiral K(u) = kmax u exp(2 pi i turns u) through
+    spiral_time_law on n points."""
+    u = np.linspace(0.0, 1.0, n)
+    K = kmax * u * np.exp(2j * np.pi * turns * u)
+    return trajectories.spiral_time_law(u, K, g_max=g_max, slew=slew)[1]
+
+
+def test_spiral_time_law_amplitude_limited_is_arc_length():
+    """With the slew limit out of reach the time is the arc length of K over
+    gamma G: kmax / (gamma G) * (a sqrt(1 + a^2) + asinh a) / (2 a), a = 2 pi
+    turns, gamma = 2 pi gammabar, G = 0.95 g_max."""
+    kmax, turns = 500.0, 6.0
+    T = _spiral_p1_law(256, turns, kmax, g_max=33.0, slew=1e9)
+    a = 2 * np.pi * turns
+    arc = kmax * (a * np.sqrt(1 + a * a) + np.arcsinh(a)) / (2 * a)
+    want = arc / (2 * np.pi * trajectories.GAMMABAR * 33e-3 * 0.95)
+    np.testing.assert_allclose(T, want, rtol=5e-3)
+
+
+def test_spiral_time_law_slew_limited_matches_quadrature():
+    """With the amplitude limit out of reach the time is the integral of
+    sqrt(|K''(u)| / (gamma S)), |K''| = kmax a sqrt((a u)^2 + 4) for p = 1."""
+    from scipy.integrate import quad
+
+    kmax, turns = 500.0, 6.0
+    T = _spiral_p1_law(256, turns, kmax, g_max=1e9, slew=180.0)
+    a = 2 * np.pi * turns
+    gS = 2 * np.pi * trajectories.GAMMABAR * 180.0 * 0.95
+    want = quad(lambda u: np.sqrt(kmax * a * np.sqrt((a * u) ** 2 + 4) / gS), 0.0, 1.0)[0]
+    np.testing.assert_allclose(T, want, rtol=1e-2)
+
+
+def test_spiral_time_law_respects_both_limits():
+    """On the default scanner the discrete gradient |dK/dt| / gamma of every
+    interval stays under 0.95 G_max."""
+    u = np.linspace(0.0, 1.0, 200)
+    K = 400.0 * u ** 2 * np.exp(2j * np.pi * 6.0 * u ** 0.5)
+    t, T = trajectories.spiral_time_law(u, K)
+    assert np.all(np.diff(t) > 0) and t[0] == 0.0 and T == t[-1]
+    G = np.abs(np.diff(K)) / np.diff(t) / (2 * np.pi * trajectories.GAMMABAR)
+    assert G.max() <= 33e-3 * 0.95 * 1.01
+
+
+@pytest.mark.parametrize("res,p,bw", [((64, 8, 1), 2.0, 128e3), ((100, 12, 3), 1.0, 256e3), ((16, 4, 2), 1.5, 64e3)])
+def test_spiral_stack_adc_grid_ends_and_planes(res, p, bw):
+    fov = (0.24, 0.22, 0.08)
+    (kx, ky, kz), t, T_ms = trajectories.spiral_stack(fov=fov, res=res, oversampling=2, t_start=1.5,
+                                                      receiver_bw=bw, density_exponent=p, dtype=np.float64)
+    n0, n_il, n_sl = res
+    n = 2 * int(np.round(T_ms * 1e-3 * bw))
+    assert kx.shape == ky.shape == kz.shape == t.shape == (n, n_il, n_sl)
+    # ADC times: uniform over [0, T], the same for every interleave and plane
+    np.testing.assert_allclose(t[:, 0, 0], 1.5 + np.linspace(0.0, T_ms, n), rtol=0, atol=1e-12)
+    assert np.all(t == t[:, :1, :1])
+    # ends: the centre, then kmax at 2 pi turns, turns = min(max(1, res/2), max(1, res/32))
+    K = kx[:, :, 0] + 1j * ky[:, :, 0]
+    kmax = 0.5 * n0 / fov[0]
+    turns = min(max(1.0, 0.5 * n0), max(1.0, n0 / 32))
+    np.testing.assert_allclose(K[0], 0.0, atol=1e-9)
+    np.testing.assert_allclose(K[-1, 0], kmax * np.exp(2j * np.pi * turns), rtol=1e-9, atol=1e-9 * kmax)
+    # interleaves are rotations by 2 pi m / M; radius kmax u^p along the base spiral
+    for m in range(n_il):
+        np.testing.assert_allclose(K[:, m], K[:, 0] * np.exp(2j * np.pi * m / n_il), atol=1e-9 * kmax)
+    # kz planes at (s - n // 2) / FOV_z
+    np.testing.assert_allclose(kz[0, 0, :], (np.arange(n_sl) - n_sl // 2) / fov[2], rtol=1e-12)
+    assert np.all(kz == kz[:1, :1, :])
+
