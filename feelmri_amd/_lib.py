@@ -57,8 +57,8 @@ _SIGNATURES = {
                        _p],
     "fm_bloch_chain_marked": [_int, _i64, _p, _dbl, _dbl, _p, _i64, _p, _p, _p, _p, _p, _int, _p, _p, _p, _p, _p,
                               _i64, _p, _p, _p],
-    "fm_events_bound": [_i64, _p, _p, _p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
-    "fm_events_build": [_i64, _p, _p, _p, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
+    "fm_events_bound": [_i64, _p, _p, _i64, _p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    "fm_events_build": [_i64, _p, _p, _p, _p, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
     "fm_mass_p1_color": [_i64, _i64, _p, _p, _p, _int, ctypes.POINTER(_int)],
     "fm_mass_p1_lumped": [_i64, _i64, _p, _int, _p, _p, _p, _int, _p, _p],
     "fm_mass_p1_incidence": [_i64, _i64, _p, _p, _p, _p, _p, _i64, ctypes.POINTER(_i64)],

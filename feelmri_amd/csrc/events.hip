@@ -5,8 +5,10 @@
 // dt_rf grids, the gradient knots and dt_gr grids, sorted and
 // de-duplicated), :44-49 (__call__: RF and gradients summed over the block's
 // pulses and gradients at each time) and :340 (dt = diff(times, prepend=0)).
-// Pulses are hard (constant B1 over (start, end]), gradients trapezoids
-// evaluated as numpy's interp over their four knots.
+// Pulses are hard (the reference's RF table of one constant value, read by
+// linear interpolation with 0 outside the closed interval [start, end],
+// python/feelmri/MRObjects.py:591-614), gradients trapezoids evaluated as
+// numpy's interp over their four knots.
 //
 // Three launches, no host round trip: (1) one workgroup per block generates
 // its candidate times into LDS, sorts them (bitonic) and counts the distinct
@@ -68,7 +70,7 @@ __host__ __device__ inline void trap_knots(const fm_trapezoid &g, double x[4])
 // The candidate runs of one block, in the host rule's order (the order does
 // not change the sorted result).  Returns the run count, or -1 when the
 // block has more parts than FM_EVENTS_MAX_PARTS.
-__host__ __device__ inline int block_segments(const fm_block_desc &bd, const fm_hard_pulse *pulses,
+__host__ __device__ inline int block_segments(const fm_block_desc &bd, const fm_rf_pulse *pulses,
                                               const fm_trapezoid *traps, EvSeg *seg, int64_t *n_cand)
 {
 #pragma clang fp contract(off)
@@ -92,7 +94,7 @@ __host__ __device__ inline int block_segments(const fm_block_desc &bd, const fm_
         }
     }
     for (int k = 0; k < bd.n_pulses; ++k) {
-        const fm_hard_pulse &p = pulses[bd.pulse0 + k];
+        const fm_rf_pulse &p = pulses[bd.pulse0 + k];
         const double end = p.start + p.dur;
         seg[ns++] = EvSeg{0.0, 0.0, 2, 2, bd.pulse0 + k};
         total += 2;
@@ -111,7 +113,7 @@ __host__ __device__ inline int block_segments(const fm_block_desc &bd, const fm_
     return ns;
 }
 
-__device__ inline double seg_value(const EvSeg &s, int64_t j, const fm_hard_pulse *pulses,
+__device__ inline double seg_value(const EvSeg &s, int64_t j, const fm_rf_pulse *pulses,
                                    const fm_trapezoid *traps)
 {
 #pragma clang fp contract(off)
@@ -121,8 +123,60 @@ __device__ inline double seg_value(const EvSeg &s, int64_t j, const fm_hard_puls
         trap_knots(traps[s.idx], x);
         return x[j];
     }
-    const fm_hard_pulse &p = pulses[s.idx];
+    const fm_rf_pulse &p = pulses[s.idx];
     return j == 0 ? p.start : p.start + p.dur;
+}
+
+// A pulse's value at t: its constant amp on [start, start + dur], or numpy's
+// interp(t, linspace(start, start + dur, n), table, left=0, right=0) -- the
+// grid j * step + start with step = (stop - start) / (n - 1), the last point
+// stop, the knot hits, slope times offset plus value.
+__device__ inline void pulse_value(const fm_rf_pulse &p, const double *__restrict__ tables, double t, double &re,
+                                   double &im)
+{
+#pragma clang fp contract(off)
+    const double stop = p.start + p.dur;
+    re = im = 0.0;
+    if (t < p.start || t > stop) return;
+    if (p.n_table == 0) {
+        re = p.amp_re;
+        im = p.amp_im;
+        return;
+    }
+    const int n = p.n_table;
+    const double *f = tables + 2 * p.table0;
+    if (t == stop) {
+        re = f[2 * (n - 1)];
+        im = f[2 * (n - 1) + 1];
+        return;
+    }
+    const double step = (stop - p.start) / (double)(n - 1);
+    auto X = [&](int j) { return j == n - 1 ? stop : (double)j * step + p.start; };
+    int lo = 0, hi = n - 1;  // X(lo) <= t < X(hi)
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (X(mid) <= t) lo = mid;
+        else hi = mid;
+    }
+    const double x0 = X(lo), x1 = X(lo + 1);
+    if (x0 == t) {
+        re = f[2 * lo];
+        im = f[2 * lo + 1];
+        return;
+    }
+    const double c[2][2] = {{f[2 * lo], f[2 * lo + 2]}, {f[2 * lo + 1], f[2 * lo + 3]}};
+    double out[2];
+    for (int k = 0; k < 2; ++k) {
+        const double slope = (c[k][1] - c[k][0]) / (x1 - x0);
+        double r = slope * (t - x0) + c[k][0];
+        if (isnan(r)) {
+            r = slope * (t - x1) + c[k][1];
+            if (isnan(r) && c[k][0] == c[k][1]) r = c[k][0];
+        }
+        out[k] = r;
+    }
+    re = out[0];
+    im = out[1];
 }
 
 // numpy's interp(t, knots, [0, amp, amp, 0], left=0, right=0)
@@ -158,7 +212,7 @@ struct EvShared {
 // Candidates of block b into sh.v[0, n_cand), sorted, padded to a power of
 // two with +inf.  Returns false (for the whole workgroup) when the block
 // does not fit.
-__device__ bool block_sorted_times(EvShared &sh, const fm_block_desc &bd, const fm_hard_pulse *pulses,
+__device__ bool block_sorted_times(EvShared &sh, const fm_block_desc &bd, const fm_rf_pulse *pulses,
                                    const fm_trapezoid *traps, int64_t max_cand)
 {
     const int tid = threadIdx.x;
@@ -218,7 +272,7 @@ __device__ inline int32_t count_distinct(const EvShared &sh, int lo, int hi)
 }
 
 __global__ __launch_bounds__(kEvThreads) void k_events_count(int64_t n_blocks, const fm_block_desc *blocks,
-                                                             const fm_hard_pulse *pulses,
+                                                             const fm_rf_pulse *pulses,
                                                              const fm_trapezoid *traps, int64_t max_cand,
                                                              int64_t *offsets)
 {
@@ -289,7 +343,7 @@ __device__ inline int32_t block_code(const fm_block_desc &bd)
 }
 
 __global__ __launch_bounds__(kEvThreads) void k_events_write(int64_t n_blocks, const fm_block_desc *blocks,
-                                                             const fm_hard_pulse *pulses,
+                                                             const fm_rf_pulse *pulses, const double *rf_tables,
                                                              const fm_trapezoid *traps, int64_t max_cand,
                                                              int64_t capacity, const int64_t *offsets,
                                                              double *times, double *rf, double *G, double *dt,
@@ -329,10 +383,10 @@ __global__ __launch_bounds__(kEvThreads) void k_events_write(int64_t n_blocks, c
             dt[r] = t - (i == 0 ? 0.0 : sh.v[i - 1]);
             double re = 0.0, im = 0.0;
             for (int k = 0; k < bd.n_pulses; ++k) {
-                const fm_hard_pulse &p = pulses[bd.pulse0 + k];
-                const bool on = t > p.start && t <= p.start + p.dur;
-                re = re + (on ? p.amp_re : 0.0);
-                im = im + (on ? p.amp_im : 0.0);
+                double pr, pi;
+                pulse_value(pulses[bd.pulse0 + k], rf_tables, t, pr, pi);
+                re = re + pr;
+                im = im + pi;
             }
             rf[2 * r] = re;
             rf[2 * r + 1] = im;
@@ -355,11 +409,13 @@ __global__ __launch_bounds__(kEvThreads) void k_events_write(int64_t n_blocks, c
 
 using namespace fm;
 
-extern "C" int fm_events_bound(int64_t n_blocks, const fm_block_desc *blocks, const fm_hard_pulse *pulses,
-                               const fm_trapezoid *traps, int64_t *max_candidates, int64_t *total_bound)
+extern "C" int fm_events_bound(int64_t n_blocks, const fm_block_desc *blocks, const fm_rf_pulse *pulses,
+                               int64_t n_table_points, const fm_trapezoid *traps, int64_t *max_candidates,
+                               int64_t *total_bound)
 {
     FM_CHECK_ARG(n_blocks >= 0, "negative n_blocks");
     FM_CHECK_ARG(max_candidates && total_bound && (n_blocks == 0 || blocks), "NULL argument");
+    FM_CHECK_ARG(n_table_points >= 0, "negative n_table_points");
     int64_t mx = 0, tot = 0;
     EvSeg seg[kEvMaxSegs];
     for (int64_t b = 0; b < n_blocks; ++b) {
@@ -371,6 +427,13 @@ extern "C" int fm_events_bound(int64_t n_blocks, const fm_block_desc *blocks, co
         const int ns = block_segments(bd, pulses, traps, seg, &n);
         FM_CHECK_ARG(ns > 0, "block %lld: at most %d pulses and %d trapezoids", (long long)b, FM_EVENTS_MAX_PARTS,
                      FM_EVENTS_MAX_PARTS);
+        for (int k = 0; k < bd.n_pulses; ++k) {
+            const fm_rf_pulse &p = pulses[bd.pulse0 + k];
+            FM_CHECK_ARG(p.n_table == 0 || (p.n_table >= 2 && p.table0 >= 0 && p.table0 + p.n_table <= n_table_points),
+                         "block %lld pulse %d: table [%lld, +%d) outside the %lld table points", (long long)b, k,
+                         (long long)p.table0, p.n_table, (long long)n_table_points);
+            FM_CHECK_ARG(p.dur > 0.0 || p.n_table == 0, "block %lld pulse %d: a table needs dur > 0", (long long)b, k);
+        }
         FM_CHECK_ARG(n >= 1, "block %lld has no event (empty extent)", (long long)b);
         mx = n > mx ? n : mx;
         tot += n;
@@ -380,9 +443,9 @@ extern "C" int fm_events_bound(int64_t n_blocks, const fm_block_desc *blocks, co
     return FM_OK;
 }
 
-extern "C" int fm_events_build(int64_t n_blocks, const fm_block_desc *blocks, const fm_hard_pulse *pulses,
-                               const fm_trapezoid *traps, int64_t max_candidates, int64_t capacity,
-                               int64_t *offsets, double *times, double *rf, double *G, double *dt, int32_t *marks,
+extern "C" int fm_events_build(int64_t n_blocks, const fm_block_desc *blocks, const fm_rf_pulse *pulses,
+                               const double *rf_tables, const fm_trapezoid *traps, int64_t max_candidates,
+                               int64_t capacity, int64_t *offsets, double *times, double *rf, double *G, double *dt, int32_t *marks,
                                int32_t *status, void *stream)
 {
     FM_CHECK_ARG(n_blocks >= 1, "need at least one block");
@@ -397,7 +460,8 @@ extern "C" int fm_events_build(int64_t n_blocks, const fm_block_desc *blocks, co
     FM_LAUNCH_CHECK("k_events_count");
     k_events_scan<<<1, kEvThreads, 0, st>>>(n_blocks, capacity, offsets, status);
     FM_LAUNCH_CHECK("k_events_scan");
-    k_events_write<<<nb, kEvThreads, 0, st>>>(n_blocks, blocks, pulses, traps, max_candidates, capacity, offsets,
+    k_events_write<<<nb, kEvThreads, 0, st>>>(n_blocks, blocks, pulses, rf_tables, traps, max_candidates, capacity,
+                                              offsets,
                                               times, rf, G, dt, marks);
     FM_LAUNCH_CHECK("k_events_write");
     return FM_OK;

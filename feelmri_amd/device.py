@@ -321,11 +321,12 @@ def bloch(nodes: BlochNodes, events: EventTable, M0: float, gamma: float, mxy0: 
     return mxy_out, mz_out
 
 
-# C layouts of fm_block_desc, fm_hard_pulse, fm_trapezoid
+# C layouts of fm_block_desc, fm_rf_pulse, fm_trapezoid
 BLOCK_DESC = np.dtype([("t0", "f8"), ("dur", "f8"), ("dt", "f8"), ("dt_rf", "f8"), ("dt_gr", "f8"),
                        ("pulse0", "i4"), ("n_pulses", "i4"), ("trap0", "i4"), ("n_traps", "i4"),
                        ("spoil", "i4"), ("store_col", "i4")], align=True)
-HARD_PULSE = np.dtype([("start", "f8"), ("dur", "f8"), ("amp_re", "f8"), ("amp_im", "f8")], align=True)
+RF_PULSE = np.dtype([("start", "f8"), ("dur", "f8"), ("amp_re", "f8"), ("amp_im", "f8"), ("table0", "i8"),
+                     ("n_table", "i4"), ("reserved", "i4")], align=True)
 TRAPEZOID = np.dtype([("start", "f8"), ("ramp", "f8"), ("flat", "f8"), ("amp", "f8"), ("axis", "i4"),
                       ("reserved", "i4")], align=True)
 
@@ -347,23 +348,28 @@ def marks_from_blocks(block_len, codes) -> np.ndarray:
 class DeviceEvents:
     """A block chain's event tables built on the device (``fm_events_build``)
     from block descriptions: ``desc`` (BLOCK_DESC records), ``pulses``
-    (HARD_PULSE), ``traps`` (TRAPEZOID).  Holds the device descriptions and
-    output tables; ``build()`` (re)writes the tables on a stream with no host
-    round trip, so it can be recorded into a HIP graph, and reads nothing
-    back: ``n_events`` stays on the device."""
+    (RF_PULSE), ``traps`` (TRAPEZOID) and ``rf_tables`` ((n, 2) float64 or
+    complex table points of the tabulated pulses).  Holds the device
+    descriptions and output tables; ``build()`` (re)writes the tables on a
+    stream with no host round trip, so it can be recorded into a HIP graph,
+    and reads nothing back: ``n_events`` stays on the device."""
 
-    def __init__(self, desc, pulses, traps, device=None):
+    def __init__(self, desc, pulses, traps, rf_tables=None, device=None):
         device = torch.device(device) if device is not None else default_device()
         desc = np.ascontiguousarray(desc, dtype=BLOCK_DESC)
-        pulses = np.ascontiguousarray(pulses, dtype=HARD_PULSE)
+        pulses = np.ascontiguousarray(pulses, dtype=RF_PULSE)
         traps = np.ascontiguousarray(traps, dtype=TRAPEZOID)
+        tab = np.zeros((0, 2)) if rf_tables is None else np.asarray(rf_tables)
+        if np.iscomplexobj(tab):
+            tab = np.stack([tab.real, tab.imag], axis=-1)
+        tab = np.ascontiguousarray(tab, dtype=np.float64).reshape(-1, 2)
         self.n_blocks = int(desc.size)
         if self.n_blocks == 0:
             raise ValueError("no block")
         mx, tot = ctypes.c_int64(), ctypes.c_int64()
         vp = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a.size else None  # noqa: E731
-        _lib.call("fm_events_bound", self.n_blocks, vp(desc), vp(pulses), vp(traps), ctypes.byref(mx),
-                  ctypes.byref(tot))
+        _lib.call("fm_events_bound", self.n_blocks, vp(desc), vp(pulses), int(tab.shape[0]), vp(traps),
+                  ctypes.byref(mx), ctypes.byref(tot))
         self.max_candidates, self.capacity = int(mx.value), max(int(tot.value), 1)
         if self.max_candidates > _lib.EVENTS_MAX_CANDIDATES:
             raise ValueError(f"a block has {self.max_candidates} grid candidates; the builder sorts at most "
@@ -371,6 +377,7 @@ class DeviceEvents:
         self.n_store = int(desc["store_col"].max()) + 1 if self.n_blocks else 0
         u8 = lambda a: torch.from_numpy(a.view(np.uint8).copy()).to(device)  # noqa: E731
         self.desc, self.pulses, self.traps = u8(desc), u8(pulses), u8(traps)
+        self.rf_tables = torch.from_numpy(tab.copy()).to(device)
         C = self.capacity
         self.offsets = torch.zeros(self.n_blocks + 1, dtype=torch.int64, device=device)
         self.times = torch.zeros(C, dtype=torch.float64, device=device)
@@ -389,6 +396,7 @@ class DeviceEvents:
     def build(self, stream=None) -> "DeviceEvents":
         with torch.cuda.device(self.device):
             _lib.call("fm_events_build", self.n_blocks, ptr(self.desc), ptr(self.pulses) if self.pulses.numel() else None,
+                      ptr(self.rf_tables) if self.rf_tables.numel() else None,
                       ptr(self.traps) if self.traps.numel() else None, self.max_candidates, self.capacity,
                       ptr(self.offsets), ptr(self.times), ptr(torch.view_as_real(self.rf)), ptr(self.G), ptr(self.dt),
                       ptr(self.marks), ptr(self.status), stream_handle(stream))

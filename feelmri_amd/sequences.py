@@ -12,10 +12,15 @@ extent's end is not on the grid), sorted and de-duplicated; then
 ``dt = diff(times, prepend=0)`` (Bloch.py:340).  Step i -> i+1 of the
 integrator uses the RF/gradient values at ``times[i+1]``.
 
-Pulses here are hard (constant B1 over (start, end]); gradients are
-trapezoids.  The reference's sinc pulses and slice-profile machinery
-(python/feelmri/MRObjects.py, MRImaging.py) are host-side sequence design
-and are not restated.
+RF pulses follow the reference's ``RF`` (python/feelmri/MRObjects.py:466-620):
+the unit pulse (hard, sinc or apodized sinc, optional phase and frequency
+offsets) is tabulated on ``nb_samples`` points over [start, start + dur],
+scaled so that gamma times its trapezoid-rule area is the flip angle, and
+read by linear interpolation, 0 outside the closed interval (so a pulse is
+on at its start and end knots).  ``HardPulse`` is the hard shape with one
+constant value, which the device event builder also forms; ``RFPulse``
+takes any shape (host tables).  Gradients are trapezoids.  The slice-profile
+machinery (MRImaging.py) is host-side sequence design and is not restated.
 """
 from __future__ import annotations
 
@@ -26,21 +31,90 @@ import numpy as np
 GAMMA = 42.58e6 * 2.0 * np.pi / 1e6   # rad/(ms*mT), python/feelmri/MRObjects.py:39-40
 
 
+def _flip_scale(tt: np.ndarray, unit: np.ndarray, flip: float) -> float:
+    """|flip / (gamma * trapezoid area of the unit pulse)| on the table grid
+    (MRObjects.py:591-600: the step is the grid's first spacing)."""
+    area = np.sum((unit[1:] + unit[:-1]) / 2) * (tt[1] - tt[0]) * GAMMA
+    return float(np.abs(flip / area))
+
+
 @dataclass
 class HardPulse:
+    """The reference's RF with ``shape='hard'`` and no frequency offset:
+    B1 = scale * exp(i phase) on [start, start + dur], 0 outside."""
     start: float        # ms
     dur: float          # ms
     flip: float         # rad
     phase: float = 0.0  # rad
+    nb_samples: int = 200
 
     @property
     def b1(self) -> float:
-        return self.flip / (GAMMA * self.dur)
+        tt = np.linspace(self.start, self.start + self.dur, self.nb_samples)
+        return _flip_scale(tt, np.ones(tt.size), self.flip)
 
     def __call__(self, t):
         t = np.asarray(t, dtype=np.float64)
-        on = (t > self.start) & (t <= self.start + self.dur)
+        on = (t >= self.start) & (t <= self.start + self.dur)
         return np.where(on, self.b1 * np.exp(1j * self.phase), 0.0 + 0.0j)
+
+
+@dataclass
+class RFPulse:
+    """The reference's ``RF`` (MRObjects.py:466-620) for ``time - ref =
+    start``: shape ``"apodized_sinc"`` (default, alpha 0.46), ``"sinc"``
+    (alpha 0) or ``"hard"``; ``n_lobes`` = (left, right) lobes; phase (rad)
+    and frequency offset (Hz) applied as exp(i (phase + 2 pi f t')), t' the
+    time from the main lobe's centre.
+
+    Unit pulse: with bw = (n_left + n_right + 2) / dur and t' = t - start -
+    (n_left + 1) / bw, (1 / bw) ((1 - alpha) + alpha cos(pi bw t' / N))
+    sinc(bw t'), N = max lobes, on [start, start + dur].  Tabulated on
+    ``nb_samples`` points, scaled to the flip angle and interpolated
+    linearly, 0 outside."""
+    start: float
+    dur: float
+    flip: float
+    shape: str = "apodized_sinc"
+    n_lobes: tuple = (2, 2)
+    alpha: float = 0.46
+    nb_samples: int = 200
+    phase: float = 0.0
+    freq_offset: float = 0.0   # Hz
+
+    def __post_init__(self):
+        if self.shape not in ("apodized_sinc", "sinc", "hard"):
+            raise ValueError(f"unknown RF shape {self.shape!r}")
+        if self.shape == "sinc":
+            self.alpha = 0.0
+        self._t = np.linspace(self.start, self.start + self.dur, self.nb_samples)
+        unit = self.unit(self._t)
+        scale = _flip_scale(self._t, unit, self.flip)
+        self._re, self._im = scale * unit.real, scale * unit.imag
+
+    def unit(self, t):
+        """The unscaled pulse (MRObjects.py:522-589)."""
+        t = np.asarray(t, dtype=np.float64)
+        nl, nr = self.n_lobes
+        half1 = (nl + 1) / (nl + nr + 2) * self.dur
+        ts = t - self.start - half1
+        window = (t >= self.start) & (t <= self.start + self.dur)
+        if self.shape == "hard":
+            b = np.where(window, 1.0, 0.0)
+        else:
+            bw = (nl + nr + 2) / self.dur
+            n = max(nl, nr)
+            b = (1.0 / bw) * ((1 - self.alpha) + self.alpha * np.cos(np.pi * bw * ts / n)) * np.sinc(bw * ts)
+            b = b * window
+        b = b + 0j
+        if self.phase != 0.0 or self.freq_offset != 0.0:
+            b = b * np.exp(1j * (self.phase + 2 * np.pi * (self.freq_offset * 1e-3) * ts))
+        return b
+
+    def __call__(self, t):
+        t = np.asarray(t, dtype=np.float64)
+        return np.interp(t, self._t, self._re, left=0.0, right=0.0) + \
+            1j * np.interp(t, self._t, self._im, left=0.0, right=0.0)
 
 
 @dataclass
@@ -118,13 +192,17 @@ def make_block(t0: float, dur: float, pulses=(), gradients=(), dt_rf: float = 0.
 def block_descriptions(blocks):
     """The blocks' specifications (``make_block``'s arguments, kept in
     ``Block.info["spec"]``) as the records of the device event builder:
-    (desc, pulses, trapezoids) numpy structured arrays in the C layouts of
-    fm_block_desc, fm_hard_pulse and fm_trapezoid.  Store columns are
-    numbered over the storing blocks in order; a non-empty block spoils."""
-    from .device import BLOCK_DESC, HARD_PULSE, TRAPEZOID
+    (desc, pulses, trapezoids, tables) -- numpy structured arrays in the C
+    layouts of fm_block_desc, fm_rf_pulse and fm_trapezoid, and the
+    (points, 2) float64 table points of the tabulated pulses.  A
+    ``HardPulse`` is one constant value; an ``RFPulse`` is its interpolation
+    table.  Store columns are numbered over the storing blocks in order; a
+    non-empty block spoils."""
+    from .device import BLOCK_DESC, RF_PULSE, TRAPEZOID
 
     desc = np.zeros(len(blocks), dtype=BLOCK_DESC)
-    pl, tl = [], []
+    pl, tl, tabs = [], [], []
+    n_tab = 0
     col = 0
     for b, blk in enumerate(blocks):
         spec = blk.info.get("spec")
@@ -139,11 +217,20 @@ def block_descriptions(blocks):
         d["store_col"] = col if blk.store else -1
         col += int(blk.store)
         for p in spec["pulses"]:
-            amp = complex(p.b1 * np.exp(1j * p.phase))   # the value HardPulse.__call__ forms
-            pl.append((p.start, p.dur, amp.real, amp.imag))
+            if isinstance(p, HardPulse):
+                amp = complex(p.b1 * np.exp(1j * p.phase))   # the value HardPulse.__call__ forms
+                pl.append((p.start, p.dur, amp.real, amp.imag, 0, 0, 0))
+            elif isinstance(p, RFPulse):
+                tabs.append(np.stack([p._re, p._im], axis=1))
+                pl.append((p.start, p.dur, 0.0, 0.0, n_tab, p._re.size, 0))
+                n_tab += p._re.size
+            else:
+                raise NotImplementedError(f"block {b}: the device event builder takes HardPulse and RFPulse, "
+                                          f"not {type(p).__name__}")
         for g in spec["gradients"]:
             tl.append((g.start, g.ramp, g.flat, g.amp, g.axis, 0))
-    return desc, np.array(pl, dtype=HARD_PULSE), np.array(tl, dtype=TRAPEZOID)
+    return (desc, np.array(pl, dtype=RF_PULSE), np.array(tl, dtype=TRAPEZOID),
+            np.concatenate(tabs) if tabs else np.zeros((0, 2)))
 
 
 def delay_block(t0: float, dur: float, dt: float = 1.0) -> Block:

@@ -231,8 +231,8 @@ int fm_bloch_chain(int precision, int64_t n_pos, const double *packed, double M0
 /* ------------------------------------------------------------------------
  * Sequence event tables built on the device -- replaces
  * SequenceBlock._discretization and __call__ (python/feelmri/Bloch.py:105-129,
- * :44-49) with dt = diff(times, prepend=0) (Bloch.py:340), for hard pulses
- * and trapezoidal gradients; with fm_bloch_chain_marked a whole block chain
+ * :44-49) with dt = diff(times, prepend=0) (Bloch.py:340), for RF pulses
+ * (constant or tabulated) and trapezoidal gradients; with fm_bloch_chain_marked a whole block chain
  * (BlochSolver.solve, Bloch.py:298-380) runs from device descriptions with
  * nothing read back to the host.
  * ---------------------------------------------------------------------- */
@@ -242,7 +242,7 @@ int fm_bloch_chain(int precision, int64_t n_pos, const double *packed, double M0
  * start + i * ((start + step) - start); dt_rf, dt_gr <= 0: no grid).  The
  * extent runs from min(t0, pulse starts, trapezoid starts) to
  * max(t0 + dur, pulse ends, trapezoid ends).  RF at time t: the sum, in
- * order, of amp over the pulses with start < t <= start + dur; gradient on
+ * order, of each pulse's value (fm_rf_pulse); gradient on
  * an axis: the sum, in order, of numpy's interp of each trapezoid of that
  * axis over its knots (zero outside).  dt must be > 0. */
 typedef struct fm_block_desc {
@@ -253,10 +253,19 @@ typedef struct fm_block_desc {
     int32_t store_col;                  /* store column, -1: none */
 } fm_block_desc;
 
-typedef struct fm_hard_pulse {
-    double start, dur;                  /* ms: constant B1 over (start, start + dur] */
-    double amp_re, amp_im;              /* mT */
-} fm_hard_pulse;
+/* One RF pulse on [start, start + dur], as the reference's RF reads its
+ * table (MRObjects.py:591-614: linear interpolation, 0 outside the closed
+ * interval).  n_table == 0: the constant amp (the hard shape);
+ * n_table >= 2: the (re, im) table rf_tables[table0 .. table0 + n_table) on
+ * the grid linspace(start, start + dur, n_table) (j * step + start, the last
+ * point start + dur), read as numpy's interp -- any shape (apodized sinc,
+ * sinc, hard with a frequency offset). */
+typedef struct fm_rf_pulse {
+    double start, dur;                  /* ms */
+    double amp_re, amp_im;              /* mT, when n_table == 0 */
+    int64_t table0;                     /* first table point, when n_table >= 2 */
+    int32_t n_table, reserved;
+} fm_rf_pulse;
 
 typedef struct fm_trapezoid {
     double start, ramp, flat, amp;      /* ms, mT/m: knots start, +ramp, +flat, +ramp */
@@ -269,11 +278,15 @@ typedef struct fm_trapezoid {
 /* Host: per-block candidate counts (grid points before de-duplication):
  * max_candidates receives the largest (the builder's per-block bound) and
  * total_bound their sum (a bound on the event count).  Same arithmetic as
- * the device builder.  Host arrays. */
-int fm_events_bound(int64_t n_blocks, const fm_block_desc *blocks, const fm_hard_pulse *pulses,
-                    const fm_trapezoid *traps, int64_t *max_candidates, int64_t *total_bound);
+ * the device builder.  Also checks every pulse table against the
+ * n_table_points points of rf_tables: fm_events_build reads the
+ * descriptions as given, so run this check on them first.  Host arrays. */
+int fm_events_bound(int64_t n_blocks, const fm_block_desc *blocks, const fm_rf_pulse *pulses,
+                    int64_t n_table_points, const fm_trapezoid *traps, int64_t *max_candidates,
+                    int64_t *total_bound);
 
-/* Device: the blocks' event tables, concatenated.  blocks, pulses, traps:
+/* Device: the blocks' event tables, concatenated.  blocks, pulses,
+ * rf_tables (2 x float64 per point; NULL when no pulse has a table), traps:
  * device arrays; max_candidates: a bound from fm_events_bound (at most
  * FM_EVENTS_MAX_CANDIDATES); capacity: entries of times, dt (float64), rf
  * (2 x float64), G (3 x float64) and, plus one, of marks (int32).
@@ -285,8 +298,9 @@ int fm_events_bound(int64_t n_blocks, const fm_block_desc *blocks, const fm_hard
  * events exceed capacity (the tables are then incomplete; nothing is
  * written out of bounds).  Three launches, nothing allocated: the call can
  * be recorded into a HIP graph. */
-int fm_events_build(int64_t n_blocks, const fm_block_desc *blocks, const fm_hard_pulse *pulses,
-                    const fm_trapezoid *traps, int64_t max_candidates, int64_t capacity, int64_t *offsets,
+int fm_events_build(int64_t n_blocks, const fm_block_desc *blocks, const fm_rf_pulse *pulses,
+                    const double *rf_tables, const fm_trapezoid *traps, int64_t max_candidates, int64_t capacity,
+                    int64_t *offsets,
                     double *times, double *rf, double *G, double *dt, int32_t *marks, int32_t *status,
                     void *stream);
 

@@ -39,10 +39,10 @@ def test_events_bound_counts_the_host_rules_candidates():
     from feelmri_amd import _lib
 
     blocks = _blocks()
-    desc, pulses, traps = sequences.block_descriptions(blocks)
+    desc, pulses, traps, _ = sequences.block_descriptions(blocks)
     mx, tot = ctypes.c_int64(), ctypes.c_int64()
     vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    _lib.call("fm_events_bound", desc.size, vp(desc), vp(pulses), vp(traps), ctypes.byref(mx), ctypes.byref(tot))
+    _lib.call("fm_events_bound", desc.size, vp(desc), vp(pulses), 0, vp(traps), ctypes.byref(mx), ctypes.byref(tot))
     want = [_candidates(b.info["spec"]) for b in blocks]
     assert mx.value == max(want) and tot.value == sum(want)
     assert all(b.n_events <= w for b, w in zip(blocks, want))
@@ -50,17 +50,25 @@ def test_events_bound_counts_the_host_rules_candidates():
     bad = desc[:1].copy()
     bad["n_pulses"] = _lib.EVENTS_MAX_PARTS + 1
     with pytest.raises(RuntimeError, match="at most"):
-        _lib.call("fm_events_bound", 1, vp(bad), vp(pulses), vp(traps), ctypes.byref(mx), ctypes.byref(tot))
+        _lib.call("fm_events_bound", 1, vp(bad), vp(pulses), 0, vp(traps), ctypes.byref(mx), ctypes.byref(tot))
     empty = np.zeros(1, dtype=device.BLOCK_DESC)
     empty["dt"], empty["store_col"] = 1.0, -1
     with pytest.raises(RuntimeError, match="no event"):
-        _lib.call("fm_events_bound", 1, vp(empty), None, None, ctypes.byref(mx), ctypes.byref(tot))
+        _lib.call("fm_events_bound", 1, vp(empty), None, 0, None, ctypes.byref(mx), ctypes.byref(tot))
+    # a pulse table outside the table points is refused
+    tp = pulses[:1].copy()
+    tp["n_table"], tp["table0"] = 200, 10
+    one = desc[2:3].copy()
+    one["pulse0"], one["n_pulses"], one["n_traps"] = 0, 1, 0
+    with pytest.raises(RuntimeError, match="table"):
+        _lib.call("fm_events_bound", 1, vp(one), vp(tp), 205, None, ctypes.byref(mx), ctypes.byref(tot))
+    _lib.call("fm_events_bound", 1, vp(one), vp(tp), 210, None, ctypes.byref(mx), ctypes.byref(tot))
 
 
 def test_block_records():
     blocks = _blocks()
-    desc, pulses, traps = sequences.block_descriptions(blocks)
-    assert desc.dtype.itemsize == 64 and pulses.dtype.itemsize == 32 and traps.dtype.itemsize == 40
+    desc, pulses, traps, _ = sequences.block_descriptions(blocks)
+    assert desc.dtype.itemsize == 64 and pulses.dtype.itemsize == 48 and traps.dtype.itemsize == 40
     assert list(desc["store_col"]) == [-1, -1, -1, -1, 0, -1, 1, -1, 2, -1]
     assert list(desc["spoil"]) == [0 if b.empty else 1 for b in blocks]
     assert desc["n_pulses"].sum() == pulses.size and desc["n_traps"].sum() == traps.size
@@ -68,7 +76,21 @@ def test_block_records():
     p = blocks[2].info["spec"]["pulses"][0]
     on = p.start + 0.5 * p.dur
     q = pulses[desc[2]["pulse0"]]
-    assert complex(q["amp_re"], q["amp_im"]) == complex(p(np.array([on]))[0])
+    assert complex(q["amp_re"], q["amp_im"]) == complex(p(np.array([on]))[0]) and q["n_table"] == 0
+
+
+def test_block_records_with_tabulated_pulses():
+    """RFPulse tables go into the table array in block and pulse order;
+    their records point at them."""
+    s1 = sequences.RFPulse(start=0.2, dur=1.0, flip=0.3, n_lobes=(1, 1), nb_samples=50)
+    s2 = sequences.RFPulse(start=2.1, dur=0.8, flip=0.5, shape="sinc", phase=0.4, nb_samples=80)
+    h = HardPulse(1.5, 0.2, 0.2)
+    blocks = [make_block(0.0, 1.5, pulses=[s1, h], dt_rf=0.05, dt=0.5), make_block(2.0, 1.0, pulses=[s2], dt=0.5)]
+    desc, pulses, traps, tab = sequences.block_descriptions(blocks)
+    assert pulses["n_table"].tolist() == [50, 0, 80] and pulses["table0"].tolist() == [0, 0, 50]
+    assert tab.shape == (130, 2)
+    np.testing.assert_array_equal(tab[:50, 0] + 1j * tab[:50, 1], s1(np.linspace(s1.start, s1.start + s1.dur, 50)))
+    np.testing.assert_array_equal(tab[50:, 0] + 1j * tab[50:, 1], s2(np.linspace(s2.start, s2.start + s2.dur, 80)))
 
 
 def test_marks_from_blocks():

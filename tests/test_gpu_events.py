@@ -2,7 +2,8 @@
 
 * fm_events_build's tables equal, bit for bit, the host rule's
   (sequences.make_block, restating python/feelmri/Bloch.py:105-129, :44-49,
-  :340): times, RF, gradients, dt, block offsets and chain marks.
+  :340): times, RF (hard pulses and tabulated sinc pulses read as numpy's
+  interp), gradients, dt, block offsets and chain marks.
 * A chain over device-built tables equals the chain over host tables and the
   per-block launches, bit for bit (static nodes, both precisions).
 * Moving nodes chain in one launch, equal bit for bit to one launch per
@@ -15,7 +16,7 @@ import pytest
 
 import motion_fixtures
 from feelmri_amd import sequences
-from feelmri_amd.sequences import HardPulse, Trapezoid, make_block
+from feelmri_amd.sequences import HardPulse, RFPulse, Trapezoid, make_block
 
 pytestmark = pytest.mark.gpu
 
@@ -28,6 +29,14 @@ def _mixed_blocks(seed=0):
          Trapezoid(2, t + 0.9, 3.3, 0.21, 0.1), Trapezoid(0, t + 2.05, -4.0, 0.11, 0.3)]
     out.append(make_block(t, 2.9, pulses=[HardPulse(t + 0.3, 0.3, 0.2, 0.4), HardPulse(t + 0.5, 0.35, 0.1)],
                           gradients=g, dt_rf=0.01, dt_gr=0.05, dt=0.1, store=True))
+    t = out[-1].info["extent"][1]
+    # tabulated pulses: an apodized sinc beside a slice gradient, a plain sinc
+    # with phase and frequency offsets, read between and on their knots
+    out.append(make_block(t, 1.6, pulses=[RFPulse(t + 0.2, 1.2, 0.4, n_lobes=(2, 2)),
+                                          RFPulse(t + 0.5, 0.6, 0.1, shape="sinc", n_lobes=(1, 3), phase=0.3,
+                                                  freq_offset=250.0, nb_samples=73)],
+                          gradients=[Trapezoid(2, t + 0.1, 6.0, 0.1, 1.2)], dt_rf=0.01, dt_gr=0.04, dt=0.2,
+                          store=True))
     t = out[-1].info["extent"][1]
     out.append(make_block(t, 0.0, gradients=[Trapezoid(1, t, float(rng.normal(0, 5)), 0.07, 0.2)], dt=0.03))
     out.append(sequences.delay_block(out[-1].info["extent"][1], 1.7, dt=0.45))
@@ -54,6 +63,7 @@ def test_device_tables_equal_host_tables(gpu):
     assert np.array_equal(rf, np.concatenate([b.rf for b in blocks]))
     assert np.array_equal(G, np.concatenate([b.G for b in blocks]))
     desc = sequences.block_descriptions(blocks)[0]
+    assert (sequences.block_descriptions(blocks)[1]["n_table"] > 0).sum() == 2
     codes = desc["spoil"] | ((desc["store_col"] + 1) << 1)
     assert np.array_equal(marks, device.marks_from_blocks(lens, codes))
     # rebuilt in place: the same tables

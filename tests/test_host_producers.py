@@ -45,7 +45,10 @@ def test_block_rf_area_is_the_flip_and_gradient_follows_trapezoid():
     # step i integrates over (t_{i-1}, t_i] with rf(t_i): the hard pulse's
     # B1 times its duration, so gamma * sum |rf| dt is the flip angle
     np.testing.assert_allclose(sequences.GAMMA * np.sum(np.abs(b.rf[1:]) * b.dt[1:]), flip, rtol=1e-12)
-    assert b.rf[0] == 0                                              # (start, end]: off at the start knot
+    # [start, end]: on at the start knot too (the reference's RF table is
+    # read with 0 outside the closed interval); the first event's value is
+    # never integrated
+    assert b.rf[0] == p.b1 and b.rf[-1] == p.b1
     g = sequences.Trapezoid(axis=2, start=1.0, amp=-7.0, ramp=0.2, flat=0.6)
     b2 = sequences.make_block(1.0, 1.0, gradients=[g], dt_gr=0.05, dt=1.0)
     area = np.sum(b2.G[1:, 2] * b2.dt[1:])                           # right-point rule on a piecewise-linear G
@@ -192,3 +195,64 @@ def test_spiral_stack_adc_grid_ends_and_planes(res, p, bw):
     np.testing.assert_allclose(kz[0, 0, :], (np.arange(n_sl) - n_sl // 2) / fov[2], rtol=1e-12)
     assert np.all(kz == kz[:1, :1, :])
 
+
+def test_hard_pulse_on_closed_interval_adds_the_preceding_step():
+    """A hard pulse that starts after its block's first event is on at its
+    start knot, so the step that ends there carries it too: gamma * sum
+    |rf| dt over the steps is the flip times (dur + that step) / dur."""
+    flip = np.deg2rad(20.0)
+    p = sequences.HardPulse(start=0.3, dur=0.2, flip=flip)
+    b = sequences.make_block(0.0, 1.0, pulses=[p], dt_rf=0.01, dt=0.25)
+    k = int(np.flatnonzero(np.abs(b.times - 0.3) < 1e-12)[0])
+    assert b.times[k - 1] == 0.25 and b.rf[k] == p.b1 and b.rf[k - 1] == 0
+    area = sequences.GAMMA * np.sum(np.abs(b.rf[1:]) * b.dt[1:])
+    np.testing.assert_allclose(area, flip * (0.2 + 0.05) / 0.2, rtol=1e-12)
+
+
+def test_rf_pulse_shapes():
+    """The reference's RF: gamma times the trapezoid area of the table is
+    the flip angle; the sinc's zero crossings sit on its ends (bw (n_l + 1)
+    lobes from the centre), the main lobe peaks at start + (n_l + 1) / bw;
+    the hard shape equals HardPulse; a phase offset rotates the table."""
+    flip = np.deg2rad(30.0)
+    for shape in ("apodized_sinc", "sinc", "hard"):
+        p = sequences.RFPulse(start=1.0, dur=2.0, flip=flip, shape=shape, n_lobes=(2, 3))
+        tt = np.linspace(1.0, 3.0, 200)
+        v = p(tt)
+        area = np.sum((v[1:] + v[:-1]) / 2) * (tt[1] - tt[0]) * sequences.GAMMA
+        np.testing.assert_allclose(abs(area), flip, rtol=1e-12)
+        assert p(np.array([0.999, 3.001])).tolist() == [0, 0]
+        if shape != "hard":
+            bw = 7 / 2.0
+            np.testing.assert_allclose(abs(v[0]), 0.0, atol=1e-12 * abs(v).max())
+            np.testing.assert_allclose(abs(p(np.array([3.0]))[0]), 0.0, atol=1e-12 * abs(v).max())
+            peak = 1.0 + 3 / bw
+            fine = np.linspace(1.0, 3.0, 20001)
+            assert abs(fine[np.argmax(np.abs(p(fine)))] - peak) < 2.0 / 199
+    hard = sequences.RFPulse(start=1.0, dur=2.0, flip=flip, shape="hard", phase=0.3)
+    hp = sequences.HardPulse(start=1.0, dur=2.0, flip=flip, phase=0.3)
+    t = np.array([0.5, 1.0, 1.7, 3.0, 3.5])
+    np.testing.assert_allclose(hard(t), hp(t), rtol=1e-15, atol=0)
+    s0 = sequences.RFPulse(start=0.0, dur=1.0, flip=flip)
+    s1 = sequences.RFPulse(start=0.0, dur=1.0, flip=flip, phase=0.7)
+    np.testing.assert_allclose(s1(t), s0(t) * np.exp(0.7j), rtol=1e-12, atol=1e-15)
+
+
+
+def test_sinc_pulse_excitation_through_the_oracle():
+    """An on-resonance node (no gradient, no off-resonance, no relaxation)
+    under a real sinc pulse rotates about one axis: Mz = cos(gamma sum rf dt)
+    exactly over the block's steps, and that sum is the nominal flip to the
+    event grid's quadrature error."""
+    import oracle
+
+    flip = np.deg2rad(40.0)
+    p = sequences.RFPulse(start=0.0, dur=2.0, flip=flip, n_lobes=(2, 2))
+    b = sequences.make_block(0.0, 2.0, pulses=[p], dt_rf=0.005, dt=1.0)
+    ang = sequences.GAMMA * np.sum(b.rf[1:].real * b.dt[1:])
+    np.testing.assert_allclose(ang, flip, rtol=1e-3)
+    r0 = np.zeros((1, 3))
+    mxy, mz = oracle.solve_mri(r0, np.array([1e12]), np.array([1e12]), np.zeros(1), 1.0, sequences.GAMMA, b.rf, b.G,
+                               b.dt, None, np.zeros(1, complex), np.ones(1), None)
+    np.testing.assert_allclose(mz[0, -1], np.cos(ang), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(abs(mxy[0, -1]), np.sin(ang), rtol=0, atol=1e-12)
