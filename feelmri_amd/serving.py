@@ -19,7 +19,10 @@ its trajectory in FP64 or mixed precision.  The loop
   nodes, the batch padded up to the class);
 * keeps trajectory plans resident across requests and each admitted
   request's packed nodes and output resident until it completes;
-* cuts each request's signal into chunks of readout rows and interleaves
+* cuts each request's signal into chunks -- readout rows of a Cartesian
+  trajectory, whole spokes of a radial one (written to a staging buffer and
+  copied into place), so that the spokes keep the linear structure the
+  per-line GEMM uses -- and interleaves
   the chunks of all active requests over their own HIP streams, so a
   request that arrives while others run starts at the next chunk boundary
   and its work overlaps theirs on the device;
@@ -219,7 +222,7 @@ class _State:
     req: ScanRequest
     stream: object
     plans: list
-    rows: list                    # (i0, i1) readout rows per chunk
+    rows: list                    # per chunk: (0, i0, i1) readout rows or (1, j0, j1) whole lines
     out: object = None
     ws: object = None
     nodes: object = None
@@ -227,6 +230,7 @@ class _State:
     bloch: object = None
     w: object = None
     mxy: object = None            # the request's steady-state Mxy (M0 applied), (N, 1) complex128
+    stage: object = None          # line-chunk output staging (radial requests)
     next_chunk: int = 0
     events: list = field(default_factory=list)
     host: object = None
@@ -358,7 +362,8 @@ class GpuBackend:
     """Runs requests on the device; the node pool, trajectory plans and the
     steady-state graphs stay resident."""
 
-    def __init__(self, dev=None, chunk_rows: int = 32, n_dummy: int = 8, n_max: int = 2_000_000):
+    def __init__(self, dev=None, chunk_rows: int = 32, n_dummy: int = 8, n_max: int = 2_000_000,
+                 chunk_lines: int = 64):
         import torch
 
         from . import device
@@ -368,6 +373,7 @@ class GpuBackend:
         self._pool = None
         self.plans = {}
         self.chunk_rows = chunk_rows
+        self.chunk_lines = chunk_lines
         self.n_dummy = n_dummy
         self.streams = []
         self._graphs = None
@@ -393,15 +399,29 @@ class GpuBackend:
         key = (req.traj, req.shape)
         if key not in self.plans:
             kloc, t = request_trajectory(req)
-            n = req.shape[0]
+            n, nl = req.shape[0], req.shape[1]
             rows, plans = [], []
-            for i0 in range(0, n, self.chunk_rows):
-                i1 = min(n, i0 + self.chunk_rows)
-                sub = tuple(np.ascontiguousarray(k[i0:i1]) for k in kloc)
-                plans.append(device.SignalPlan(sub, np.ascontiguousarray(t[i0:i1]), device=self.dev))
-                rows.append((i0, i1))
+            if req.traj == "radial":
+                # whole spokes per chunk: the lines stay linear over all their
+                # samples, which the per-line GEMM of the mixed form needs
+                for j0 in range(0, nl, self.chunk_lines):
+                    j1 = min(nl, j0 + self.chunk_lines)
+                    sub = tuple(np.ascontiguousarray(k[:, j0:j1]) for k in kloc)
+                    plans.append(device.SignalPlan(sub, np.ascontiguousarray(t[:, j0:j1]), device=self.dev))
+                    rows.append((1, j0, j1))
+            else:
+                for i0 in range(0, n, self.chunk_rows):
+                    i1 = min(n, i0 + self.chunk_rows)
+                    sub = tuple(np.ascontiguousarray(k[i0:i1]) for k in kloc)
+                    plans.append(device.SignalPlan(sub, np.ascontiguousarray(t[i0:i1]), device=self.dev))
+                    rows.append((0, i0, i1))
             self.plans[key] = (plans, rows)
         return self.plans[key]
+
+    def _line_buffer_samples(self, req: ScanRequest) -> int:
+        """Samples of the staging buffer a line chunk is written to (0 for
+        readout-row chunks, which write their rows of the output in place)."""
+        return req.shape[0] * min(req.shape[1], self.chunk_lines) * req.shape[2] if req.traj == "radial" else 0
 
     def steady_blocks(self):
         from . import sequences
@@ -440,8 +460,9 @@ class GpuBackend:
         N, S = req.n_nodes, req.n_samples
         plans, _ = self._chunks(req)
         ws = max(p.workspace_bytes(req.precision, N, 1) for p in plans)
-        # tissue (6 rows), Bloch table (6), signal table (7), weights, output
-        return N * 8 * (6 + 6 + 7 + 1) + ws + S * 16
+        # tissue (6 rows), Bloch table (6), signal table (7), weights, output,
+        # line-chunk staging
+        return N * 8 * (6 + 6 + 7 + 1) + ws + S * 16 + self._line_buffer_samples(req) * 16
 
     def start(self, req: ScanRequest, slot: int):
         return self.start_batch([(req, slot)])[0]
@@ -492,6 +513,8 @@ class GpuBackend:
                 st.nodes = device.SignalNodes.from_arrays(pool.nodes[:N], st.tissue["phi"], st.tissue["T2s"],
                                                           st.mxy, weights=st.w, device=self.dev, stream=s)
                 st.out = torch.empty((req.n_samples, 1), dtype=torch.complex128, device=self.dev)
+                nb = self._line_buffer_samples(req)
+                st.stage = torch.empty((nb, 1), dtype=torch.complex128, device=self.dev) if nb else None
                 need = max(p.workspace_bytes(req.precision, N, 1) for p in st.plans)
                 st.ws = torch.empty(max(need, 1), dtype=torch.uint8, device=self.dev)
             # tensors made on the batch stream and used on this one
@@ -506,12 +529,18 @@ class GpuBackend:
         pending = [e for e in st.events if not e.query()]
         if st.next_chunk >= len(st.plans) or len(pending) >= ahead:
             return False
-        i0, i1 = st.rows[st.next_chunk]
-        per_row = st.req.n_samples // st.req.shape[0]
+        axis, a0, a1 = st.rows[st.next_chunk]
         plan = st.plans[st.next_chunk]
         with torch.cuda.stream(st.stream):
-            view = st.out[i0 * per_row:i1 * per_row]
-            plan.execute(st.nodes, precision=st.req.precision, out=view, workspace=st.ws, stream=st.stream)
+            if axis == 0:
+                per_row = st.req.n_samples // st.req.shape[0]
+                view = st.out[a0 * per_row:a1 * per_row]
+                plan.execute(st.nodes, precision=st.req.precision, out=view, workspace=st.ws, stream=st.stream)
+            else:
+                n, nl, nk = st.req.shape
+                stage = st.stage[:n * (a1 - a0) * nk]
+                plan.execute(st.nodes, precision=st.req.precision, out=stage, workspace=st.ws, stream=st.stream)
+                st.out.view(n, nl, nk)[:, a0:a1].copy_(stage.view(n, a1 - a0, nk))
             ev = torch.cuda.Event()
             ev.record(st.stream)
         st.events.append(ev)
@@ -529,5 +558,5 @@ class GpuBackend:
 
     def finish(self, st: _State):
         res = st.host.numpy().reshape(*st.req.shape, 1).copy()
-        st.nodes = st.out = st.ws = st.host = st.w = st.mxy = None
+        st.nodes = st.out = st.ws = st.host = st.w = st.mxy = st.stage = None
         return res
