@@ -20,7 +20,6 @@ it.
 """
 from __future__ import annotations
 
-import struct
 import zlib
 
 import numpy as np
@@ -58,9 +57,12 @@ class File:
     def __init__(self, path):
         with open(path, "rb") as f:
             data = f.read()
-        base = data.find(SIGNATURE)
-        if base < 0 or base % 512 != 0 and base != 0:
-            raise ValueError(f"{path}: no HDF5 signature")
+        # the superblock sits at 0 or after a user block of 512 * 2^k bytes
+        base, k = 0, 512
+        while data[base:base + 8] != SIGNATURE:
+            base, k = k, 2 * k
+            if base >= len(data):
+                raise ValueError(f"{path}: no HDF5 signature")
         self.path = str(path)
         self._d = data
         self._base = base
