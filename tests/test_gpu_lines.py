@@ -27,6 +27,8 @@ pytestmark = pytest.mark.gpu
     (1024, 5, 3333, 3),     # Q 32, P 32
     (1536, 4, 2000, 1),     # Q 32, P 48
     (2048, 3, 2600, 1),     # Q 32, P 64
+    (256, 5, 1, 1),         # one node: one split, one build of mostly empty lanes
+    (300, 4, 33, 2),        # two builds, the second with one node
 ])
 def test_line_gemm_radial(gpu, n_meas, n_spokes, n, v):
     from feelmri_amd import device
