@@ -49,3 +49,33 @@ def test_xdmf_phantom_weights_and_signal(gpu, tmp_path):
         err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
         print(prec, f"{err:.2e}")
         assert err <= tol, (prec, err)
+
+
+def test_fem_phantom_mass_matrices(gpu, tmp_path):
+    """FEMPhantom.mass_matrix on the GPU: the consistent matrix of the local
+    part against the oracle's element matrices, its row sums equal to the
+    lumped weights, and the lumped weights of a 3-way partition summing to
+    the whole mesh's (partial weights on shared nodes)."""
+    import oracle
+    from feelmri_amd.Phantom import FEMPhantom, lumped_weights
+
+    m = phantoms.box_mesh((-0.02, -0.02, -0.01), (0.02, 0.02, 0.01), (12, 11, 6))
+    w = H5Writer()
+    w.create_dataset("/g", m.nodes)
+    w.create_dataset("/t", m.elems.astype(np.int64))
+    w.save(tmp_path / "m.h5")
+    n, e = m.n_nodes, len(m.elems)
+    (tmp_path / "m.xdmf").write_text(f"""<Xdmf Version="3.0"><Domain><Grid Name="mesh">
+<Geometry GeometryType="XYZ"><DataItem Dimensions="{n} 3" Format="HDF" Precision="8">m.h5:/g</DataItem></Geometry>
+<Topology TopologyType="Tetrahedron" NumberOfElements="{e}"><DataItem DataType="Int" Dimensions="{e} 4" Format="HDF" Precision="8">m.h5:/t</DataItem></Topology>
+</Grid></Domain></Xdmf>""")
+    want = oracle.mass_p1_lumped(m.elems, m.nodes)
+    ph = FEMPhantom(str(tmp_path / "m.xdmf"), dtype=np.float64)
+    M = ph.mass_matrix()
+    np.testing.assert_allclose(np.asarray(M.sum(axis=1)).ravel(), want, rtol=1e-12)
+    np.testing.assert_allclose(lumped_weights(ph.mass_matrix(lumped=True)), want, rtol=1e-12)
+    acc = np.zeros(n)
+    for part in range(3):
+        p = FEMPhantom(str(tmp_path / "m.xdmf"), dtype=np.float64, n_parts=3, part=part)
+        acc[p.local_to_global_nodes] += lumped_weights(p.mass_matrix(lumped=True))
+    np.testing.assert_allclose(acc, want, rtol=1e-12)

@@ -201,3 +201,43 @@ def test_phantom_from_xdmf_weights_and_partition(tmp_path):
     for p in parts:
         acc[p.nodes_l2g] += oracle.mass_p1_lumped(p.elems, ph.nodes[p.nodes_l2g])
     np.testing.assert_allclose(acc, want, rtol=1e-13, atol=0)
+
+
+def test_fem_phantom_reads_scales_and_partitions(tmp_path):
+    """FEMPhantom (Phantom.py:49-422) on an XDMF time series: nodes scaled
+    and cast, frames read with the displacement/velocity scaled by the scale
+    factor, parts of a 3-way partition that cover every element once, a
+    submesh and the data carried to it, orient/reorient round trip."""
+    from feelmri_amd.Phantom import FEMPhantom
+
+    m, _, _, _ = _mesh()
+    rng = np.random.default_rng(4)
+    times = [0.0, 0.1]
+    vel = [rng.normal(size=(m.n_nodes, 3)) for _ in times]
+    w, xml = _series_xdmf(m, vel, times, "meshio", "ph.h5")
+    w.save(tmp_path / "ph.h5")
+    (tmp_path / "ph.xdmf").write_text(xml)
+    ph = FEMPhantom(str(tmp_path / "ph.xdmf"), scale_factor=0.01, dtype=np.float64)
+    assert ph.Nfr == 2 and ph.cell_type == "tetra"
+    np.testing.assert_array_equal(ph.global_nodes, m.nodes * 0.01)
+    np.testing.assert_array_equal(ph.local_nodes, ph.global_nodes)       # one part: the whole mesh
+    ph.read_data(1)
+    np.testing.assert_allclose(ph.point_data["velocity"], vel[1] * 0.01, rtol=1e-15)
+    seen = np.zeros(len(m.elems), dtype=int)
+    for part in range(3):
+        p = FEMPhantom(str(tmp_path / "ph.xdmf"), dtype=np.float64, n_parts=3, part=part)
+        seen[p.local_to_global_elems] += 1
+        np.testing.assert_array_equal(p.global_nodes[p.local_to_global_nodes][p.local_elements],
+                                      p.global_nodes[p.global_elements[p.local_to_global_elems]])
+    assert np.all(seen == 1)
+    # submesh of the elements left of x = 0, node data carried to it
+    cx = ph.global_nodes[ph.global_elements].mean(axis=1)[:, 0]
+    full = ph.global_nodes.copy()
+    ph.create_submesh(cx < 0)
+    assert ph.global_elements.max() == ph.global_nodes.shape[0] - 1
+    np.testing.assert_array_equal(ph.to_submesh(full, global_mesh=True), ph.global_nodes)
+    R = np.array([[0.0, 1.0, 0.0], [-1.0, 0.0, 0.0], [0.0, 0.0, 1.0]])
+    before = ph.global_nodes.copy()
+    ph.orient(R, np.array([0.001, -0.002, 0.0]))
+    ph.reorient(R, np.array([0.001, -0.002, 0.0]))
+    np.testing.assert_allclose(ph.global_nodes, before, atol=1e-15)
