@@ -52,10 +52,10 @@ def test_xdmf_phantom_weights_and_signal(gpu, tmp_path):
 
 
 def test_fem_phantom_mass_matrices(gpu, tmp_path):
-    """FEMPhantom.mass_matrix on the GPU: the consistent matrix of the local
-    part against the oracle's element matrices, its row sums equal to the
-    lumped weights, and the lumped weights of a 3-way partition summing to
-    the whole mesh's (partial weights on shared nodes)."""
+    """FEMPhantom.mass_matrix on the GPU: the consistent matrix's row sums
+    and the lumped matrix's diagonal equal the oracle's lumped weights, and
+    the lumped weights of a 3-way partition sum to the whole mesh's (partial
+    weights on shared nodes)."""
     import oracle
     from feelmri_amd.Phantom import FEMPhantom, lumped_weights
 
