@@ -78,6 +78,31 @@ def mass_case():
                 element_matrices=oracle.mass_p1_element_matrices(m.elems, m.nodes))
 
 
+def sinc_block():
+    """Slice-selective excitation: an apodized sinc on a slice gradient, then
+    the refocusing lobe (half the slice gradient's area, negative)."""
+    rf = sequences.RFPulse(0.1, 2.0, np.deg2rad(60), n_lobes=(2, 2), phase=0.2)
+    gs = sequences.Trapezoid(2, 0.0, 6.0, 0.1, 2.0)
+    gr = sequences.Trapezoid(2, 2.2, -6.0, 0.1, 0.95)
+    return sequences.make_block(0.0, 3.5, pulses=[rf], gradients=[gs, gr], dt_rf=0.02, dt_gr=0.05, dt=0.5)
+
+
+def bloch_sinc_case():
+    rng = np.random.default_rng(8)
+    n = 200
+    blk = sinc_block()
+    r0 = np.zeros((n, 3))
+    r0[:, 2] = np.linspace(-0.01, 0.01, n)
+    r0[:, :2] = rng.uniform(-0.02, 0.02, (n, 2))
+    T1, T2 = rng.uniform(500, 1500, n), rng.uniform(40, 100, n)
+    dB = rng.normal(0, 5e-5, n)
+    mxy0 = np.zeros(n, complex)
+    mz0 = np.ones(n)
+    mxy, mz = oracle.solve_mri(r0, T1, T2, dB, 1.0, sequences.GAMMA, blk.rf, blk.G, blk.dt, None, mxy0, mz0, None)
+    return dict(r0=r0, T1=T1, T2=T2, dB=dB, mxy0=mxy0, mz0=mz0, rf=blk.rf, G=blk.G, dt=blk.dt,
+                gamma=np.float64(sequences.GAMMA), mxy=mxy, mz=mz)
+
+
 def pod_case():
     rng = np.random.default_rng(11)
     modes = rng.normal(size=(500, 3, 6))
@@ -88,6 +113,7 @@ def pod_case():
 def main():
     cases = dict(signal_cases())
     cases["bloch_hard_pulse_gradients"] = bloch_case()
+    cases["bloch_sinc_slice"] = bloch_sinc_case()
     cases["mass_p1_ellipsoid"] = mass_case()
     cases["pod_tensordot"] = pod_case()
     for name, arrays in cases.items():

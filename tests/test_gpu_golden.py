@@ -54,6 +54,20 @@ def test_bloch_matches_golden(gpu):
     assert rel(mxy, g["mxy"]) <= 1e-6 and rel(mz, g["mz"]) <= 1e-6
 
 
+def test_bloch_sinc_slice_matches_golden(gpu):
+    """The slice-selective sinc excitation on the GPU: FP64 form (complex64 /
+    float32 history) and the FP32 form against the fixture."""
+    from feelmri_amd import BlochSimulator
+
+    g = load("bloch_sinc_slice")
+    args = (g["r0"], g["T1"], g["T2"], g["dB"], 1.0, float(g["gamma"]), g["rf"], g["G"], g["dt"], None, g["mxy0"],
+            g["mz0"], None)
+    mxy, mz = BlochSimulator.solve_mri(*args, precision="f64")
+    assert rel(mxy, g["mxy"]) <= 1e-6 and rel(mz, g["mz"]) <= 1e-6
+    mxy, mz = BlochSimulator.solve_mri(*args, precision="f32")
+    assert rel(mxy, g["mxy"]) <= 1e-4 and rel(mz, g["mz"]) <= 1e-4
+
+
 def test_mass_and_pod_match_golden(gpu):
     import torch
 
