@@ -323,6 +323,46 @@ __global__ __launch_bounds__(64 * W, (CB <= 3 ? 3 : 2)) void k_cart_moving_bf16(
     }
 }
 
+// The instantiation a shape launches: a standard stack whose column tiles
+// are all full takes the STD form.
+static bool moving_std(const MovingShape &s, const CartesianTables &c)
+{
+    return c.axis_aligned && s.KV == s.n_ct * 32 * s.CB;
+}
+
+static size_t moving_dyn_lds(const MovingShape &s) { return 2 * (size_t)s.ez_rows * kEzStride * sizeof(float2); }
+
+template <int W, int CB>
+static int moving_blocks_wc(bool std_stack, size_t dyn)
+{
+    return std_stack ? blocks_per_cu(k_cart_moving_bf16<W, CB, true>, 64 * W, dyn)
+                     : blocks_per_cu(k_cart_moving_bf16<W, CB, false>, 64 * W, dyn);
+}
+
+template <int W>
+static int moving_blocks_w(int CB, bool std_stack, size_t dyn)
+{
+    switch (CB) {
+    case 1: return moving_blocks_wc<W, 1>(std_stack, dyn);
+    case 2: return moving_blocks_wc<W, 2>(std_stack, dyn);
+    case 3: return moving_blocks_wc<W, 3>(std_stack, dyn);
+    default: return moving_blocks_wc<W, 4>(std_stack, dyn);
+    }
+}
+
+// Workgroups of the shape's instantiation that one CU holds at once.
+static int moving_blocks_per_cu(const MovingShape &s, const CartesianTables &c)
+{
+    const bool st = moving_std(s, c);
+    const size_t dyn = moving_dyn_lds(s);
+    switch (s.W) {
+    case 1: return moving_blocks_w<1>(s.CB, st, dyn);
+    case 2: return moving_blocks_w<2>(s.CB, st, dyn);
+    case 3: return moving_blocks_w<3>(s.CB, st, dyn);
+    default: return moving_blocks_w<4>(s.CB, st, dyn);
+    }
+}
+
 MovingShape moving_shape(const CartesianTables &c, int64_t N, int V)
 {
     MovingShape s{};
@@ -338,14 +378,16 @@ MovingShape moving_shape(const CartesianTables &c, int64_t N, int V)
     s.n_ct = cdiv(ncb, s.CB);
     const int64_t chunk = 64 * s.W;
     const int64_t base = s.M * s.n_rt * s.n_ct;
-    // two rounds of workgroups at least, and at most ~16 384 nodes per split
-    // (FP32 accumulation)
-    int64_t splits = std::max(cdiv(4 * (int64_t)device_cu_count(), base), cdiv(std::max<int64_t>(N, 1), 16384));
-    splits = std::max<int64_t>(1, std::min(splits, cdiv(std::max<int64_t>(N, 1), chunk)));
-    s.nodes_per_split = cdiv(cdiv(std::max<int64_t>(N, 1), splits), chunk) * chunk;
-    s.n_splits = cdiv(std::max<int64_t>(N, 1), s.nodes_per_split);
     const int64_t cols = 32 * (int64_t)s.CB;
     s.ez_rows = (int)std::min<int64_t>(cols, (cols - 1) / V + 2);
+    // at most ~16 384 nodes per split (FP32 accumulation), four workgroups
+    // per CU at least, and whole rounds of the workgroups the device holds
+    const int64_t s_min =
+        std::max(cdiv(4 * (int64_t)device_cu_count(), base), cdiv(std::max<int64_t>(N, 1), 16384));
+    const int64_t capacity = (int64_t)moving_blocks_per_cu(s, c) * device_cu_count();
+    const int64_t splits = balanced_splits(N, chunk, base, s_min, capacity);
+    s.nodes_per_split = cdiv(cdiv(std::max<int64_t>(N, 1), splits), chunk) * chunk;
+    s.n_splits = cdiv(std::max<int64_t>(N, 1), s.nodes_per_split);
     s.part_bytes = (size_t)s.n_splits * (size_t)s.M * (size_t)s.L * (size_t)s.KV * sizeof(float2);
     return s;
 }
@@ -355,8 +397,8 @@ static void launch_moving_wc(const MovingShape &s, const CartesianTables &c, con
                              const float *modes, int R, const double *mcoef, float2 *part, hipStream_t st)
 {
     const int64_t grid = s.n_splits * s.M * s.n_rt * s.n_ct;
-    const size_t dyn = 2 * (size_t)s.ez_rows * kEzStride * sizeof(float2);
-    const bool std_stack = c.axis_aligned && s.KV == s.n_ct * 32 * CB;
+    const size_t dyn = moving_dyn_lds(s);
+    const bool std_stack = moving_std(s, c);
 #define FM_MV(STDV)                                                                                           \
     k_cart_moving_bf16<W, CB, STDV><<<(unsigned)grid, 64 * W, dyn, st>>>(                                    \
         packed, N, (int)s.V, modes, R, mcoef, c.d_ro_group, c.d_F, c.d_G, c.d_H, c.d_t, s.M, s.L, s.K, s.n_rt, \

@@ -14,9 +14,11 @@
 //     (FP64 phase, range reduction, sincospi, exp);
 //   * per 16-node step (4 k-steps), wave w builds its A fragments (16 lines
 //     of row block W rt + w) in registers, one FP64 sincospi per (line, node);
-//     the kz factors exp(-2 pi i H_k . x) of the step's nodes go to LDS one
-//     step ahead and the workgroup forms the B fragments (16 (k,v) columns
-//     per column block, FP64 products) in one LDS buffer;
+//     the kz factors exp(-2 pi i H_k . x) of the step's nodes are in LDS
+//     (one buffer, written during the previous step's products) and the
+//     workgroup forms the B fragments (16 (k,v) columns per column block,
+//     FP64 products) in one LDS buffer.  Single buffers keep a workgroup
+//     under a third of the CU's LDS, so three share a CU;
 //   * per k-step and column block, 4 MFMAs: Re += Ar Br - Ai Bi,
 //     Im += Ar Bi + Ai Br.
 // FP64 accumulators need no bound on the split length; node splits exist for
@@ -60,7 +62,7 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_f64(
     __shared__ double sB[kFrag * 2 * 64];        // (fragment, Re/Im plane, lane)
     __shared__ double sH[3 * (kCols + 1)];
     __shared__ double scoef[kMaxModes + 3];
-    extern __shared__ double2 sEz64[];           // 2 buffers of ez_rows rows of kEz64Stride
+    extern __shared__ double2 sEz64[];           // ez_rows rows of kEz64Stride
 
     // workgroup -> (split, readout index, row tile, column tile); the ids of
     // one split are contiguous and dealt onto one XCD
@@ -117,12 +119,9 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_f64(
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc_re[cb][q] = acc_im[cb][q] = 0.0;
 
-    const int ez_buf = ez_rows * kEz64Stride;
-    int pz = 0;
-
-    // kz factors of step s's 16 nodes into buffer `buf`
-    auto ez_gen = [&](int s, int buf) {
-        double2 *dst = sEz64 + buf * ez_buf;
+    // kz factors of step s's 16 nodes
+    auto ez_gen = [&](int s) {
+        double2 *dst = sEz64;
         for (int e = tid; e < nk * 16; e += kThreads) {
             const int kk = e / 16, m = s * 16 + e % 16;
             const double u = fma(sH[3 * kk], sx[m], fma(sH[3 * kk + 1], sy[m], sH[3 * kk + 2] * sz[m]));
@@ -175,13 +174,12 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_f64(
         }
         __syncthreads();
         const int nsteps = (int)std::min<int64_t>(kSteps, (n_end - base + 15) / 16);
-        ez_gen(0, pz);
+        ez_gen(0);
         __syncthreads();
         for (int s = 0; s < nsteps; ++s) {
-            if (s + 1 < nsteps) ez_gen(s + 1, pz ^ 1);
             // B fragments of step s: node 16 s + 4 st + lane/16, column c
             {
-                const double2 *ez = sEz64 + pz * ez_buf;
+                const double2 *ez = sEz64;
 #pragma unroll
                 for (int r = 0; r < kBF; ++r) {
                     const int f = (tid + r * kThreads) / 64;
@@ -205,7 +203,8 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_f64(
                 const double u = fma(Gx, sx[m], fma(Gy, sy[m], Gz * sz[m]));
                 cis_neg_f64(u, ar[st], ai[st]);
             }
-            __syncthreads();
+            __syncthreads();  // B planes written, kz factors of step s read
+            if (s + 1 < nsteps) ez_gen(s + 1);
 #pragma unroll
             for (int st = 0; st < 4; ++st) {
 #pragma unroll
@@ -223,8 +222,7 @@ __global__ __launch_bounds__(64 * W, 2) void k_cart_moving_f64(
                     acc_im[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[st], br, acc_im[cb], 0, 0, 0);
                 }
             }
-            pz ^= 1;
-            __syncthreads();  // B planes read before the next step rewrites them
+            __syncthreads();  // B planes read and the next kz factors written
         }
     }
 
@@ -252,12 +250,38 @@ static int pick_blocks(int64_t n, std::initializer_list<int> opts, int cap)
     return 1;
 }
 
-// LDS bytes of one workgroup (static arrays + the two kz-factor buffers).
+// LDS bytes of one workgroup (static arrays + the kz-factor buffer).
 static size_t lds_bytes_f64(int W, int CB, int ez_rows)
 {
     const size_t chunk = 64 * (size_t)W;
     return 3 * chunk * 8 + kMovMaxV * (chunk + 1) * 16 + (size_t)CB * 4 * 2 * 64 * 8 +
-           3 * (16 * (size_t)CB + 1) * 8 + (kMaxModes + 3) * 8 + 2 * (size_t)ez_rows * kEz64Stride * 16;
+           3 * (16 * (size_t)CB + 1) * 8 + (kMaxModes + 3) * 8 + (size_t)ez_rows * kEz64Stride * 16;
+}
+
+template <int W>
+static int mf64_blocks_w(int CB, size_t dyn)
+{
+    switch (CB) {
+    case 1: return blocks_per_cu(k_cart_moving_f64<W, 1>, 64 * W, dyn);
+    case 2: return blocks_per_cu(k_cart_moving_f64<W, 2>, 64 * W, dyn);
+    case 3: if constexpr (W >= 2) return blocks_per_cu(k_cart_moving_f64<W, 3>, 64 * W, dyn); break;
+    case 4: if constexpr (W >= 2) return blocks_per_cu(k_cart_moving_f64<W, 4>, 64 * W, dyn); break;
+    case 6: if constexpr (W >= 3) return blocks_per_cu(k_cart_moving_f64<W, 6>, 64 * W, dyn); break;
+    default: break;
+    }
+    return 1;
+}
+
+// Workgroups of the shape's instantiation that one CU holds at once.
+static int mf64_blocks_per_cu(int W, int CB, int ez_rows)
+{
+    const size_t dyn = (size_t)ez_rows * kEz64Stride * sizeof(double2);
+    switch (W) {
+    case 1: return mf64_blocks_w<1>(CB, dyn);
+    case 2: return mf64_blocks_w<2>(CB, dyn);
+    case 3: return mf64_blocks_w<3>(CB, dyn);
+    default: return mf64_blocks_w<4>(CB, dyn);
+    }
 }
 
 MovingShape moving_shape_f64(const CartesianTables &c, int64_t N, int V)
@@ -286,12 +310,13 @@ MovingShape moving_shape_f64(const CartesianTables &c, int64_t N, int V)
     s.n_ct = cdiv(ncb, s.CB);
     const int64_t chunk = 64 * s.W;
     const int64_t base = s.M * s.n_rt * s.n_ct;
-    // four rounds of workgroups for balance; FP64 accumulators need no bound
-    int64_t splits = cdiv(4 * (int64_t)device_cu_count(), base);
-    splits = std::max<int64_t>(1, std::min(splits, cdiv(std::max<int64_t>(N, 1), chunk)));
+    s.ez_rows = ez_rows_of(s.CB);
+    // four workgroups per CU at least, in whole rounds of the workgroups the
+    // device holds; FP64 accumulators need no bound on the split length
+    const int64_t capacity = (int64_t)mf64_blocks_per_cu(s.W, s.CB, s.ez_rows) * device_cu_count();
+    const int64_t splits = balanced_splits(N, chunk, base, cdiv(4 * (int64_t)device_cu_count(), base), capacity);
     s.nodes_per_split = cdiv(cdiv(std::max<int64_t>(N, 1), splits), chunk) * chunk;
     s.n_splits = cdiv(std::max<int64_t>(N, 1), s.nodes_per_split);
-    s.ez_rows = ez_rows_of(s.CB);
     s.part_bytes = (size_t)s.n_splits * (size_t)s.M * (size_t)s.L * (size_t)s.KV * sizeof(double2);
     return s;
 }
@@ -301,7 +326,7 @@ static void launch_mf64_wc(const MovingShape &s, const CartesianTables &c, const
                            const float *modes, int R, const double *mcoef, double2 *part, hipStream_t st)
 {
     const int64_t grid = s.n_splits * s.M * s.n_rt * s.n_ct;
-    const size_t dyn = 2 * (size_t)s.ez_rows * kEz64Stride * sizeof(double2);
+    const size_t dyn = (size_t)s.ez_rows * kEz64Stride * sizeof(double2);
     k_cart_moving_f64<W, CB><<<(unsigned)grid, 64 * W, dyn, st>>>(
         packed, N, (int)s.V, modes, R, mcoef, c.d_ro_group, c.d_F, c.d_G, c.d_H, c.d_t, s.M, s.L, s.K, s.n_rt,
         s.n_ct, s.n_splits, s.nodes_per_split, s.ez_rows, part);

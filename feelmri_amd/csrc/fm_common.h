@@ -62,6 +62,24 @@ constexpr double kTwoPi = 6.283185307179586476925286766559;
 // Number of CUs on the current device (cached per device).
 int device_cu_count();
 
+// Workgroups of `kernel` (a __global__ function) that one CU holds at once
+// with `threads` threads and `dyn` bytes of dynamic LDS; 1 when the runtime
+// cannot tell.
+template <typename F>
+int blocks_per_cu(F kernel, int threads, size_t dyn)
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, dyn) != hipSuccess || n < 1) n = 1;
+    return n;
+}
+
+// Node splits of a grid of `base` workgroups per split, each split a whole
+// number of `chunk`-node chunks: the count in [s_min, 2 s_min] that
+// minimises (rounds of `capacity` concurrent workgroups) x (nodes per split),
+// the fewest on a tie.  Workgroups of one grid take the same time, so a last
+// round that is only partly filled leaves the rest of the device idle.
+int64_t balanced_splits(int64_t N, int64_t chunk, int64_t base, int64_t s_min, int64_t capacity);
+
 // Zero n doubles on the stream with a kernel (not a memset node, so the
 // call can be captured into a HIP graph and replayed).
 int zero_f64_async(double *x, int64_t n, hipStream_t st);

@@ -1,4 +1,5 @@
 // Error reporting and device queries shared by every entry point.
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -16,6 +17,24 @@ void set_error(const char *fmt, ...)
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     g_last_error = buf;
+}
+
+int64_t balanced_splits(int64_t N, int64_t chunk, int64_t base, int64_t s_min, int64_t capacity)
+{
+    const int64_t n = std::max<int64_t>(N, 1);
+    const int64_t s_top = cdiv(n, chunk);  // one chunk per split at most
+    s_min = std::max<int64_t>(1, std::min(s_min, s_top));
+    capacity = std::max<int64_t>(capacity, 1);
+    int64_t best = s_min, best_cost = -1;
+    for (int64_t s = s_min; s <= std::min(2 * s_min, s_top); ++s) {
+        const int64_t nps = cdiv(cdiv(n, s), chunk) * chunk;
+        const int64_t cost = cdiv(base * cdiv(n, nps), capacity) * nps;
+        if (best_cost < 0 || cost < best_cost) {
+            best = s;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 int device_cu_count()
