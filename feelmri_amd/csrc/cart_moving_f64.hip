@@ -17,8 +17,8 @@
 //     the kz factors exp(-2 pi i H_k . x) of the step's nodes are in LDS
 //     (one buffer, written during the previous step's products) and the
 //     workgroup forms the B fragments (16 (k,v) columns per column block,
-//     FP64 products) in one LDS buffer.  Single buffers keep a workgroup
-//     under a third of the CU's LDS, so three share a CU;
+//     FP64 products) in one LDS buffer.  The kz factors of the next step
+//     are formed between the step's barrier and its MFMAs;
 //   * per k-step and column block, 4 MFMAs: Re += Ar Br - Ai Bi,
 //     Im += Ar Bi + Ai Br.
 // FP64 accumulators need no bound on the split length; node splits exist for
