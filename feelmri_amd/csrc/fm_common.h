@@ -74,11 +74,13 @@ int blocks_per_cu(F kernel, int threads, size_t dyn)
 }
 
 // Node splits of a grid of `base` workgroups per split, each split a whole
-// number of `chunk`-node chunks: the count in [s_min, 2 s_min] that
-// minimises (rounds of `capacity` concurrent workgroups) x (nodes per split),
-// the fewest on a tie.  Workgroups of one grid take the same time, so a last
-// round that is only partly filled leaves the rest of the device idle.
-int64_t balanced_splits(int64_t N, int64_t chunk, int64_t base, int64_t s_min, int64_t capacity);
+// number of `chunk`-node chunks: the count in [s_min, 2 s_min] (and at most
+// s_max, 0 = one chunk per split) that minimises (rounds of `capacity`
+// concurrent workgroups) x (nodes per split), the fewest on a tie.
+// Workgroups of one grid take the same time, so a last round that is only
+// partly filled leaves the rest of the device idle.
+int64_t balanced_splits(int64_t N, int64_t chunk, int64_t base, int64_t s_min, int64_t capacity,
+                        int64_t s_max = 0);
 
 // Zero n doubles on the stream with a kernel (not a memset node, so the
 // call can be captured into a HIP graph and replayed).

@@ -19,10 +19,11 @@ void set_error(const char *fmt, ...)
     g_last_error = buf;
 }
 
-int64_t balanced_splits(int64_t N, int64_t chunk, int64_t base, int64_t s_min, int64_t capacity)
+int64_t balanced_splits(int64_t N, int64_t chunk, int64_t base, int64_t s_min, int64_t capacity, int64_t s_max)
 {
     const int64_t n = std::max<int64_t>(N, 1);
-    const int64_t s_top = cdiv(n, chunk);  // one chunk per split at most
+    int64_t s_top = cdiv(n, chunk);  // one chunk per split at most
+    if (s_max > 0) s_top = std::max<int64_t>(1, std::min(s_top, s_max));
     s_min = std::max<int64_t>(1, std::min(s_min, s_top));
     capacity = std::max<int64_t>(capacity, 1);
     int64_t best = s_min, best_cost = -1;

@@ -287,6 +287,18 @@ struct LineShape {
     size_t part_bytes;
 };
 
+// Workgroups of the (NRT, NCT) instantiation that one CU holds at once.
+static int line_blocks_per_cu(int nrt, int nct)
+{
+    if (nct == 1) return blocks_per_cu(k_signal_lines_bf16<1, 1>, 64, 0);
+    switch (nrt) {
+    case 1: return blocks_per_cu(k_signal_lines_bf16<1, 2>, 64, 0);
+    case 2: return blocks_per_cu(k_signal_lines_bf16<2, 2>, 64, 0);
+    case 3: return blocks_per_cu(k_signal_lines_bf16<3, 2>, 64, 0);
+    default: return blocks_per_cu(k_signal_lines_bf16<4, 2>, 64, 0);
+    }
+}
+
 static LineShape line_shape(const fm_signal_plan *p, int64_t N)
 {
     LineShape s{};
@@ -295,12 +307,14 @@ static LineShape line_shape(const fm_signal_plan *p, int64_t N)
     const int64_t Q = 16 * s.nct;
     s.nrt = (int)cdiv(cdiv(L, Q), 16);
     s.PQ = 16 * (int64_t)s.nrt * Q;
-    const int64_t nl = p->n_line_tabs;
+    const int64_t nl = std::max<int64_t>(p->n_line_tabs, 1);
     // ~8 workgroups per CU at least, at most 16 384 nodes per split (FP32
-    // accumulation), at least 256 nodes per split
-    int64_t splits = std::max(cdiv(8 * (int64_t)device_cu_count(), std::max<int64_t>(nl, 1)),
-                              cdiv(std::max<int64_t>(N, 1), kLineNodesPerSplit));
-    splits = std::max<int64_t>(1, std::min(splits, cdiv(std::max<int64_t>(N, 1), 256)));
+    // accumulation), at least 256 nodes per split, in whole rounds of the
+    // workgroups the device holds
+    const int64_t s_min = std::max(cdiv(8 * (int64_t)device_cu_count(), nl),
+                                   cdiv(std::max<int64_t>(N, 1), kLineNodesPerSplit));
+    const int64_t capacity = (int64_t)line_blocks_per_cu(s.nrt, s.nct) * device_cu_count();
+    const int64_t splits = balanced_splits(N, kLineNB, nl, s_min, capacity, cdiv(std::max<int64_t>(N, 1), 256));
     s.nodes_per_split = cdiv(cdiv(std::max<int64_t>(N, 1), splits), kLineNB) * kLineNB;
     s.n_splits = cdiv(std::max<int64_t>(N, 1), s.nodes_per_split);
     s.part_bytes = (size_t)s.n_splits * (size_t)nl * (size_t)s.PQ * sizeof(float2);
