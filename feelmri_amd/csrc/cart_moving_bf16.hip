@@ -33,8 +33,9 @@
 //     lh, with -Ai formed once per step.
 // Nodes are split across workgroups, at most ~16 384 per split: the FP32
 // accumulators are the dominant error term (at 131 072 nodes per split the
-// 4-D flow case is 1.9e-5 from FP64, at 16 384 4.5e-6), and k_cart_reduce
-// sums the per-split partial tiles in FP64.  The workgroups of one node
+// 4-D flow case is 1.9e-5 from FP64, at 16 384 4.5e-6, at 10 176 3.1e-6),
+// and k_cart_reduce sums the per-split partial tiles in FP64.  The split
+// count fills whole rounds of the workgroups the device holds.  The workgroups of one node
 // split are dealt onto one XCD, so the split's node data is fetched from HBM
 // about once and re-read from L2 by the readout indices.
 #include <algorithm>
