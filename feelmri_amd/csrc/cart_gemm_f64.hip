@@ -12,16 +12,31 @@ namespace fm {
 // l%16 and node 4*st + l/16 of k-step st.  Within a k-step the MFMAs are
 // issued term by term across the RB row blocks, so an accumulator is
 // updated every RB-th MFMA.  RB <= 4 keeps the kernel within 256 registers
-// (two waves per SIMD).
+// (two waves per SIMD).  The E tiles of two 16-node steps form one LDS stage
+// (double-buffered), copied by direct global -> LDS DMA a stage ahead, so
+// the workgroup synchronises once per 32 nodes and stages through no
+// registers; the B operands (Ey * Ez) come from global memory one step ahead.
+template <int RB>
+__device__ inline void stage_E64_to_lds(const double *__restrict__ src, double *dst, int wave, int lane)
+{
+    // RB row-block tiles of 4 KB: RB * 4 chunks of 1 KB (64 lanes x 16 B)
+#pragma unroll
+    for (int e = 0; e < RB; ++e) {
+        const int chunk = wave * RB + e;
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void *)(reinterpret_cast<const uint4 *>(src) + chunk * 64 + lane),
+            (__attribute__((address_space(3))) void *)(reinterpret_cast<uint4 *>(dst) + chunk * 64), 16, 0, 0);
+    }
+}
+
 template <int RB>
 __global__ __launch_bounds__(kGemmThreads) void k_cart_gemm_f64(
     const double *__restrict__ E, const double2 *__restrict__ Ey, const double2 *__restrict__ Ez, int64_t Np,
     int64_t Mp, int64_t M, int64_t NC, int64_t n_kzV, int64_t n_rt, int64_t n_ct, int64_t n_splits,
     int64_t steps_per_split, int64_t ksteps, double2 *__restrict__ part)
 {
-    constexpr int kStage = RB * 2 * 4 * 64;            // doubles per stage
-    constexpr int kPer = kStage / kGemmThreads;        // = 2 RB
-    __shared__ double sE[2][kStage];
+    constexpr int kStep = RB * 2 * 4 * 64;             // doubles of one step's E tiles
+    __shared__ double sE[2][2 * kStep];
     int64_t split, rt, ct;
     gemm_tile_ids(n_rt, n_ct, n_splits, split, rt, ct);
     const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
@@ -41,64 +56,73 @@ __global__ __launch_bounds__(kGemmThreads) void k_cart_gemm_f64(
 #pragma unroll
         for (int q = 0; q < 4; ++q) acc_re[r][q] = acc_im[r][q] = 0.0;
 
-    double pre[kPer];
+    // E tiles of steps ks and ks + 1 (the second only below k1) into stage
+    // buffer b
+    auto stage = [&](int64_t ks, int b) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            if (ks + h < k1)
+                stage_E64_to_lds<RB>(E + ((ks + h) * nrb + rt * RB) * 2 * 4 * 64, sE[b] + h * kStep, wave, lane);
+    };
+
     double2 cy[4], cz[4], ny[4], nz[4];
     if (k0 < k1) {
-        const double *src = E + (k0 * nrb + rt * RB) * 2 * 4 * 64;
-#pragma unroll
-        for (int e = 0; e < kPer; ++e) sE[0][tid + e * kGemmThreads] = src[tid + e * kGemmThreads];
+        stage(k0, 0);
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
             cy[st] = eyrow[k0 * 16 + 4 * st];
             cz[st] = ezrow[k0 * 16 + 4 * st];
         }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // LDS copies are tracked by vmcnt
     __syncthreads();
     int buf = 0;
-    for (int64_t ks = k0; ks < k1; ++ks) {
-        double br[4], bi[4];
+    for (int64_t ks0 = k0; ks0 < k1; ks0 += 2) {
+        if (ks0 + 2 < k1) stage(ks0 + 2, buf ^ 1);
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
-            br[st] = cy[st].x * cz[st].x - cy[st].y * cz[st].y;
-            bi[st] = cy[st].x * cz[st].y + cy[st].y * cz[st].x;
-        }
-        const bool has_next = ks + 1 < k1;
-        if (has_next) {
-            const double *src = E + ((ks + 1) * nrb + rt * RB) * 2 * 4 * 64;
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) pre[e] = src[tid + e * kGemmThreads];
+        for (int h = 0; h < 2; ++h) {
+            const int64_t ks = ks0 + h;
+            if (ks >= k1) break;
+            double br[4], bi[4];
 #pragma unroll
             for (int st = 0; st < 4; ++st) {
-                ny[st] = eyrow[(ks + 1) * 16 + 4 * st];
-                nz[st] = ezrow[(ks + 1) * 16 + 4 * st];
+                br[st] = cy[st].x * cz[st].x - cy[st].y * cz[st].y;
+                bi[st] = cy[st].x * cz[st].y + cy[st].y * cz[st].x;
             }
-        }
+            if (ks + 1 < k1) {
 #pragma unroll
-        for (int st = 0; st < 4; ++st) {
-            double er[RB], ei[RB];
-#pragma unroll
-            for (int r = 0; r < RB; ++r) {
-                er[r] = sE[buf][(r * 2 * 4 + 0 * 4 + st) * 64 + lane];
-                ei[r] = sE[buf][(r * 2 * 4 + 1 * 4 + st) * 64 + lane];
+                for (int st = 0; st < 4; ++st) {
+                    ny[st] = eyrow[(ks + 1) * 16 + 4 * st];
+                    nz[st] = ezrow[(ks + 1) * 16 + 4 * st];
+                }
             }
-#pragma unroll
-            for (int r = 0; r < RB; ++r) acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[r], br[st], acc_re[r], 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < RB; ++r) acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[r], bi[st], acc_im[r], 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < RB; ++r) acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ei[r], bi[st], acc_re[r], 0, 0, 0);
-#pragma unroll
-            for (int r = 0; r < RB; ++r) acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei[r], br[st], acc_im[r], 0, 0, 0);
-        }
-        if (has_next) {
-#pragma unroll
-            for (int e = 0; e < kPer; ++e) sE[buf ^ 1][tid + e * kGemmThreads] = pre[e];
+            const double *se = sE[buf] + h * kStep;
 #pragma unroll
             for (int st = 0; st < 4; ++st) {
-                cy[st] = ny[st];
-                cz[st] = nz[st];
+                double er[RB], ei[RB];
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    er[r] = se[(r * 2 * 4 + 0 * 4 + st) * 64 + lane];
+                    ei[r] = se[(r * 2 * 4 + 1 * 4 + st) * 64 + lane];
+                }
+#pragma unroll
+                for (int r = 0; r < RB; ++r) acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[r], br[st], acc_re[r], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < RB; ++r) acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(er[r], bi[st], acc_im[r], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < RB; ++r) acc_re[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ei[r], bi[st], acc_re[r], 0, 0, 0);
+#pragma unroll
+                for (int r = 0; r < RB; ++r) acc_im[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei[r], br[st], acc_im[r], 0, 0, 0);
+            }
+            if (ks + 1 < k1) {
+#pragma unroll
+                for (int st = 0; st < 4; ++st) {
+                    cy[st] = ny[st];
+                    cz[st] = nz[st];
+                }
             }
         }
+        __builtin_amdgcn_s_waitcnt(0);  // the next stage has landed
         __syncthreads();
         buf ^= 1;
     }
