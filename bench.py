@@ -406,12 +406,42 @@ def run_flow(args, dev) -> dict:
         ev[2].record()
         torch.cuda.synchronize()
         scan_ms = (ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]))
+    # the mixed form's acquisition: the same blocks with FP32 Bloch rotations
+    # (timed like the FP64 one; its Mxy feeds the mixed end-to-end scan)
+    solver32 = BlochSolver(blocks, m.nodes, tis.T1, tis.T2, delta_B=tis.delta_B_mT, pod_trajectory=wl.velocity,
+                           precision="f32", dev=dev)
+    m32xy0, m32z0 = solver32.mxy.clone(), solver32.mz.clone()
+    bloch32_ms = None
+    for rep in range(2):
+        solver32.mxy.copy_(m32xy0)
+        solver32.mz.copy_(m32z0)
+        torch.cuda.synchronize()
+        ev[1].record()
+        mxy32_store, _ = solver32.solve_device()
+        ev[2].record()
+        torch.cuda.synchronize()
+        bloch32_ms = ev[1].elapsed_time(ev[2])
+    del solver32
+    bloch32_err = float(torch.linalg.vector_norm(mxy32_store - mxy_store) / torch.linalg.vector_norm(mxy_store))
+    log(f"FP32 moving-node acquisition {bloch32_ms:.1f} ms, Mxy relL2 {bloch32_err:.2e} vs FP64")
     phases = []
     for p in range(n_ph):
         mxy = mxy_store[:, 4 * p:4 * p + 4].to(torch.complex128) * M0[:, None]
         nodes = device.SignalNodes.from_arrays(nodes_d, phi, T2s, mxy, weights=w, device=dev)
         phases.append((nodes, coefs[p], mxy))
     torch.cuda.synchronize()
+    # packing every phase as a scan does it, one phase at a time into reused
+    # buffers (the second pass is timed; the first one fills the allocator)
+    pack_ms = None
+    for rep in range(2):
+        torch.cuda.synchronize()
+        t_pack = time.perf_counter()
+        for p in range(n_ph):
+            mxy_p = mxy_store[:, 4 * p:4 * p + 4].to(torch.complex128) * M0[:, None]
+            nodes_p = device.SignalNodes.from_arrays(nodes_d, phi, T2s, mxy_p, weights=w, device=dev)
+            del nodes_p, mxy_p
+        torch.cuda.synchronize()
+        pack_ms = (time.perf_counter() - t_pack) * 1e3
     work = m.n_nodes * wl.n_samples * 4 * n_ph
     log(f"set-up: lumped mass {scan_ms[0]:.3f} ms, moving-node Bloch acquisition ({len(blocks)} blocks) "
         f"{scan_ms[1]:.1f} ms")
@@ -442,7 +472,7 @@ def run_flow(args, dev) -> dict:
     value = work / (ms * 1e-3)
     log(f"{args.form}: {ms:.1f} ms/step ({n_ph} phases), {value:.3e} node*k/s, dominant kernel {k_ms:.2f} ms/phase")
     forms = {args.form: {"ms_per_step": ms, "node_ksamples_per_s": value, "kernel_ms_per_phase": k_ms}}
-    err64 = err64_img = None
+    err64 = err64_img = scan_err = None
     out0 = outs[0].cpu().numpy()
     o64_0 = None
     if not args.no_f64 and args.form != "f64":
@@ -464,6 +494,20 @@ def run_flow(args, dev) -> dict:
         err64 = (num / den) ** 0.5
         err64_img = (inum / iden) ** 0.5
         o64_0 = o64[0].cpu().numpy()
+        # the mixed scan end to end (FP32 Bloch Mxy, mixed signal) against
+        # the FP64 scan (FP64 Bloch Mxy, FP64 signal), all phases
+        num = 0.0
+        ws32 = torch.empty(max(plan.workspace_bytes(args.form, m.n_nodes, 4, moving=True), 1), dtype=torch.uint8,
+                           device=dev)
+        o32 = torch.empty((plan.n_samples, 4), dtype=torch.complex128, device=dev)
+        for p, b in enumerate(o64):
+            mxy = mxy32_store[:, 4 * p:4 * p + 4].to(torch.complex128) * M0[:, None]
+            nd32 = device.SignalNodes.from_arrays(nodes_d, phi, T2s, mxy, weights=w, device=dev)
+            plan.execute(nd32, precision=args.form, motion=(modes, coefs[p]), out=o32, workspace=ws32)
+            num += float(torch.linalg.vector_norm(o32 - b)) ** 2
+        scan_err = (num / den) ** 0.5
+        del ws32, o32, nd32
+        log(f"mixed scan (FP32 Bloch + {args.form} signal) vs FP64 scan: relL2 {scan_err:.2e}")
         del o64
         log(f"f64: {ms64:.1f} ms/step; {args.form} vs f64 relL2 {err64:.2e} over all phases (image "
             f"{err64_img:.2e}); speed-up {ms64 / ms:.2f}x")
@@ -534,8 +578,23 @@ def run_flow(args, dev) -> dict:
         "relL2_vs_cpu_baseline_readouts": err_cpu,
         "forms": forms, "roofline": roof, "cpu_baseline": cpu,
         "setup_ms": {"lumped_mass": scan_ms[0], "bloch_moving_acquisition": scan_ms[1],
-                     "bloch_blocks": len(blocks)},
+                     "bloch_moving_acquisition_f32": bloch32_ms, "bloch_blocks": len(blocks),
+                     "pack_all_phases": pack_ms},
+        # end to end: weights, the moving-node Bloch acquisition, the packed
+        # nodes of every phase and the k-space of every phase (FP32 Bloch in
+        # the mixed form, FP64 in the FP64 form)
+        "scan": scan_entry(args.form, scan_ms[0], bloch32_ms if args.form != "f64" else scan_ms[1], pack_ms, ms,
+                           work, scan_err, bloch32_err),
     }
+
+
+def scan_entry(form, mass_ms, bloch_ms, pack_ms, signal_ms, work, err_vs_f64_scan, bloch_err) -> dict:
+    total = mass_ms + bloch_ms + pack_ms + signal_ms
+    return {"ms": total, "node_ksamples_per_s": work / (total * 1e-3),
+            "parts_ms": {"lumped_mass": mass_ms, "bloch": bloch_ms, "pack": pack_ms, "signal": signal_ms},
+            "bloch_precision": "f64" if form == "f64" else "f32",
+            "relL2_vs_f64_scan": err_vs_f64_scan if form != "f64" else None,
+            "bloch_mxy_relL2_f32_vs_f64": bloch_err}
 
 
 def measure_static(args, dev, wl, form: str, with_cpu: bool) -> dict:
