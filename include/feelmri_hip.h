@@ -112,7 +112,9 @@ int fm_signal_plan_runs(const fm_signal_plan *plan, int64_t *n_runs, int *run_le
  * introspection). */
 int fm_signal_plan_lines(const fm_signal_plan *plan, int64_t *n_lines, int *line_gemm);
 /* Workspace (device bytes) fm_signal_execute needs for this plan, with
- * static nodes (moving = 0) or moving nodes (moving = 1: modes given). */
+ * static nodes (moving = 0) or moving nodes (moving = 1: modes given).  The
+ * GEMM paths size their node splits by the current device's occupancy, so
+ * query on the device that will execute. */
 int fm_signal_workspace_size(const fm_signal_plan *plan, int precision, int64_t n_nodes,
                              int n_enc, int moving, size_t *bytes);
 
