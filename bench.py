@@ -792,6 +792,26 @@ def row_benchmarks(wl, dev) -> dict:
                          "per_unit": f"80 flops + sqrt, rcp, sin, cos per node*event; {n_rf} of {n_ev} steps carry "
                                      "RF; the spoiled-GRE scan chain (16 dummy TRs + 1) as one chained launch"}}
         del solver
+    # event tables on the device (fm_events_build) against the host grid rule
+    # (make_block, as the reference's SequenceBlock discretises) for the
+    # same block descriptions: the scan's spoiled-GRE blocks
+    from feelmri_amd import sequences
+    desc, pulses, traps, tabs = sequences.block_descriptions(blocks)
+    de = device.DeviceEvents(desc, pulses, traps, rf_tables=tabs, device=dev)
+    ms_dev = _time_cuda(lambda: de.build(), 20)
+    de.check()
+    n_built = int(de.offsets[-1].item())
+    specs = [b.info["spec"] for b in blocks]
+    t0 = time.perf_counter()
+    for _ in range(5):
+        for sp in specs:
+            sequences.make_block(**sp)
+    ms_host = (time.perf_counter() - t0) * 1e3 / 5
+    rows["events_build"] = {"blocks": len(blocks), "events": n_built, "ms": ms_dev, "host_rule_ms": ms_host,
+                            "max_candidates": de.max_candidates,
+                            "note": "fm_events_build (three launches: candidates sorted per block in LDS, "
+                                    "offsets scanned, tables written) for the scan's blocks, against the "
+                                    "host grid rule on one core for the same specifications"}
     Th = 200
     evh = device.EventTable(rf[:Th], G[:Th], np.full(Th, 0.01), device=dev)
     ms = _time_cuda(lambda: device.bloch(bn, evh, 1.0, GAMMA, mxy0, mz0, precision="f32", history=True), 3)
@@ -920,7 +940,8 @@ def main() -> int:
                 "data": "synthetic", **sec}
     if not args.no_rows and aux_wl is not None:
         rows = row_benchmarks(aux_wl, dev)
-        log("rows: " + ", ".join(f"{k} {v['ms']:.3f} ms ({v['roofline']['frac']:.2f} of {v['roofline']['bound']})"
+        log("rows: " + ", ".join(f"{k} {v['ms']:.3f} ms" + (f" ({v['roofline']['frac']:.2f} of {v['roofline']['bound']})"
+                                                             if "roofline" in v else "")
                                   for k, v in rows.items()))
         line["rows"] = rows
     if args.serve_requests > 0:
