@@ -838,6 +838,30 @@ def row_benchmarks(wl, dev) -> dict:
                                            "per_unit": "compulsory bytes: 16 B per element (4 x int32 connectivity) "
                                                        "+ 24 B per node coordinates read + 8 B per node weight "
                                                        "written"}}
+    # consistent P1 mass: element matrices on the device, the CSR assembled
+    # once on the host (scipy), then M . Mxy as a device SpMV
+    import scipy.sparse as sp
+    elems_d = torch.from_numpy(np.ascontiguousarray(m.elems, dtype=np.int32)).to(dev)
+    ms_em = _time_cuda(lambda: device.element_matrices(elems_d, nodes_d), 5)
+    vals = device.element_matrices(elems_d, nodes_d).cpu().numpy()
+    el = np.ascontiguousarray(m.elems, dtype=np.int64)
+    Mc = sp.csr_matrix((vals.ravel(), (np.repeat(el, 4, axis=1).ravel(), np.tile(el, (1, 4)).ravel())), shape=(N, N))
+    ip = torch.from_numpy(Mc.indptr.astype(np.int64)).to(dev)
+    ix = torch.from_numpy(Mc.indices.astype(np.int64)).to(dev)
+    dv = torch.from_numpy(Mc.data).to(dev)
+    xv = torch.randn(N, 1, dtype=torch.complex128, device=dev)
+    ms_spmv = _time_cuda(lambda: device.csr_spmv(ip, ix, dv, xv), 10)
+    by_em = E * (16 + 128) + N * 24
+    by_spmv = Mc.nnz * 16 + (N + 1) * 8 + N * 32
+    rows["consistent_mass_p1"] = {
+        "elements": E, "nnz": int(Mc.nnz), "ms": ms_em + ms_spmv, "element_matrices_ms": ms_em, "spmv_ms": ms_spmv,
+        "roofline": {"bound": "hbm", "achieved": by_spmv / (ms_spmv * 1e-3) / 1e9, "peak": PEAKS["hbm_gbs"],
+                     "unit": "GB/s", "frac": by_spmv / (ms_spmv * 1e-3) / 1e9 / PEAKS["hbm_gbs"],
+                     "element_matrices_gbs": by_em / (ms_em * 1e-3) / 1e9,
+                     "per_unit": "SpMV: 16 B per stored entry (value + column) + 8 B per row pointer + 32 B per "
+                                 "node (complex x read, y written); element matrices: 16 B connectivity + 128 B "
+                                 "written per element + 24 B per node coordinates"}}
+    del Mc, ip, ix, dv, xv, vals
     # POD contraction at 2 M nodes, 10 modes (float32)
     P, R = 2_000_000, 10
     modes = torch.randn((P, 3, R), dtype=torch.float32, device=dev)

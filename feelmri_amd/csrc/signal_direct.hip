@@ -711,23 +711,50 @@ __global__ void k_nodes_pack(int64_t N, int V, const T *__restrict__ r0, const T
     }
 }
 
-// CSR SpMV, one thread per (row, encoding); the row's terms are added in
-// column-index order.
-__global__ void k_csr_spmv_c128(int64_t n_rows, int V, const int64_t *__restrict__ indptr,
-                                const int64_t *__restrict__ indices, const double *__restrict__ data,
-                                const double2 *__restrict__ x, double2 *__restrict__ y)
+// CSR SpMV, G lanes per row (G a power of two, at most 64): lane j of a
+// row's group adds the row's terms j, j + G, ... for up to 4 encodings at a
+// time (the row's values and column indices read once per pass, coalesced
+// across the group), then the group sums its lanes in a fixed butterfly, so
+// a result does not depend on the launch.  Groups past the last row work on
+// the last row and write nothing.
+template <int G>
+__global__ __launch_bounds__(256) void k_csr_spmv_c128(int64_t n_rows, int V, const int64_t *__restrict__ indptr,
+                                                        const int64_t *__restrict__ indices,
+                                                        const double *__restrict__ data,
+                                                        const double2 *__restrict__ x, double2 *__restrict__ y)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_rows * V) return;
-    const int64_t r = i / V, v = i % V;
-    double re = 0.0, im = 0.0;
-    for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
-        const double a = data[k];
-        const double2 xv = x[indices[k] * V + v];
-        re += a * xv.x;
-        im += a * xv.y;
+    const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int j = threadIdx.x % G;
+    const bool live = g < n_rows;
+    const int64_t r = live ? g : n_rows - 1;
+    const int64_t k0 = indptr[r], k1 = indptr[r + 1];
+    for (int v0 = 0; v0 < V; v0 += 4) {
+        double re[4] = {0.0, 0.0, 0.0, 0.0}, im[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int64_t k = k0 + j; k < k1; k += G) {
+            const double a = data[k];
+            const double2 *xr = x + indices[k] * V + v0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (v0 + u < V) {
+                    const double2 xv = xr[u];
+                    re[u] = fma(a, xv.x, re[u]);
+                    im[u] = fma(a, xv.y, im[u]);
+                }
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                re[u] += __shfl_xor(re[u], off, G);
+                im[u] += __shfl_xor(im[u], off, G);
+            }
+        if (live && j == 0) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (v0 + u < V) y[r * V + v0 + u] = make_double2(re[u], im[u]);
+        }
     }
-    y[i] = make_double2(re, im);
 }
 
 // ---------------------------------------------------------------------------
@@ -1553,8 +1580,12 @@ extern "C" int fm_csr_spmv_c128(int64_t n_rows, int n_enc, const int64_t *indptr
     FM_CHECK_ARG(n_rows >= 0 && n_enc >= 1, "bad sizes");
     if (n_rows == 0) return FM_OK;
     FM_CHECK_ARG(indptr && indices && data && x && y, "NULL array");
-    const int64_t total = n_rows * n_enc;
-    k_csr_spmv_c128<<<(unsigned)cdiv(total, 256), 256, 0, as_stream(stream)>>>(
+    // 16 lanes per row: P1 mass rows hold ~15 entries on tetrahedral meshes
+    // (longer rows take more passes)
+    constexpr int G = 16;
+    const int64_t threads = n_rows * G;
+    FM_CHECK_ARG(cdiv(threads, 256) <= 0x7fffffff, "too many rows for one launch");
+    k_csr_spmv_c128<G><<<(unsigned)cdiv(threads, 256), 256, 0, as_stream(stream)>>>(
         n_rows, n_enc, indptr, indices, data, reinterpret_cast<const double2 *>(x), reinterpret_cast<double2 *>(y));
     FM_LAUNCH_CHECK("k_csr_spmv_c128");
     return FM_OK;

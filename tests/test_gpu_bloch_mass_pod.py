@@ -400,6 +400,35 @@ def test_consistent_mass_assembly(gpu, dtype):
     assert diff <= (1e-12 if dtype == np.float64 else 1e-6) * abs(ref).max()
 
 
+# CSR SpMV (16 lanes per row): empty rows, rows longer than the lane group,
+# more than 4 encodings (two passes), a row count that ends in a partial
+# workgroup; FP64 against scipy, and the same bits on a second call
+@pytest.mark.parametrize("n_rows,V", [(1003, 1), (257, 3), (64, 5), (1, 2)])
+def test_csr_spmv_against_scipy(gpu, n_rows, V):
+    import scipy.sparse as sp
+    import torch
+    from feelmri_amd import device
+
+    rng = np.random.default_rng(n_rows + V)
+    lengths = rng.integers(0, 40, n_rows)
+    lengths[:: 7] = 0
+    lengths[3 % n_rows] = 150
+    rows = np.repeat(np.arange(n_rows), lengths)
+    cols = rng.integers(0, n_rows, rows.size)
+    M = sp.csr_matrix((rng.normal(size=rows.size), (rows, cols)), shape=(n_rows, n_rows))
+    M.sort_indices()
+    x = rng.normal(size=(n_rows, V)) + 1j * rng.normal(size=(n_rows, V))
+    xd = torch.from_numpy(x).to(gpu)
+    y = device.csr_spmv(M.indptr, M.indices, M.data, xd)
+    y2 = device.csr_spmv(M.indptr, M.indices, M.data, xd)
+    want = M @ x
+    got = y.cpu().numpy()
+    assert got.shape == want.shape
+    scale = np.abs(M).sum(axis=1).A.ravel()[:, None] * np.abs(x).max() + 1e-300
+    assert np.max(np.abs(got - want) / scale) <= 1e-14
+    assert torch.equal(y, y2)
+
+
 # M <= 32 modes take the LDS-tiled kernel, more modes the per-row one; row
 # counts that are not a multiple of the 256-row tile end in a partial tile
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
